@@ -1,0 +1,171 @@
+"""Benchmark: fluid particle-steps/s of the plain WCSPH step on one MI355X.
+
+Workload: the 2D dam break refined to dr = 0.001 m (1000 x 2000 = 2.0 M fluid
+particles, 3 layers of fixed wall particles), Wendland C2 with h = 2 dr, Tait
+rho0 = 1000, xi = 7, cs0 = 10 sqrt(2 g H), artificial viscosity alpha = 0.02,
+delta-SPH delta = 0.1, SSPRK3 with CFL 0.25; FP32 pair sums with Kahan
+accumulation, FP64 positions/velocities/densities and integrator.
+
+One "step" is one SSPRK3 step of every particle (cell rebuild, three stages
+of boundary extrapolation + fused pair sums/update, CFL selection).  Inputs
+are resident in HBM before the timed region starts.
+
+Prints one JSON line (see the module's argument parser for flags).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+# Dam-break workload (tank 3.22 x 2.4 m, open top; column 1.0 x 2.0 m).
+TANK = [3.22, 2.4]
+COLUMN = [1.0, 2.0]
+
+
+def stage_bytes_per_particle(dim: int) -> float:
+    """Compulsory HBM bytes per fluid particle of one fused stage launch,
+    averaged over the three SSPRK3 stages: read r, v, rho (FP64) of the stage
+    input; stages 2 and 3 also read the step-start r, v, rho; write r, v, rho.
+    """
+    state = 8 * (2 * dim + 1)
+    return (2 * state + 3 * state + 3 * state) / 3.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dr", type=float, default=0.001)
+    ap.add_argument("--profile-steps", type=int, default=60,
+                    help="eager steps with per-launch HIP events after the "
+                         "timed region (kernel durations)")
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="oracle steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    import titsolver_amd as ta
+
+    dr = args.dr
+    print(f"bench: building 2D dam break dr={dr}", file=sys.stderr, flush=True)
+    pa = ta.dam_break(2, dr, TANK, COLUMN)
+    eq = ta.cfg_equations(2, dr, COLUMN[1])
+    nf = pa.n_fluid
+    ctx = ta.HipContext(eq, order=3)
+    ctx.upload([pa])
+    print(f"bench: {nf} fluid + {pa.n_fixed} fixed particles uploaded; "
+          f"warmup {args.warmup}", file=sys.stderr, flush=True)
+    ctx.step(args.warmup)
+
+    # Timed region: K steps replayed from the captured two-step hipGraph;
+    # step() returns after the context's stream has drained.
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = nf * args.steps / wall
+    print(f"bench: {args.steps} steps in {wall:.3f} s ({ms_per_step:.4f} ms/step)",
+          file=sys.stderr, flush=True)
+
+    # Kernel durations: eager launches with an event pair around each.
+    ctx.set_profiling(True)
+    ctx.step(args.profile_steps)
+    stats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    kernels = {k: {"launches": n, "avg_us": 1e3 * t / max(n, 1)}
+               for k, (n, t) in stats.items()}
+    st_launches, st_ms = stats["stage"]
+    avg_stage_s = st_ms / st_launches * 1e-3
+    bytes_launch = stage_bytes_per_particle(2) * nf
+    achieved = bytes_launch / avg_stage_s / 1e9
+    t_step, dt_next = ctx.tank_clock()
+
+    out = {
+        "metric": "fluid particle-steps/s on 1 MI355X: plain WCSPH vs learned "
+                  "step at <=1% density err",
+        "value": value,
+        "unit": "fluid particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state/integrator, f32 pair sums (Kahan)",
+        "data": "synthetic (lattice dam break, hydrostatic start)",
+        "config": {"workload": f"2D dam break dr={dr} m, {nf} fluid + "
+                               f"{pa.n_fixed} fixed particles, Wendland C2 h=2dr, "
+                               "delta-SPH 0.1, alpha 0.02, SSPRK3 CFL 0.25",
+                   "fluid_particles": nf, "fixed_particles": pa.n_fixed,
+                   "sim_time_s": float(t_step[0]), "dt_next_s": float(dt_next[0])},
+        "roofline": {"bound": "hbm", "kernel": "k_stage (fused pair sums + SSPRK update)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_launch,
+                     "avg_launch_us": avg_stage_s * 1e6},
+        "kernels": kernels,
+    }
+    ctx.close()
+
+    if not args.no_cpu and args.cpu_steps > 0:
+        out["cpu_baseline"], out["parity"] = cpu_baseline(pa, eq, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(pa, eq, steps):
+    """FP64 oracle (single thread) on the same workload for a few steps, and
+    the GPU's error against it after the same steps from the same state."""
+    import oracle
+    import titsolver_amd as ta
+
+    nf = pa.n_fluid
+    print(f"bench: CPU oracle {steps} steps on {pa.n} particles", file=sys.stderr,
+          flush=True)
+    prm = oracle.params(eq)
+    st = oracle.State.from_particles(pa)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        oracle.ssprk_step(prm, st, 3)
+    wall = time.perf_counter() - t0
+    cpu = {"value": nf * steps / wall, "unit": "fluid particle-steps/s", "cores": 1,
+           "kind": "port",
+           "sample": f"{steps} SSPRK3 steps of the same {nf}-fluid-particle dam break "
+                     f"(FP64 oracle, grid search per stage), {wall:.1f} s"}
+    ctx = ta.HipContext(eq, order=3)
+    ctx.upload([pa])
+    ctx.step(steps)
+    g = ctx.download()
+    ctx.close()
+    rho_err = float(np.max(np.abs(g["rho"][:nf] - st.rho[:nf]) / st.rho[:nf]))
+    p_err = float(np.max(np.abs(g["p"][:nf] - st.p[:nf])) / np.abs(st.p[:nf]).max())
+    front_g = float(g["r"][:nf, 0].max())
+    front_o = float(st.r[:nf, 0].max())
+    parity = {"steps": steps, "rho_max_rel_err": rho_err,
+              "p_max_err_over_pmax": p_err,
+              "front_x_rel_err": abs(front_g - front_o) / front_o}
+    return cpu, parity
+
+
+if __name__ == "__main__":
+    main()

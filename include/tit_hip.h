@@ -1,0 +1,180 @@
+/*
+ * tit_hip.h -- C ABI of the MI355X (gfx950) WCSPH step.
+ *
+ * The functions below are the drop-in boundary for TitSolver's WCSPH step.
+ * Plain C types only: no exceptions, no C++ or torch types cross it.  Every
+ * entry point that can fail returns a tit_status; tit_hip_last_error() gives
+ * the message of the last failure on a context.
+ *
+ * Threading: one context owns one HIP stream and its device buffers; calls on
+ * one context must not overlap.  Several contexts may coexist.
+ * Ownership: the caller owns every host array it passes; the library copies
+ * what it needs and keeps no host pointer after a call returns.
+ * Layout: host arrays use the reference layout -- component k of particle i at
+ * [i * dim + k], fluid particles first, then fixed ones, tank after tank
+ * (source/tit/sph/particle_array.hpp:204-239).
+ */
+#ifndef TIT_HIP_H
+#define TIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tit_hip_ctx tit_hip_ctx;
+
+typedef enum tit_status {
+  TIT_OK = 0,
+  TIT_E_ARG = 1,     /* invalid argument or unsupported configuration */
+  TIT_E_HIP = 2,     /* HIP runtime error (no device, launch failure, ...) */
+  TIT_E_OOM = 3,     /* device allocation failed */
+  TIT_E_NUMERIC = 4, /* non-finite state or neighbour-skin overrun */
+  TIT_E_STATE = 5    /* call out of order (e.g. step before upload) */
+} tit_status;
+
+/* Smoothing kernels: QuarticWendlandKernel (C2) and SixthOrderWendlandKernel
+ * (C4), source/tit/sph/kernel.hpp:447-451. */
+enum { TIT_KERNEL_WENDLAND_C2 = 0, TIT_KERNEL_WENDLAND_C4 = 1 };
+
+/* Term sets.  TIT_FLAVOUR_REF: the reference's Ferrari density diffusion and
+ * physical viscosity mu (source/tit/sph/fluid_equations.hpp:249-304).
+ * TIT_FLAVOUR_CFG: delta-SPH density diffusion (Molteni & Colagrossi 2009)
+ * and Monaghan (1992) artificial viscosity alpha. */
+enum { TIT_FLAVOUR_REF = 0, TIT_FLAVOUR_CFG = 1 };
+
+/* Constructor arguments of FluidEquations + TaitEquationOfState + kernel
+ * (source/tit/sph/fluid_equations.hpp:61-69, equation_of_state.hpp:27-34,
+ * titwcsph/wcsph.cpp:79-90) plus the time-step constants that the reference
+ * keeps private (fluid_equations.hpp:518-520). */
+typedef struct tit_hip_params {
+  int dim;          /* 2 or 3 */
+  int kernel;       /* TIT_KERNEL_* */
+  int flavour;      /* TIT_FLAVOUR_* */
+  int ssprk_order;  /* 2 or 3 (SSPRKOrder, time_integrator.hpp:134-137) */
+  double h;         /* kernel width; support radius is 2h */
+  double m;         /* particle mass (uniform) */
+  double rho0, cs0, xi; /* Tait EOS */
+  double g;         /* gravity magnitude along -axis 1 */
+  double mu;        /* physical viscosity (REF flavour) */
+  double alpha;     /* artificial viscosity (CFG flavour) */
+  double delta;     /* delta-SPH coefficient (CFG flavour) */
+  double cfl, c_force, c_visc; /* time-step coefficients */
+  double skin;      /* neighbour-cell skin in units of the support radius;
+                       0 selects the default 0.1 */
+} tit_hip_params;
+
+/* One tank: the box its cell grid covers.  Particles that leave the box are
+ * clamped into the border cells (neighbour sets stay exact). */
+typedef struct tit_hip_tank {
+  double lo[3];
+  double hi[3];
+} tit_hip_tank;
+
+/* Host view of one tank's ParticleArray fields (reference layout). */
+typedef struct tit_hip_host_view {
+  const double* r;      /* (n_fluid + n_fixed) * dim */
+  const double* v;      /* (n_fluid + n_fixed) * dim, may be NULL (zero) */
+  const double* rho;    /* n_fluid + n_fixed */
+  const double* dv_dt;  /* (n_fluid + n_fixed) * dim, may be NULL (zero):
+                           the previous momentum RHS, read by the first
+                           time-step selection only */
+  const double* normal; /* n_fixed * dim wall normals of the fixed particles */
+  size_t n_fluid;
+  size_t n_fixed;
+} tit_hip_host_view;
+
+/* Creates a context on HIP device `device`. */
+tit_status tit_hip_create(int device, const tit_hip_params* params,
+                          tit_hip_ctx** out);
+
+/* Copies one tank's particles to the device (replaces any previous state).
+ * Replaces: the ParticleArray field vectors that SSPRKIntegrator::step
+ * reads (source/tit/sph/particle_array.hpp:242-276). */
+tit_status tit_hip_upload(tit_hip_ctx* ctx, const tit_hip_host_view* view,
+                          const tit_hip_tank* tank);
+
+/* Copies n_tanks independent tanks; all later calls act on every tank, each
+ * with its own time step.  No reference counterpart (the reference runs one
+ * tank per process, titwcsph/wcsph.cpp:35-196). */
+tit_status tit_hip_upload_batch(tit_hip_ctx* ctx,
+                                const tit_hip_host_view* views,
+                                const tit_hip_tank* tanks, size_t n_tanks);
+
+/* Advances every tank by n_steps SSPRK steps and returns the last step size
+ * of tank 0.  Replaces: SSPRKIntegrator::step (time_integrator.hpp:159-184),
+ * without post_integrate; called as in titwcsph/wcsph.cpp:181. */
+tit_status tit_hip_step(tit_hip_ctx* ctx, int n_steps, double* dt_last);
+
+/* Copies the state back in reference order.  Any pointer may be NULL.
+ * p is the Tait pressure of rho (FluidEquations fills p in compute_momentum,
+ * fluid_equations.hpp:272-273). */
+tit_status tit_hip_download(tit_hip_ctx* ctx, double* r, double* v,
+                            double* rho, double* p);
+
+/* Per-tank simulated time and the step size the next step will use. */
+tit_status tit_hip_tank_clock(tit_hip_ctx* ctx, double* time, double* dt_next);
+
+/* FluidEquations::prepare without the wall terms: neighbour index rebuild and
+ * setup_boundary (fluid_equations.hpp:99-105, :122-164). */
+tit_status tit_hip_prepare(tit_hip_ctx* ctx);
+
+/* FluidEquations::compute_time_step (fluid_equations.hpp:199-222) on the
+ * current state and the last momentum RHS; one value per tank. */
+tit_status tit_hip_compute_time_step(tit_hip_ctx* ctx, double* dt);
+
+/* FluidEquations::compute_continuity + compute_momentum
+ * (fluid_equations.hpp:232-305) on the current state; results in reference
+ * order (fixed particles get zero).  Does not advance the state. */
+tit_status tit_hip_compute_rhs(tit_hip_ctx* ctx, double* drho_dt,
+                               double* dv_dt);
+
+/* ParticleMesh adjacency (particle_mesh.hpp:123-147): per particle, the
+ * ascending list of neighbour indices (reference order, self included) with
+ * norm2(r_b - r_a) <= (2h)^2.  off has n+1 entries; idx has cap entries.
+ * *total receives the number of entries; TIT_E_ARG if cap is too small. */
+tit_status tit_hip_neighbors(tit_hip_ctx* ctx, uint64_t* off, uint32_t* idx,
+                             size_t cap, size_t* total);
+
+/* Kernel timing with HIP events on the context's stream.  While enabled,
+ * steps run eagerly (no graph replay) with an event pair around every launch.
+ * Mirrors TIT_PROFILE_SECTION (source/tit/core/profiler.hpp:42-44). */
+typedef struct tit_hip_kernel_stat {
+  char name[48];
+  uint64_t launches;
+  double total_ms;
+} tit_hip_kernel_stat;
+tit_status tit_hip_set_profiling(tit_hip_ctx* ctx, int enable);
+tit_status tit_hip_kernel_stats(tit_hip_ctx* ctx, tit_hip_kernel_stat* out,
+                                int cap, int* n_out);
+
+/* Number of particles (all tanks) and of neighbour-index rebuilds so far. */
+tit_status tit_hip_counts(tit_hip_ctx* ctx, size_t* n_particles,
+                          uint64_t* n_rebuilds);
+
+const char* tit_hip_last_error(const tit_hip_ctx* ctx);
+void tit_hip_destroy(tit_hip_ctx* ctx);
+
+/* ---- host-side scene set-up (no device needed) ---------------------------
+ * 2D/3D dam break on a lattice as titwcsph builds it (wcsph.cpp:104-142):
+ * fluid at dr*(i+1, j+1[, k+1]) for a column of column[0] x column[1]
+ * (x column[2]), hydrostatic density from the reference's Fourier series,
+ * and `layers` layers of fixed particles at offsets 0, -dr, -2dr, ... outside
+ * the tank walls x = 0, x = tank[0], y = 0 (and z = 0, z = tank[2] in 3D),
+ * walls rising to tank[1].  Returns the particle counts; the fill call writes
+ * r, rho and the fixed particles' wall normals. */
+tit_status tit_dam_break_count(int dim, double dr, const double* tank,
+                               const double* column, int layers,
+                               size_t* n_fluid, size_t* n_fixed);
+tit_status tit_dam_break_fill(int dim, double dr, const double* tank,
+                              const double* column, int layers, double rho0,
+                              double g, double cs0, double* r, double* rho,
+                              double* normal);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TIT_HIP_H */

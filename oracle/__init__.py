@@ -1,0 +1,218 @@
+"""ctypes wrapper of the FP64 CPU oracle (tit_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker, and by bench.py's CPU-baseline leg.  The product library never
+uses it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libtit_oracle.so"
+
+KERNEL_WENDLAND_C2, KERNEL_WENDLAND_C4 = 0, 1
+FLAVOUR_REF, FLAVOUR_CFG = 0, 1
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("dim", C.c_int), ("kernel", C.c_int), ("flavour", C.c_int),
+                ("reserved", C.c_int)] + [
+        (n, C.c_double) for n in ("h", "m", "rho0", "cs0", "xi", "g", "mu",
+                                  "alpha", "delta", "cfl", "c_force", "c_visc")]
+
+
+_DP = C.POINTER(C.c_double)
+
+
+class OrcState(C.Structure):
+    _fields_ = [("n_fluid", C.c_size_t), ("n_fixed", C.c_size_t),
+                ("r", _DP), ("v", _DP), ("rho", _DP), ("drho_dt", _DP),
+                ("dv_dt", _DP), ("p", _DP), ("cs", _DP), ("normal", _DP)]
+
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.fspath(HERE)], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} missing; run `make -C oracle`")
+    lib = C.CDLL(os.fspath(LIB_PATH))
+    P = C.POINTER(OrcParams)
+    S = C.POINTER(OrcState)
+    U64 = C.POINTER(C.c_uint64)
+    U32 = C.POINTER(C.c_uint32)
+    sig = {
+        "orc_kernel_weight": (C.c_double, [C.c_int, C.c_int]),
+        "orc_kernel_unit_value": (C.c_double, [C.c_int, C.c_double]),
+        "orc_kernel_unit_deriv": (C.c_double, [C.c_int, C.c_double]),
+        "orc_kernel_radius": (C.c_double, [C.c_int, C.c_double]),
+        "orc_kernel_value": (C.c_double, [C.c_int, C.c_int, _DP, C.c_double]),
+        "orc_kernel_grad": (None, [C.c_int, C.c_int, _DP, C.c_double, _DP]),
+        "orc_eos_pressure": (C.c_double, [P, C.c_double]),
+        "orc_eos_density_from_pressure": (C.c_double, [P, C.c_double]),
+        "orc_eos_sound_speed": (C.c_double, [P, C.c_double]),
+        "orc_eos_potential": (C.c_double, [P, C.c_double]),
+        "orc_eos_density_from_potential": (C.c_double, [P, C.c_double]),
+        "orc_grid_search": (C.c_longlong, [C.c_int, C.c_size_t, _DP, C.c_double,
+                                           C.c_double, U64, U32, C.c_size_t]),
+        "orc_naive_search": (C.c_longlong, [C.c_int, C.c_size_t, _DP, C.c_double,
+                                            C.c_int, U64, U32, C.c_size_t]),
+        "orc_setup_boundary": (C.c_int, [P, S, U64, U32]),
+        "orc_compute_continuity": (C.c_int, [P, S, U64, U32]),
+        "orc_compute_momentum": (C.c_int, [P, S, U64, U32]),
+        "orc_compute_time_step": (C.c_double, [P, S]),
+        "orc_ssprk_step": (C.c_int, [P, S, C.c_int, _DP]),
+        "orc_ssprk_generic": (C.c_int, [C.c_int, C.c_double, C.c_size_t, _DP,
+                                        C.c_void_p, C.c_void_p]),
+        "orc_set_mutation": (None, [C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], a.dtype
+    return a.ctypes.data_as(_DP)
+
+
+def params(eq) -> OrcParams:
+    """OrcParams from a titsolver_amd.FluidEquations (same field names)."""
+    p = OrcParams()
+    p.dim, p.kernel, p.flavour = eq.dim, eq.kernel, eq.flavour
+    for n in ("h", "m", "rho0", "cs0", "xi", "g", "mu", "alpha", "delta", "cfl",
+              "c_force", "c_visc"):
+        setattr(p, n, float(getattr(eq, n)))
+    return p
+
+
+def set_mutation(m: int):
+    load().orc_set_mutation(int(m))
+
+
+def _search(fn, *args, n, dim, pts):
+    lib = load()
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(n, dim)
+    off = np.zeros(n + 1, dtype=np.uint64)
+    cap = max(64 * n, 1)
+    while True:
+        idx = np.zeros(cap, dtype=np.uint32)
+        r = fn(dim, n, _dp(pts), *args,
+               off.ctypes.data_as(C.POINTER(C.c_uint64)),
+               idx.ctypes.data_as(C.POINTER(C.c_uint32)), cap)
+        if r == -1:
+            raise ValueError("bad search arguments")
+        if r < 0:
+            cap = -r
+            continue
+        return off, idx[:r]
+
+
+def grid_search(pts, radius, size_hint):
+    pts = np.asarray(pts, dtype=np.float64)
+    n, dim = pts.shape
+    return _search(load().orc_grid_search, float(radius), float(size_hint),
+                   n=n, dim=dim, pts=pts)
+
+
+def naive_search(pts, radius, strict=False):
+    pts = np.asarray(pts, dtype=np.float64)
+    n, dim = pts.shape
+    return _search(load().orc_naive_search, float(radius), 1 if strict else 0,
+                   n=n, dim=dim, pts=pts)
+
+
+class State:
+    """Particle state in the reference layout, owned by numpy arrays."""
+
+    def __init__(self, r, rho, n_fluid, normal, v=None, dv_dt=None):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        self.n, self.dim = r.shape
+        self.n_fluid = int(n_fluid)
+        self.r = r.copy()
+        self.v = np.zeros_like(r) if v is None else np.ascontiguousarray(v, dtype=np.float64).copy()
+        self.rho = np.ascontiguousarray(rho, dtype=np.float64).copy()
+        self.drho_dt = np.zeros(self.n)
+        self.dv_dt = np.zeros_like(r) if dv_dt is None else np.ascontiguousarray(dv_dt, dtype=np.float64).copy()
+        self.p = np.zeros(self.n)
+        self.cs = np.zeros(self.n)
+        self.normal = np.ascontiguousarray(normal, dtype=np.float64).reshape(-1, self.dim).copy()
+        if self.normal.shape[0] == 0:
+            self.normal = np.zeros((1, self.dim))
+        self.c = OrcState(self.n_fluid, self.n - self.n_fluid, _dp(self.r), _dp(self.v),
+                          _dp(self.rho), _dp(self.drho_dt), _dp(self.dv_dt),
+                          _dp(self.p), _dp(self.cs), _dp(self.normal))
+
+    @classmethod
+    def from_particles(cls, pa):
+        return cls(pa.r, pa.rho, pa.n_fluid, pa.normal, pa.v, pa.dv_dt)
+
+
+def _csr(off, idx):
+    return (off.ctypes.data_as(C.POINTER(C.c_uint64)),
+            idx.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+
+def adjacency(st: State, prm: OrcParams):
+    """Sorted neighbour lists as ParticleMesh builds them (hint h, radius 2h)."""
+    return grid_search(st.r, 2.0 * prm.h, prm.h)
+
+
+def setup_boundary(prm, st, off, idx):
+    load().orc_setup_boundary(C.byref(prm), C.byref(st.c), *_csr(off, idx))
+
+
+def compute_continuity(prm, st, off, idx):
+    load().orc_compute_continuity(C.byref(prm), C.byref(st.c), *_csr(off, idx))
+
+
+def compute_momentum(prm, st, off, idx):
+    load().orc_compute_momentum(C.byref(prm), C.byref(st.c), *_csr(off, idx))
+
+
+def compute_time_step(prm, st) -> float:
+    return load().orc_compute_time_step(C.byref(prm), C.byref(st.c))
+
+
+def ssprk_step(prm, st, order=3) -> float:
+    dt = C.c_double()
+    rc = load().orc_ssprk_step(C.byref(prm), C.byref(st.c), order, C.byref(dt))
+    if rc != 0:
+        raise RuntimeError("oracle step failed")
+    return dt.value
+
+
+RHS_FN = C.CFUNCTYPE(None, C.c_void_p, _DP, _DP, C.c_size_t)
+
+
+def ssprk_generic(order, dt, u, rhs):
+    """u' = rhs(u) stepped once with the oracle's SSPRK combination."""
+    u = np.ascontiguousarray(u, dtype=np.float64).copy()
+    n = u.size
+
+    def cb(_ctx, up, dup, nn):
+        uu = np.ctypeslib.as_array(up, shape=(nn,))
+        du = np.ctypeslib.as_array(dup, shape=(nn,))
+        du[:] = rhs(uu.copy())
+
+    f = RHS_FN(cb)
+    rc = load().orc_ssprk_generic(order, dt, n, _dp(u), C.cast(f, C.c_void_p), None)
+    if rc != 0:
+        raise RuntimeError("ssprk_generic failed")
+    return u

@@ -1,0 +1,687 @@
+/*
+ * FP64 CPU oracle for the WCSPH step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference algorithm (TitSolver, paths relative
+ * to its source tree).  Built with -ffp-contract=off: every product and sum is
+ * rounded on its own, left to right, which fixes the floating-point convention
+ * that the reference leaves to -ffast-math/-ffp-contract=fast and Highway's
+ * horizontal sums (cmake/clang.cmake:111-114, core/_vec/vec.hpp:628-640).
+ *
+ * What pins each part (tests under tests/ check these):
+ *   - neighbour search: O(N^2) brute force with the production predicate and
+ *     the reference's own naive-search fixture (geom/search.test.cpp:41-163);
+ *     integer output, compared bit-exactly.
+ *   - kernel: integral of W over the support equals 1 (kernel.test.cpp:33-91),
+ *     grad W equals a central difference of W (kernel.test.cpp:95-122),
+ *     weights equal the closed forms of the theory guide's table
+ *     (manual/.../02-kernel-functions.md:145-152).
+ *   - EOS: p(rho0) = 0, rho(p(rho)) = rho, rho(H(rho)) = rho, cs^2 = dp/drho.
+ *   - right-hand sides: pairwise momentum conservation, Galilean invariance,
+ *     rotation covariance, rest state (drho/dt = 0, dv/dt = -g), linear
+ *     velocity field divergence and linear pressure gradient on a lattice,
+ *     planted-mistake checks (mutation switch below).
+ *   - SSPRK: one step on u' = lambda u equals the cubic Taylor polynomial,
+ *     third-order convergence.
+ *   - time step: closed form on single-particle states.
+ */
+#include "tit_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Mutation switch used only by the oracle's own pinning tests: a non-zero
+ * value plants a known mistake so the tests can show that their invariants
+ * catch it.  0 = faithful restatement. */
+static int g_mutation = 0;
+void orc_set_mutation(int m) { g_mutation = m; }
+
+/* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
+/* core/math.hpp:85-163 and core/_vec/vec.hpp:627-679 conventions. */
+
+/* tiny_v<double> = eps^(1/3) (core/math.hpp:140-146). */
+static double tiny_v(void) { return cbrt(DBL_EPSILON); }
+static int is_tiny(double a) { return fabs(a) <= tiny_v(); }
+static double pow2(double a) { return a * a; }
+
+static double dot(int dim, const double* a, const double* b) {
+  double r = a[0] * b[0];
+  for (int k = 1; k < dim; ++k) r += a[k] * b[k];
+  return r;
+}
+static double norm2(int dim, const double* a) { return dot(dim, a, a); }
+static double norm(int dim, const double* a) { return sqrt(norm2(dim, a)); }
+
+/* normalize returns zero for |a|^2 < tiny^2 (core/_vec/vec.hpp:663-679). */
+static void normalize(int dim, const double* a, double* out) {
+  const double n2 = norm2(dim, a);
+  const double eps2 = pow2(tiny_v());
+  if (n2 >= eps2) {
+    const double s = sqrt(n2);
+    for (int k = 0; k < dim; ++k) out[k] = a[k] / s;
+  } else {
+    for (int k = 0; k < dim; ++k) out[k] = 0.0;
+  }
+}
+
+/* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
+/* Smoothing kernels.  Unit polynomials from sph/kernel.gen.cpp:539-546, both
+ * with support q < 2 (unit radius 2, kernel.gen.cpp:501-507). */
+
+double orc_kernel_weight(int kernel, int dim) {
+  /* 1 / (zero moment * unit sphere area) (sph/kernel.hpp:126-131); closed
+   * forms as in manual/.../02-kernel-functions.md:145-152. */
+  const double pi = 3.14159265358979323846;
+  if (kernel == ORC_KERNEL_WENDLAND_C2) {
+    if (dim == 2) return 7.0 / (4.0 * pi);
+    if (dim == 3) return 21.0 / (16.0 * pi);
+  } else if (kernel == ORC_KERNEL_WENDLAND_C4) {
+    if (dim == 2) return 9.0 / (4.0 * pi);
+    if (dim == 3) return 495.0 / (256.0 * pi);
+  }
+  return NAN;
+}
+
+double orc_kernel_unit_value(int kernel, double q) {
+  if (!(q < 2.0)) return 0.0;
+  const double t = 1.0 - q / 2.0;
+  if (kernel == ORC_KERNEL_WENDLAND_C2) {
+    return (1.0 + 2.0 * q) * (t * t) * (t * t);
+  }
+  if (kernel == ORC_KERNEL_WENDLAND_C4) {
+    const double t3 = t * t * t;
+    return (1.0 + 3.0 * q + 35.0 / 12.0 * q * q) * (t3 * t3);
+  }
+  return NAN;
+}
+
+double orc_kernel_unit_deriv(int kernel, double q) {
+  if (!(q < 2.0)) return 0.0;
+  const double t = 1.0 - q / 2.0;
+  if (kernel == ORC_KERNEL_WENDLAND_C2) {
+    /* d/dq (1+2q)(1-q/2)^4 = -5 q (1-q/2)^3 */
+    return -5.0 * q * (t * t * t);
+  }
+  if (kernel == ORC_KERNEL_WENDLAND_C4) {
+    /* d/dq (1+3q+35/12 q^2)(1-q/2)^6 = -14/3 q (1 + 5q/2) (1-q/2)^5 */
+    const double t5 = t * t * t * t * t;
+    return -14.0 / 3.0 * q * (1.0 + 2.5 * q) * t5;
+  }
+  return NAN;
+}
+
+double orc_kernel_radius(int kernel, double h) {
+  (void)kernel;
+  return 2.0 * h; /* sph/kernel.hpp:134-139 with unit radius 2 */
+}
+
+static double ipow(double a, int d) {
+  double r = a;
+  for (int k = 1; k < d; ++k) r *= a;
+  return r;
+}
+
+/* sph/kernel.hpp:142-151 */
+double orc_kernel_value(int kernel, int dim, const double* x, double h) {
+  const double h_inv = 1.0 / h;
+  const double w = orc_kernel_weight(kernel, dim) * ipow(h_inv, dim);
+  const double q = h_inv * norm(dim, x);
+  return w * orc_kernel_unit_value(kernel, q);
+}
+
+/* sph/kernel.hpp:154-163 */
+void orc_kernel_grad(int kernel, int dim, const double* x, double h,
+                     double* out) {
+  const double h_inv = 1.0 / h;
+  const double w = orc_kernel_weight(kernel, dim) * ipow(h_inv, dim);
+  const double q = h_inv * norm(dim, x);
+  double nx[3] = {0.0, 0.0, 0.0};
+  normalize(dim, x, nx);
+  const double s = w * orc_kernel_unit_deriv(kernel, q);
+  for (int k = 0; k < dim; ++k) out[k] = s * (nx[k] * h_inv);
+}
+
+/* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
+/* Tait equation of state (sph/equation_of_state.hpp:19-72). */
+
+static double eos_B(const orc_params* prm) {
+  return prm->rho0 * pow2(prm->cs0) / prm->xi;
+}
+double orc_eos_pressure(const orc_params* prm, double rho) {
+  return eos_B(prm) * (pow(rho / prm->rho0, prm->xi) - 1.0); /* :37-40 */
+}
+double orc_eos_density_from_pressure(const orc_params* prm, double p) {
+  return prm->rho0 * pow(1.0 + p / eos_B(prm), 1.0 / prm->xi); /* :43-46 */
+}
+double orc_eos_sound_speed(const orc_params* prm, double rho) {
+  return prm->cs0 * pow(rho / prm->rho0, (prm->xi - 1.0) / 2.0); /* :49-52 */
+}
+double orc_eos_potential(const orc_params* prm, double rho) {
+  const double xi_1 = prm->xi - 1.0; /* :55-58 */
+  return pow2(prm->cs0) * (pow(rho / prm->rho0, xi_1) - 1.0) / xi_1;
+}
+double orc_eos_density_from_potential(const orc_params* prm, double H) {
+  const double xi_1 = prm->xi - 1.0; /* :61-64 */
+  return prm->rho0 * pow(1.0 + xi_1 * H / pow2(prm->cs0), 1.0 / xi_1);
+}
+
+/* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
+/* Uniform grid index (geom/search/grid_search.hpp:73-130, geom/grid.hpp:86-169,
+ * geom/bbox.hpp:76-119, geom/point_range.hpp:82-87, geom/bsphere.hpp:47-54). */
+
+typedef struct grid_index {
+  int dim;
+  double lo[3], hi[3];
+  size_t nc[3];
+  double cext[3], inv[3];
+  size_t ncells;
+  size_t* cell_off; /* ncells + 1 */
+  uint32_t* cell_pts;
+} grid_index;
+
+static void grid_cell_index(const grid_index* g, const double* p, size_t* out) {
+  for (int k = 0; k < g->dim; ++k) {
+    const double f = (p[k] - g->lo[k]) * g->inv[k]; /* grid.hpp:105-112 */
+    out[k] = (size_t)f;
+  }
+}
+static size_t grid_flatten(const grid_index* g, const size_t* idx) {
+  size_t flat = idx[0]; /* grid.hpp:120-128: last axis fastest */
+  for (int k = 1; k < g->dim; ++k) flat = g->nc[k] * flat + idx[k];
+  return flat;
+}
+
+static int grid_build(grid_index* g, int dim, size_t n, const double* pts,
+                      double hint) {
+  memset(g, 0, sizeof(*g));
+  g->dim = dim;
+  if (n == 0) return 0;
+  for (int k = 0; k < dim; ++k) g->lo[k] = g->hi[k] = pts[k];
+  for (size_t i = 1; i < n; ++i) {
+    for (int k = 0; k < dim; ++k) {
+      const double x = pts[i * dim + k];
+      if (x < g->lo[k]) g->lo[k] = x;
+      if (x > g->hi[k]) g->hi[k] = x;
+    }
+  }
+  g->ncells = 1;
+  for (int k = 0; k < dim; ++k) {
+    g->lo[k] -= hint / 2.0; /* grow(size_hint / 2), grid_search.hpp:55 */
+    g->hi[k] += hint / 2.0;
+    const double ext = g->hi[k] - g->lo[k];
+    double ncf = ceil(ext / hint); /* set_cell_extents, grid.hpp:90-98 */
+    if (ncf < 1.0) ncf = 1.0;
+    g->nc[k] = (size_t)ncf;
+    g->cext[k] = ext / ncf;
+    g->inv[k] = 1.0 / g->cext[k];
+    g->ncells *= g->nc[k];
+  }
+  g->cell_off = (size_t*)calloc(g->ncells + 1, sizeof(size_t));
+  g->cell_pts = (uint32_t*)malloc(n * sizeof(uint32_t));
+  size_t* cell_of = (size_t*)malloc(n * sizeof(size_t));
+  if (!g->cell_off || !g->cell_pts || !cell_of) {
+    free(cell_of);
+    return -1;
+  }
+  size_t ci[3];
+  for (size_t i = 0; i < n; ++i) { /* count, grid_search.hpp:61-66 */
+    grid_cell_index(g, pts + i * dim, ci);
+    cell_of[i] = grid_flatten(g, ci);
+    g->cell_off[cell_of[i] + 1] += 1;
+  }
+  for (size_t c = 0; c < g->ncells; ++c) g->cell_off[c + 1] += g->cell_off[c];
+  size_t* fill = (size_t*)malloc(g->ncells * sizeof(size_t));
+  if (!fill) {
+    free(cell_of);
+    return -1;
+  }
+  memcpy(fill, g->cell_off, g->ncells * sizeof(size_t));
+  for (size_t i = 0; i < n; ++i) g->cell_pts[fill[cell_of[i]]++] = (uint32_t)i;
+  free(fill);
+  free(cell_of);
+  return 0;
+}
+
+static void grid_free(grid_index* g) {
+  free(g->cell_off);
+  free(g->cell_pts);
+  g->cell_off = NULL;
+  g->cell_pts = NULL;
+}
+
+/* Inclusive containment predicate of BSphere::contains (bsphere.hpp:52-54):
+ * d = point - center, s = d0*d0 + d1*d1 (+ d2*d2), s <= r^2. */
+static int sphere_contains(int dim, const double* center, double r2,
+                           const double* point) {
+  double d[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < dim; ++k) d[k] = point[k] - center[k];
+  return norm2(dim, d) <= r2;
+}
+
+typedef struct ubuf {
+  uint32_t* data;
+  size_t size, cap;
+} ubuf;
+
+static int ubuf_push(ubuf* b, uint32_t v) {
+  if (b->size == b->cap) {
+    size_t nc = b->cap ? 2 * b->cap : 1024;
+    uint32_t* nd = (uint32_t*)realloc(b->data, nc * sizeof(uint32_t));
+    if (!nd) return -1;
+    b->data = nd;
+    b->cap = nc;
+  }
+  b->data[b->size++] = v;
+  return 0;
+}
+
+static void sort_u32(uint32_t* a, size_t n) { /* insertion: lists are short */
+  for (size_t i = 1; i < n; ++i) {
+    const uint32_t x = a[i];
+    size_t j = i;
+    while (j > 0 && a[j - 1] > x) {
+      a[j] = a[j - 1];
+      --j;
+    }
+    a[j] = x;
+  }
+}
+
+/* GridIndex::search (grid_search.hpp:117-130) with cells_intersecting
+ * (grid.hpp:160-169), followed by the per-particle sort of
+ * ParticleMesh::search_ (particle_mesh.hpp:137-147). */
+static int grid_query(const grid_index* g, const double* pts,
+                      const double* center, double radius, ubuf* out) {
+  const int dim = g->dim;
+  const double r2 = pow2(radius);
+  double slo[3], shi[3];
+  size_t clo[3], chi[3];
+  for (int k = 0; k < dim; ++k) {
+    const double half = g->cext[k] / 2.0;
+    double lo = center[k] - radius; /* BSphere::box: BBox{c}.grow(r) */
+    double hi = center[k] + radius;
+    lo -= half; /* grow(half) */
+    hi += half;
+    if (g->lo[k] > lo) lo = g->lo[k]; /* intersect(box) */
+    if (g->hi[k] < hi) hi = g->hi[k];
+    lo += half; /* shrink(half) */
+    hi -= half;
+    slo[k] = lo;
+    shi[k] = hi;
+  }
+  grid_cell_index(g, slo, clo);
+  grid_cell_index(g, shi, chi);
+  const size_t start = out->size;
+  size_t c[3] = {0, 0, 0};
+  for (c[0] = clo[0]; c[0] <= chi[0]; ++c[0]) {
+    for (c[1] = clo[1]; c[1] <= chi[1]; ++c[1]) {
+      size_t z_lo = dim == 3 ? clo[2] : 0, z_hi = dim == 3 ? chi[2] : 0;
+      for (size_t z = z_lo; z <= z_hi; ++z) {
+        c[2] = z;
+        const size_t f = grid_flatten(g, c);
+        for (size_t j = g->cell_off[f]; j < g->cell_off[f + 1]; ++j) {
+          const uint32_t b = g->cell_pts[j];
+          if (sphere_contains(dim, center, r2, pts + (size_t)b * dim)) {
+            if (ubuf_push(out, b)) return -1;
+          }
+        }
+      }
+    }
+  }
+  sort_u32(out->data + start, out->size - start);
+  return 0;
+}
+
+/* Builds the full adjacency (CSR) of n points with a grid of the given hint. */
+static int build_adjacency(int dim, size_t n, const double* pts, double radius,
+                           double hint, uint64_t* off, ubuf* idx) {
+  grid_index g;
+  if (grid_build(&g, dim, n, pts, hint)) return -1;
+  idx->size = 0;
+  off[0] = 0;
+  for (size_t a = 0; a < n; ++a) {
+    if (grid_query(&g, pts, pts + a * dim, radius, idx)) {
+      grid_free(&g);
+      return -1;
+    }
+    off[a + 1] = idx->size;
+  }
+  grid_free(&g);
+  return 0;
+}
+
+static long long copy_out(const ubuf* b, uint32_t* idx, size_t cap) {
+  if (b->size > cap) return -(long long)b->size;
+  if (b->size) memcpy(idx, b->data, b->size * sizeof(uint32_t));
+  return (long long)b->size;
+}
+
+long long orc_grid_search(int dim, size_t n, const double* pts, double radius,
+                          double size_hint, uint64_t* off, uint32_t* idx,
+                          size_t cap) {
+  if ((dim != 2 && dim != 3) || !(radius > 0.0) || !(size_hint > 0.0)) {
+    return -1;
+  }
+  off[0] = 0;
+  if (n == 0) return 0;
+  ubuf b = {0, 0, 0};
+  if (build_adjacency(dim, n, pts, radius, size_hint, off, &b)) {
+    free(b.data);
+    return -1;
+  }
+  long long r = copy_out(&b, idx, cap);
+  free(b.data);
+  return r;
+}
+
+long long orc_naive_search(int dim, size_t n, const double* pts,
+                           double radius, int strict, uint64_t* off,
+                           uint32_t* idx, size_t cap) {
+  if (dim != 2 && dim != 3) return -1;
+  const double r2 = pow2(radius);
+  ubuf b = {0, 0, 0};
+  off[0] = 0;
+  for (size_t a = 0; a < n; ++a) {
+    for (size_t j = 0; j < n; ++j) {
+      double d[3] = {0.0, 0.0, 0.0};
+      for (int k = 0; k < dim; ++k) d[k] = pts[j * dim + k] - pts[a * dim + k];
+      const double s = norm2(dim, d);
+      const int in = strict ? (s < r2 || j == a) : (s <= r2);
+      if (in && ubuf_push(&b, (uint32_t)j)) {
+        free(b.data);
+        return -1;
+      }
+    }
+    off[a + 1] = b.size;
+  }
+  long long r = copy_out(&b, idx, cap);
+  free(b.data);
+  return r;
+}
+
+/* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
+/* Fluid equations (sph/fluid_equations.hpp), wall-free form: gamma = 1 and
+ * grad gamma = 0 everywhere, so the face loops vanish; fixed particles carry
+ * an explicit wall normal in place of normalize(grad_gamma). */
+
+/* setup_boundary, fluid_equations.hpp:122-164 */
+int orc_setup_boundary(const orc_params* prm, orc_state* s,
+                       const uint64_t* off, const uint32_t* idx) {
+  const int dim = prm->dim;
+  const size_t nf = s->n_fluid, n = s->n_fluid + s->n_fixed;
+  for (size_t e = nf; e < n; ++e) {
+    for (int k = 0; k < dim; ++k) s->v[e * dim + k] = 0.0; /* no-slip */
+    double S_e = 0.0, H_e = 0.0;
+    double n_e[3] = {0.0, 0.0, 0.0};
+    normalize(dim, s->normal + (e - nf) * dim, n_e);
+    for (uint64_t j = off[e]; j < off[e + 1]; ++j) {
+      const size_t b = idx[j];
+      if (b >= nf) continue; /* fluid neighbours only */
+      const double V_b = prm->m / s->rho[b];
+      double r_be[3] = {0.0, 0.0, 0.0};
+      for (int k = 0; k < dim; ++k) r_be[k] = s->r[b * dim + k] - s->r[e * dim + k];
+      const double W_be = orc_kernel_value(prm->kernel, dim, r_be, prm->h);
+      const double H_b = orc_eos_potential(prm, s->rho[b]);
+      S_e += V_b * W_be;
+      H_e += V_b * (H_b + prm->g * dot(dim, r_be, n_e) * n_e[1]) * W_be;
+    }
+    s->rho[e] = orc_eos_density_from_potential(prm, is_tiny(S_e) ? 0.0 : H_e / S_e);
+  }
+  return 0;
+}
+
+/* compute_continuity, fluid_equations.hpp:232-260.  The pair loop runs over
+ * unique pairs (b < a) of the sorted adjacency with symmetric scatter, as
+ * block_for_each over ParticleMesh::block_pairs does, in ascending order. */
+int orc_compute_continuity(const orc_params* prm, orc_state* s,
+                           const uint64_t* off, const uint32_t* idx) {
+  const int dim = prm->dim;
+  const size_t n = s->n_fluid + s->n_fixed;
+  const double m = prm->m;
+  for (size_t a = 0; a < n; ++a) s->cs[a] = orc_eos_sound_speed(prm, s->rho[a]);
+  for (size_t a = 0; a < n; ++a) s->drho_dt[a] = 0.0;
+  for (size_t a = 0; a < n; ++a) {
+    for (uint64_t j = off[a]; j < off[a + 1]; ++j) {
+      const size_t b = idx[j];
+      if (b >= a) break; /* sorted: unique pairs b < a */
+      double r_ab[3] = {0.0, 0.0, 0.0}, v_ab[3] = {0.0, 0.0, 0.0}, grad[3] = {0.0, 0.0, 0.0};
+      for (int k = 0; k < dim; ++k) {
+        r_ab[k] = s->r[a * dim + k] - s->r[b * dim + k];
+        v_ab[k] = s->v[a * dim + k] - s->v[b * dim + k];
+      }
+      orc_kernel_grad(prm->kernel, dim, r_ab, prm->h, grad);
+      const double rho_ab = s->rho[a] - s->rho[b];
+      if (prm->flavour == ORC_FLAVOUR_REF) {
+        /* Ferrari et al. (2009) diffusion, :253-258 */
+        const double cs_ab = s->cs[a] > s->cs[b] ? s->cs[a] : s->cs[b];
+        const double rn = norm(dim, r_ab);
+        double Psi[3] = {0.0, 0.0, 0.0}, ta[3] = {0.0, 0.0, 0.0}, tb[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < dim; ++k) Psi[k] = cs_ab * rho_ab * r_ab[k] / rn;
+        for (int k = 0; k < dim; ++k) {
+          ta[k] = v_ab[k] + Psi[k] / s->rho[b];
+          tb[k] = -v_ab[k] + Psi[k] / s->rho[a];
+        }
+        s->drho_dt[a] += m * dot(dim, ta, grad);
+        s->drho_dt[b] -= m * dot(dim, tb, grad);
+      } else {
+        /* delta-SPH (Molteni & Colagrossi 2009):
+         *   psi_ab = 2 (rho_a - rho_b) r_ab / |r_ab|^2,
+         *   drho_a/dt += delta h c0 V_b psi_ab . grad W_ab. */
+        const double r2 = norm2(dim, r_ab);
+        double psi[3] = {0.0, 0.0, 0.0};
+        double coef = 2.0 * rho_ab / r2;
+        if (g_mutation == 2) coef = -coef; /* planted: anti-diffusive sign */
+        for (int k = 0; k < dim; ++k) psi[k] = coef * r_ab[k];
+        const double dhc = prm->delta * prm->h * prm->cs0;
+        const double vg = dot(dim, v_ab, grad), pg = dot(dim, psi, grad);
+        s->drho_dt[a] += m * vg + dhc * (m / s->rho[b]) * pg;
+        s->drho_dt[b] += m * vg - dhc * (m / s->rho[a]) * pg;
+      }
+    }
+  }
+  return 0;
+}
+
+/* compute_momentum, fluid_equations.hpp:267-305 */
+int orc_compute_momentum(const orc_params* prm, orc_state* s,
+                         const uint64_t* off, const uint32_t* idx) {
+  const int dim = prm->dim;
+  const size_t nf = s->n_fluid, n = s->n_fluid + s->n_fixed;
+  const double m = prm->m;
+  for (size_t a = 0; a < n; ++a) {
+    s->p[a] = orc_eos_pressure(prm, s->rho[a]);
+    s->cs[a] = orc_eos_sound_speed(prm, s->rho[a]);
+  }
+  for (size_t a = 0; a < n; ++a) {
+    for (int k = 0; k < dim; ++k) s->dv_dt[a * dim + k] = 0.0;
+    if (a < nf) s->dv_dt[a * dim + 1] = -prm->g; /* unit<1>(r, -g) */
+  }
+  for (size_t a = 0; a < n; ++a) {
+    for (uint64_t j = off[a]; j < off[a + 1]; ++j) {
+      const size_t b = idx[j];
+      if (b >= a) break;
+      double r_ab[3] = {0.0, 0.0, 0.0}, v_ab[3] = {0.0, 0.0, 0.0}, grad[3] = {0.0, 0.0, 0.0};
+      for (int k = 0; k < dim; ++k) {
+        r_ab[k] = s->r[a * dim + k] - s->r[b * dim + k];
+        v_ab[k] = s->v[a * dim + k] - s->v[b * dim + k];
+      }
+      orc_kernel_grad(prm->kernel, dim, r_ab, prm->h, grad);
+      const double P_ab =
+          s->p[a] / pow2(s->rho[a]) + s->p[b] / pow2(s->rho[b]);
+      double Pi_ab;
+      if (prm->flavour == ORC_FLAVOUR_REF) {
+        Pi_ab = 2.0 * prm->mu * dot(dim, v_ab, r_ab) /
+                (s->rho[a] * s->rho[b] * norm2(dim, r_ab));
+      } else {
+        /* Monaghan (1992): Pi = -alpha c_ab h mu_ab / rho_ab for approaching
+         * pairs, mu_ab = v_ab.r_ab / (|r_ab|^2 + 0.01 h^2); written in the
+         * reference's (Pi - P) sign convention. */
+        const double vr = dot(dim, v_ab, r_ab);
+        Pi_ab = 0.0;
+        if (vr < 0.0 || g_mutation == 3) {
+          const double c_ab = 0.5 * (s->cs[a] + s->cs[b]);
+          const double rho_m = 0.5 * (s->rho[a] + s->rho[b]);
+          const double eta2 = 0.01 * pow2(prm->h);
+          Pi_ab = prm->alpha * c_ab * prm->h * vr /
+                  (rho_m * (norm2(dim, r_ab) + eta2));
+        }
+      }
+      double fa = m * (Pi_ab - P_ab);
+      const double fb = m * (Pi_ab - P_ab);
+      if (g_mutation == 1) fa = m * (Pi_ab + P_ab); /* planted: sign slip */
+      for (int k = 0; k < dim; ++k) {
+        s->dv_dt[a * dim + k] += fa * grad[k];
+        s->dv_dt[b * dim + k] -= fb * grad[k];
+      }
+    }
+  }
+  return 0;
+}
+
+/* compute_time_step, fluid_equations.hpp:199-222 (par::fold min). */
+double orc_compute_time_step(const orc_params* prm, const orc_state* s) {
+  const int dim = prm->dim;
+  double dt = DBL_MAX;
+  const double h = prm->h;
+  for (size_t a = 0; a < s->n_fluid; ++a) {
+    const double dt_ac =
+        prm->cfl * h / (orc_eos_sound_speed(prm, s->rho[a]) + norm(dim, s->v + a * dim));
+    if (dt_ac < dt) dt = dt_ac;
+    if (prm->mu > 0.0) {
+      const double dt_visc = prm->c_visc * pow2(h) * s->rho[a] / prm->mu;
+      if (dt_visc < dt) dt = dt_visc;
+    }
+    double an = norm(dim, s->dv_dt + a * dim);
+    if (an < prm->g) an = prm->g;
+    const double dt_force = prm->c_force * sqrt(h / an);
+    if (dt_force < dt) dt = dt_force;
+  }
+  return dt;
+}
+
+/* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
+/* SSPRK integrator (sph/time_integrator.hpp:140-228). */
+
+/* Weight of the new state in the Shu-Osher combination after stage `stage`
+ * (1-based; 0 = no combination), time_integrator.hpp:168-178. */
+static double ssprk_weight(int order, int stage) {
+  if (order == 2 && stage == 2) return 1.0 / 2.0;
+  if (order == 3 && stage == 2) return 1.0 / 4.0;
+  if (order == 3 && stage == 3) return 2.0 / 3.0;
+  return 0.0;
+}
+
+/* lincomb_: out = (1 - w) old + w out, time_integrator.hpp:212-223 */
+static void lincomb(size_t n, const double* old, double* out, double w) {
+  if (g_mutation == 4) w = 1.0 - w; /* planted: swapped weights */
+  for (size_t i = 0; i < n; ++i) out[i] = (1.0 - w) * old[i] + w * out[i];
+}
+
+int orc_ssprk_generic(int order, double dt, size_t n, double* u,
+                      orc_rhs_fn rhs, void* ctx) {
+  if (order != 2 && order != 3) return -1;
+  double* old = (double*)malloc(n * sizeof(double));
+  double* du = (double*)malloc(n * sizeof(double));
+  if (!old || !du) {
+    free(old);
+    free(du);
+    return -1;
+  }
+  memcpy(old, u, n * sizeof(double));
+  for (int stage = 1; stage <= order; ++stage) {
+    rhs(ctx, u, du, n);
+    for (size_t i = 0; i < n; ++i) u[i] += dt * du[i];
+    const double w = ssprk_weight(order, stage);
+    if (w != 0.0) lincomb(n, old, u, w);
+  }
+  free(old);
+  free(du);
+  return 0;
+}
+
+typedef struct step_ws {
+  uint64_t* off;
+  size_t off_cap;
+  ubuf idx;
+  double *r_old, *v_old, *rho_old;
+  size_t old_cap;
+} step_ws;
+
+static step_ws g_ws;
+
+static int ws_reserve(step_ws* w, size_t n, int dim) {
+  if (w->off_cap < n + 1) {
+    free(w->off);
+    w->off = (uint64_t*)malloc((n + 1) * sizeof(uint64_t));
+    if (!w->off) return -1;
+    w->off_cap = n + 1;
+  }
+  if (w->old_cap < n * 3) {
+    free(w->r_old);
+    free(w->v_old);
+    free(w->rho_old);
+    w->r_old = (double*)malloc(n * 3 * sizeof(double));
+    w->v_old = (double*)malloc(n * 3 * sizeof(double));
+    w->rho_old = (double*)malloc(n * sizeof(double));
+    if (!w->r_old || !w->v_old || !w->rho_old) return -1;
+    w->old_cap = n * 3;
+  }
+  (void)dim;
+  return 0;
+}
+
+/* FluidEquations::prepare without gamma: index + setup_boundary
+ * (fluid_equations.hpp:99-105).  The search uses GridSearch{h} with radius
+ * 2h as titwcsph does (wcsph.cpp:145-154, fluid_equations.hpp:112-115). */
+static int prepare(const orc_params* prm, orc_state* s, step_ws* w) {
+  const size_t n = s->n_fluid + s->n_fixed;
+  if (build_adjacency(prm->dim, n, s->r, orc_kernel_radius(prm->kernel, prm->h),
+                      prm->h, w->off, &w->idx)) {
+    return -1;
+  }
+  return orc_setup_boundary(prm, s, w->off, w->idx.data);
+}
+
+/* SSPRKIntegrator::substep_, time_integrator.hpp:188-210 */
+static int substep(const orc_params* prm, orc_state* s, step_ws* w,
+                   double* dt) {
+  const int dim = prm->dim;
+  if (prepare(prm, s, w)) return -1;
+  if (*dt <= 0.0) *dt = orc_compute_time_step(prm, s);
+  orc_compute_continuity(prm, s, w->off, w->idx.data);
+  orc_compute_momentum(prm, s, w->off, w->idx.data);
+  const double h = *dt;
+  for (size_t a = 0; a < s->n_fluid; ++a) {
+    for (int k = 0; k < dim; ++k) s->r[a * dim + k] += h * s->v[a * dim + k];
+    for (int k = 0; k < dim; ++k) s->v[a * dim + k] += h * s->dv_dt[a * dim + k];
+    s->rho[a] += h * s->drho_dt[a];
+  }
+  return 0;
+}
+
+/* SSPRKIntegrator::step, time_integrator.hpp:159-184 (post_integrate, the
+ * particle shifting stage, is not part of this path). */
+int orc_ssprk_step(const orc_params* prm, orc_state* s, int order,
+                   double* dt_out) {
+  if (order != 2 && order != 3) return -1;
+  const int dim = prm->dim;
+  const size_t n = s->n_fluid + s->n_fixed, nf = s->n_fluid;
+  step_ws* w = &g_ws;
+  if (ws_reserve(w, n, dim)) return -1;
+  memcpy(w->r_old, s->r, nf * dim * sizeof(double));
+  memcpy(w->v_old, s->v, nf * dim * sizeof(double));
+  memcpy(w->rho_old, s->rho, nf * sizeof(double));
+  double dt = 0.0;
+  for (int stage = 1; stage <= order; ++stage) {
+    if (substep(prm, s, w, &dt)) return -1;
+    const double wgt = ssprk_weight(order, stage);
+    if (wgt != 0.0) {
+      lincomb(nf * dim, w->r_old, s->r, wgt);
+      lincomb(nf * dim, w->v_old, s->v, wgt);
+      lincomb(nf, w->rho_old, s->rho, wgt);
+    }
+  }
+  if (dt_out) *dt_out = dt;
+  return 0;
+}

@@ -1,0 +1,195 @@
+"""HIP path vs the FP64 oracle (device 0).
+
+Tolerances: neighbour sets are integer data and must match bit-exactly.
+Boundary extrapolation and time-step selection run in FP64 on the device and
+must agree to 1e-12 relative.  The pair sums run in FP32 on relative
+coordinates with Kahan accumulation; their tolerance is stated per test
+relative to the largest oracle value of the same field.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import titsolver_amd as ta
+from helpers import csr_equal, lattice, perturbed_dam_break, random_cloud
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = [ta.KERNEL_WENDLAND_C2, ta.KERNEL_WENDLAND_C4]
+FLAVOURS = [ta.FLAVOUR_CFG, ta.FLAVOUR_REF]
+
+
+def _eq(dim, dr, kernel=ta.KERNEL_WENDLAND_C2, flavour=ta.FLAVOUR_CFG, h_factor=2.0):
+    eq = ta.cfg_equations(dim, dr, 0.6, kernel=kernel, h_factor=h_factor,
+                          flavour=flavour)
+    if flavour == ta.FLAVOUR_REF:
+        eq.mu, eq.cfl = 1e-3, 0.4
+    return eq
+
+
+def _ctx(eq, arrays, order=3):
+    ctx = ta.HipContext(eq, order=order)
+    ctx.upload(arrays)
+    return ctx
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_neighbors_dam_break_bit_exact(dim):
+    dr = 0.02 if dim == 2 else 0.025
+    pa = perturbed_dam_break(dim, dr, seed=1)
+    eq = _eq(dim, dr)
+    ctx = _ctx(eq, [pa])
+    got = ctx.neighbors()
+    want = oracle.grid_search(pa.r, 2.0 * eq.h, eq.h)
+    assert csr_equal(got, want)
+
+
+@pytest.mark.parametrize("dim,n_axis", [(2, 40), (3, 14)])
+def test_neighbors_lattice_ties_bit_exact(dim, n_axis):
+    # h = 2 dr: offsets (4, 0) etc. lie exactly on the support radius.
+    dr = 0.01
+    pa = lattice(dim, n_axis, dr)
+    eq = _eq(dim, dr)
+    ctx = _ctx(eq, [pa])
+    got = ctx.neighbors()
+    want = oracle.naive_search(pa.r, 2.0 * eq.h)
+    assert csr_equal(got, want)
+    counts = np.diff(got[0])
+    assert counts.max() == (49 if dim == 2 else 257)
+
+
+@pytest.mark.parametrize("dim,radius", [(2, 0.01), (2, 0.1), (3, 0.1), (3, 0.5)])
+def test_neighbors_random_cloud_bit_exact(dim, radius):
+    pa = random_cloud(dim, 2000, seed=123)
+    eq = _eq(dim, radius / 2.0)
+    eq.h = radius / 2.0
+    ctx = _ctx(eq, [pa])
+    got = ctx.neighbors()
+    want = oracle.naive_search(pa.r, radius)
+    assert csr_equal(got, want)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("flavour", FLAVOURS)
+def test_prepare_and_rhs_vs_oracle(dim, kernel, flavour):
+    dr = 0.02 if dim == 2 else 0.025
+    pa = perturbed_dam_break(dim, dr, seed=7)
+    eq = _eq(dim, dr, kernel, flavour)
+    ctx = _ctx(eq, [pa])
+    ctx.prepare()
+    st_g = ctx.download()
+    drho_g, dv_g = ctx.compute_rhs()
+
+    prm = oracle.params(eq)
+    st = oracle.State.from_particles(pa)
+    off, idx = oracle.adjacency(st, prm)
+    oracle.setup_boundary(prm, st, off, idx)
+    nf = pa.n_fluid
+    # boundary extrapolation: FP64 on both sides
+    np.testing.assert_allclose(st_g["rho"][nf:], st.rho[nf:], rtol=1e-12, atol=0)
+    oracle.compute_continuity(prm, st, off, idx)
+    oracle.compute_momentum(prm, st, off, idx)
+    # FP32 pair sums, Kahan-compensated: 2e-5 of the field's largest value.
+    scale_rho = np.abs(st.drho_dt[:nf]).max()
+    scale_v = np.abs(st.dv_dt[:nf]).max()
+    err_rho = np.abs(drho_g[:nf] - st.drho_dt[:nf]).max() / scale_rho
+    err_v = np.abs(dv_g[:nf] - st.dv_dt[:nf]).max() / scale_v
+    print(f"dim={dim} kernel={kernel} flavour={flavour}: "
+          f"drho err {err_rho:.2e} dv err {err_v:.2e}")
+    assert err_rho < 2e-5
+    assert err_v < 2e-5
+    assert np.all(drho_g[nf:] == 0.0) and np.all(dv_g[nf:] == 0.0)
+
+
+@pytest.mark.parametrize("flavour", FLAVOURS)
+def test_time_step_vs_oracle(flavour):
+    pa = perturbed_dam_break(2, 0.02, seed=3)
+    rng = np.random.default_rng(5)
+    pa.dv_dt = rng.normal(0.0, 50.0, size=pa.r.shape)
+    eq = _eq(2, 0.02, flavour=flavour)
+    ctx = _ctx(eq, [pa])
+    dt_g = ctx.compute_time_step()[0]
+    st = oracle.State.from_particles(pa)
+    dt_o = oracle.compute_time_step(oracle.params(eq), st)
+    assert abs(dt_g - dt_o) <= 1e-14 * dt_o
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_dam_break_steps_vs_oracle(order):
+    """50 steps of the configuration-1 dam break (dr = 0.02, 6218 particles)."""
+    dr = 0.02
+    pa = ta.dam_break(2, dr, [3.22, 2.4], [1.0, 2.0])
+    eq = ta.cfg_equations(2, dr, 2.0)
+    ctx = _ctx(eq, [pa], order=order)
+    st = oracle.State.from_particles(pa)
+    prm = oracle.params(eq)
+    nf = pa.n_fluid
+    for s in range(50):
+        dt_g = ctx.step(1)
+        dt_o = oracle.ssprk_step(prm, st, order)
+        assert abs(dt_g - dt_o) <= 1e-9 * dt_o, (s, dt_g, dt_o)
+    out = ctx.download()
+    rho_err = np.max(np.abs(out["rho"][:nf] - st.rho[:nf]) / st.rho[:nf])
+    p_err = np.max(np.abs(out["p"][:nf] - st.p[:nf])) / np.abs(st.p[:nf]).max()
+    r_err = np.max(np.abs(out["r"][:nf] - st.r[:nf])) / dr
+    print(f"order {order}: rho rel {rho_err:.2e}, p/pmax {p_err:.2e}, r/dr {r_err:.2e}")
+    assert rho_err < 1e-6
+    assert r_err < 1e-5
+    # fixed particles never move
+    assert np.array_equal(out["r"][nf:], pa.r[nf:])
+
+
+def test_batched_tanks_equal_single_runs():
+    dr = 0.02
+    specs = [([1.6, 1.2], [0.4, 0.8]), ([2.0, 1.5], [0.7, 1.0]),
+             ([1.2, 1.0], [0.3, 0.5])]
+    arrays = [ta.dam_break(2, dr, t, c) for t, c in specs]
+    eq = ta.cfg_equations(2, dr, 1.0)
+    batch = _ctx(eq, arrays)
+    batch.step(6)
+    out_b = batch.download()
+    t_b, _ = batch.tank_clock()
+    o = 0
+    for k, pa in enumerate(arrays):
+        single = _ctx(eq, [pa])
+        single.step(6)
+        out_s = single.download()
+        t_s, _ = single.tank_clock()
+        sl = slice(o, o + pa.n)
+        for f in ("r", "v", "rho"):
+            assert np.array_equal(out_b[f][sl], out_s[f]), (k, f)
+        assert t_b[k] == t_s[0]
+        o += pa.n
+
+
+def test_graph_replay_equals_eager():
+    pa = ta.dam_break(2, 0.02, [3.22, 2.4], [1.0, 2.0])
+    eq = ta.cfg_equations(2, 0.02, 2.0)
+    a = _ctx(eq, [pa])
+    a.step(8)  # graph of two steps, replayed
+    b = _ctx(eq, [pa])
+    b.set_profiling(True)  # eager launches with event timing
+    b.step(8)
+    ra, rb = a.download(), b.download()
+    for f in ("r", "v", "rho"):
+        assert np.array_equal(ra[f], rb[f]), f
+    stats = b.kernel_stats()
+    assert stats["stage"][0] == 8 * 3
+
+
+def test_rebuild_interleaved_with_hooks():
+    """prepare()/neighbors() between graph replays change the buffer parity."""
+    pa = ta.dam_break(2, 0.02, [3.22, 2.4], [1.0, 2.0])
+    eq = ta.cfg_equations(2, 0.02, 2.0)
+    a = _ctx(eq, [pa])
+    a.step(2)
+    a.neighbors()
+    a.step(4)
+    b = _ctx(eq, [pa])
+    b.step(6)
+    ra, rb = a.download(), b.download()
+    for f in ("r", "v", "rho"):
+        np.testing.assert_array_equal(ra[f], rb[f])

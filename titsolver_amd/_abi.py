@@ -1,0 +1,103 @@
+"""ctypes declarations of the C ABI in include/tit_hip.h.
+
+The shared library is loaded from this package directory; a missing library
+raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libtit_hip.so"
+
+TIT_OK, TIT_E_ARG, TIT_E_HIP, TIT_E_OOM, TIT_E_NUMERIC, TIT_E_STATE = range(6)
+STATUS_NAMES = {0: "TIT_OK", 1: "TIT_E_ARG", 2: "TIT_E_HIP", 3: "TIT_E_OOM",
+                4: "TIT_E_NUMERIC", 5: "TIT_E_STATE"}
+
+KERNEL_WENDLAND_C2 = 0
+KERNEL_WENDLAND_C4 = 1
+FLAVOUR_REF = 0
+FLAVOUR_CFG = 1
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("kernel", C.c_int), ("flavour", C.c_int),
+        ("ssprk_order", C.c_int),
+        ("h", C.c_double), ("m", C.c_double), ("rho0", C.c_double),
+        ("cs0", C.c_double), ("xi", C.c_double), ("g", C.c_double),
+        ("mu", C.c_double), ("alpha", C.c_double), ("delta", C.c_double),
+        ("cfl", C.c_double), ("c_force", C.c_double), ("c_visc", C.c_double),
+        ("skin", C.c_double),
+    ]
+
+
+class Tank(C.Structure):
+    _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3)]
+
+
+class HostView(C.Structure):
+    _fields_ = [
+        ("r", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
+        ("rho", C.POINTER(C.c_double)), ("dv_dt", C.POINTER(C.c_double)),
+        ("normal", C.POINTER(C.c_double)),
+        ("n_fluid", C.c_size_t), ("n_fixed", C.c_size_t),
+    ]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64),
+                ("total_ms", C.c_double)]
+
+
+# name -> (restype, argtypes); mirrors include/tit_hip.h one to one.
+_DP = C.POINTER(C.c_double)
+SIGNATURES = {
+    "tit_hip_create": (C.c_int, [C.c_int, C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "tit_hip_upload": (C.c_int, [C.c_void_p, C.POINTER(HostView), C.POINTER(Tank)]),
+    "tit_hip_upload_batch": (C.c_int, [C.c_void_p, C.POINTER(HostView),
+                                       C.POINTER(Tank), C.c_size_t]),
+    "tit_hip_step": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "tit_hip_download": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP]),
+    "tit_hip_tank_clock": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "tit_hip_prepare": (C.c_int, [C.c_void_p]),
+    "tit_hip_compute_time_step": (C.c_int, [C.c_void_p, _DP]),
+    "tit_hip_compute_rhs": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "tit_hip_neighbors": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint32), C.c_size_t,
+                                    C.POINTER(C.c_size_t)]),
+    "tit_hip_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "tit_hip_kernel_stats": (C.c_int, [C.c_void_p, C.POINTER(KernelStat), C.c_int,
+                                       C.POINTER(C.c_int)]),
+    "tit_hip_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t),
+                                 C.POINTER(C.c_uint64)]),
+    "tit_hip_last_error": (C.c_char_p, [C.c_void_p]),
+    "tit_hip_destroy": (None, [C.c_void_p]),
+    "tit_dam_break_count": (C.c_int, [C.c_int, C.c_double, _DP, _DP, C.c_int,
+                                      C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "tit_dam_break_fill": (C.c_int, [C.c_int, C.c_double, _DP, _DP, C.c_int,
+                                     C.c_double, C.c_double, C.c_double,
+                                     _DP, _DP, _DP]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libtit_hip.so from the package directory (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+            "`make -C titsolver_amd/csrc`")
+    lib = C.CDLL(os.fspath(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,803 @@
+// C ABI of the MI355X WCSPH step: context, buffers, step schedule.
+//
+// Buffer rotation of one SSPRK3 step (state in A at the step start):
+//   rebuild  keys(A) -> stable radix sort -> permute A into T, swap A and T
+//   stage 1  boundary(A); B = A + dt L(A)
+//   stage 2  boundary(B); C = 3/4 A + 1/4 (B + dt L(B))
+//   stage 3  boundary(C); A = 1/3 A + 2/3 (C + dt L(C))  (+ next dt, in place:
+//            each particle reads and writes only its own slot of A)
+// Every rebuild swaps the A/T, key and id/tag pointers, so two steps bring
+// every pointer back to where it started: the hipGraph holds two steps.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tit_hip.h"
+#include "sph_kernels.hpp"
+
+using namespace tit_amd;
+
+namespace tit_amd {
+
+// ---- profiler ----------------------------------------------------------------
+
+hipEvent_t Profiler::get_event() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void Profiler::begin(hipStream_t s, const char* name) {
+  Rec r{name, get_event(), get_event()};
+  (void)hipEventRecord(r.a, s);
+  pending_.push_back(r);
+}
+
+void Profiler::end(hipStream_t s) {
+  if (!pending_.empty()) (void)hipEventRecord(pending_.back().b, s);
+}
+
+void Profiler::collect() {
+  for (Rec& r : pending_) {
+    (void)hipEventSynchronize(r.b);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    auto it = std::find_if(totals.begin(), totals.end(),
+                           [&](const Total& t) { return t.name == r.name; });
+    if (it == totals.end()) {
+      totals.push_back(Total{r.name, 0, 0.0});
+      it = totals.end() - 1;
+    }
+    it->launches += 1;
+    it->total_ms += ms;
+    pool_.push_back(r.a);
+    pool_.push_back(r.b);
+  }
+  pending_.clear();
+}
+
+void Profiler::clear() {
+  collect();
+  totals.clear();
+}
+
+Profiler::~Profiler() {
+  for (Rec& r : pending_) {
+    pool_.push_back(r.a);
+    pool_.push_back(r.b);
+  }
+  for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+}
+
+}  // namespace tit_amd
+
+struct tit_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  tit_hip_params prm{};
+  DevParams dp{};
+  std::string err;
+  Profiler prof;
+
+  // sizes
+  uint32_t n = 0;
+  uint32_t n_fixed = 0;
+  int n_tanks = 0;
+  uint32_t n_cells = 0;
+  int key_bits = 1;
+  bool uploaded = false;
+  uint64_t rebuilds = 0;
+
+  // device memory
+  std::vector<void*> allocs;
+  DevState A{}, T{}, B{}, C{};
+  uint32_t *key = nullptr, *key_alt = nullptr, *val = nullptr, *val_alt = nullptr;
+  uint32_t *id = nullptr, *id_alt = nullptr, *tag = nullptr, *tag_alt = nullptr;
+  uint32_t* cell_off = nullptr;
+  uint32_t* fixed_list = nullptr;
+  uint32_t* counter = nullptr;  // [0] fixed count, [1] flags
+  SortScratch sort;
+  DevTank* tanks = nullptr;
+  double* normal = nullptr;
+  double *dt = nullptr, *time = nullptr, *disp = nullptr;
+  unsigned long long* dt_min = nullptr;
+  uint32_t* vmax = nullptr;
+  double *stage_r = nullptr, *stage_v = nullptr, *stage_rho = nullptr,
+         *stage_p = nullptr;
+  double* rhs_drho = nullptr;
+  double* rhs_dv[3] = {nullptr, nullptr, nullptr};
+
+  // graph of two steps, valid while the rebuild parity matches
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_def = nullptr;
+  uint64_t graph_parity = 0;
+};
+
+namespace {
+
+tit_status fail(tit_hip_ctx* c, tit_status s, const std::string& msg) {
+  if (c) c->err = msg;
+  return s;
+}
+
+#define TIT_HIP_TRY(ctx, call)                                              \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      return fail((ctx), e_ == hipErrorOutOfMemory ? TIT_E_OOM : TIT_E_HIP, \
+                  std::string(#call) + ": " + hipGetErrorString(e_));       \
+    }                                                                       \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(tit_hip_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e == hipSuccess) c->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return e;
+}
+
+void drop_graph(tit_hip_ctx* c) {
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  if (c->graph_def) {
+    (void)hipGraphDestroy(c->graph_def);
+    c->graph_def = nullptr;
+  }
+}
+
+void free_all(tit_hip_ctx* c) {
+  drop_graph(c);
+  for (void* p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  c->uploaded = false;
+}
+
+hipError_t alloc_state(tit_hip_ctx* c, DevState& s, uint32_t n, int dim) {
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 3; ++k) s.x[k] = s.v[k] = nullptr;
+  for (int k = 0; k < dim && e == hipSuccess; ++k) {
+    e = dalloc(c, &s.x[k], n);
+    if (e == hipSuccess) e = dalloc(c, &s.v[k], n);
+  }
+  if (e == hipSuccess) e = dalloc(c, &s.rho, n);
+  return e;
+}
+
+bool is_small_int(double x, int lo, int hi) {
+  return x == std::floor(x) && x >= lo && x <= hi;
+}
+
+double kernel_omega(int kernel, int dim) {
+  const double pi = 3.14159265358979323846;
+  if (kernel == TIT_KERNEL_WENDLAND_C2) {
+    return dim == 2 ? 7.0 / (4.0 * pi) : 21.0 / (16.0 * pi);
+  }
+  return dim == 2 ? 9.0 / (4.0 * pi) : 495.0 / (256.0 * pi);
+}
+
+tit_status fill_params(tit_hip_ctx* c, const tit_hip_params& p) {
+  if (p.dim != 2 && p.dim != 3) return fail(c, TIT_E_ARG, "dim must be 2 or 3");
+  if (p.kernel != TIT_KERNEL_WENDLAND_C2 && p.kernel != TIT_KERNEL_WENDLAND_C4) {
+    return fail(c, TIT_E_ARG, "unknown kernel");
+  }
+  if (p.flavour != TIT_FLAVOUR_REF && p.flavour != TIT_FLAVOUR_CFG) {
+    return fail(c, TIT_E_ARG, "unknown flavour");
+  }
+  if (p.ssprk_order != 2 && p.ssprk_order != 3) {
+    return fail(c, TIT_E_ARG, "ssprk_order must be 2 or 3");
+  }
+  if (!(p.h > 0.0) || !(p.m > 0.0) || !(p.rho0 > 0.0) || !(p.cs0 > 0.0) ||
+      !(p.xi > 1.0) || !(p.g >= 0.0) || !(p.mu >= 0.0) || !(p.cfl > 0.0) ||
+      !(p.c_force > 0.0) || !(p.skin >= 0.0)) {
+    return fail(c, TIT_E_ARG, "non-physical parameter");
+  }
+  if (p.flavour == TIT_FLAVOUR_REF && p.mu > 0.0 && !(p.c_visc > 0.0)) {
+    return fail(c, TIT_E_ARG, "c_visc must be positive when mu > 0");
+  }
+  c->prm = p;
+  DevParams& d = c->dp;
+  d = DevParams{};
+  d.dim = p.dim;
+  d.kernel = p.kernel;
+  d.flavour = p.flavour;
+  d.order = p.ssprk_order;
+  d.xi_int = is_small_int(p.xi, 2, 15) ? static_cast<int>(p.xi) : 0;
+  d.h = p.h;
+  d.m = p.m;
+  d.rho0 = p.rho0;
+  d.cs0 = p.cs0;
+  d.xi = p.xi;
+  d.g = p.g;
+  d.mu = p.mu;
+  d.alpha = p.alpha;
+  d.delta = p.delta;
+  d.cfl = p.cfl;
+  d.c_force = p.c_force;
+  d.c_visc = p.c_visc;
+  const double radius = 2.0 * p.h;
+  d.R2 = radius * radius;
+  d.eos_B = p.rho0 * (p.cs0 * p.cs0) / p.xi;
+  d.inv_rho0 = 1.0 / p.rho0;
+  const double skin = (p.skin > 0.0 ? p.skin : 0.1) * radius;
+  d.skin_half = 0.5 * skin;
+  const double omega = kernel_omega(p.kernel, p.dim);
+  const double c_w = p.kernel == TIT_KERNEL_WENDLAND_C2 ? -5.0 : -14.0 / 3.0;
+  d.inv_h = static_cast<float>(1.0 / p.h);
+  d.gsc = static_cast<float>(c_w * omega / std::pow(p.h, p.dim + 2));
+  d.wsc = static_cast<float>(omega / std::pow(p.h, p.dim));
+  d.m_f = static_cast<float>(p.m);
+  d.dhc2 = static_cast<float>(2.0 * p.delta * p.h * p.cs0);
+  d.ahc = static_cast<float>(p.alpha * p.h);
+  d.eta2 = static_cast<float>(0.01 * p.h * p.h);
+  d.two_mu = static_cast<float>(2.0 * p.mu);
+  return TIT_OK;
+}
+
+// Cell size: (support radius + skin) / stencil half-width, a hair larger so
+// that rounding in the cell index cannot break the coverage argument.
+double cell_size(const tit_hip_ctx* c) {
+  const double radius = 2.0 * c->prm.h;
+  return (radius + 2.0 * c->dp.skin_half) / kStencil * (1.0 + 1e-9);
+}
+
+// ---- step pieces ----------------------------------------------------------------
+
+void swap_state(DevState& a, DevState& b) {
+  DevState t = a;
+  a = b;
+  b = t;
+}
+
+template <class T>
+void swap_ptr(T*& a, T*& b) {
+  T* t = a;
+  a = b;
+  b = t;
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  const int i = threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+void enqueue_rebuild(tit_hip_ctx* c) {
+  hipStream_t s = c->stream;
+  {
+    ProfScope ps(c->prof, s, "cell_keys");
+    launch_cell_keys(s, c->dp, c->tanks, c->n, c->A, c->tag, c->key,
+                     c->counter + 1);
+  }
+  radix_sort_pairs(s, c->prof, c->n, c->key_bits, &c->key, &c->key_alt,
+                   &c->val, &c->val_alt, c->sort);
+  {
+    ProfScope ps(c->prof, s, "permute");
+    launch_permute(s, c->dp.dim, c->n, c->val, c->A, c->T, c->id, c->id_alt,
+                   c->tag, c->tag_alt);
+  }
+  swap_state(c->A, c->T);
+  swap_ptr(c->id, c->id_alt);
+  swap_ptr(c->tag, c->tag_alt);
+  {
+    ProfScope ps(c->prof, s, "cell_offsets");
+    launch_cell_offsets(s, c->n_cells, c->n, c->key, c->cell_off);
+  }
+  {
+    ProfScope ps(c->prof, s, "compact_fixed");
+    k_zero_u32<<<1, 64, 0, s>>>(c->counter, 1);
+    launch_compact_fixed(s, c->n, c->tag, c->fixed_list, c->counter);
+  }
+  launch_reset_disp(s, c->n_tanks, c->disp);
+  c->rebuilds += 1;
+}
+
+void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
+  BoundaryArgs a{};
+  a.P = c->dp;
+  a.tanks = c->tanks;
+  a.s = st;
+  a.key = c->key;
+  a.tag = c->tag;
+  a.id = c->id;
+  a.cell_off = c->cell_off;
+  a.fixed_list = c->fixed_list;
+  a.normal = c->normal;
+  a.n_fixed = c->n_fixed;
+  ProfScope ps(c->prof, c->stream, "boundary");
+  launch_boundary(c->stream, a);
+}
+
+StageArgs stage_args(tit_hip_ctx* c) {
+  StageArgs a{};
+  a.P = c->dp;
+  a.tanks = c->tanks;
+  a.key = c->key;
+  a.tag = c->tag;
+  a.cell_off = c->cell_off;
+  a.dt = c->dt;
+  a.n = c->n;
+  a.dt_min = c->dt_min;
+  a.vmax = c->vmax;
+  a.rhs_drho = c->rhs_drho;
+  for (int k = 0; k < 3; ++k) a.rhs_dv[k] = c->rhs_dv[k];
+  return a;
+}
+
+void enqueue_stage(tit_hip_ctx* c, const DevState& in, const DevState& old,
+                   const DevState& out, int kind, double w, bool final) {
+  StageArgs a = stage_args(c);
+  a.in = in;
+  a.old = old;
+  a.out = out;
+  a.w = w;
+  ProfScope ps(c->prof, c->stream, "stage");
+  launch_stage(c->stream, a, kind, final);
+}
+
+// One SSPRK step, SSPRKIntegrator::step (time_integrator.hpp:159-184).
+void enqueue_step(tit_hip_ctx* c) {
+  hipStream_t s = c->stream;
+  launch_step_begin(s, c->n_tanks, c->dt_min, c->dt, c->vmax);
+  enqueue_rebuild(c);
+  enqueue_boundary(c, c->A);
+  enqueue_stage(c, c->A, c->A, c->B, kEuler, 1.0, false);
+  enqueue_boundary(c, c->B);
+  if (c->prm.ssprk_order == 2) {
+    enqueue_stage(c, c->B, c->A, c->A, kCombine, 1.0 / 2.0, true);
+  } else {
+    enqueue_stage(c, c->B, c->A, c->C, kCombine, 1.0 / 4.0, false);
+    enqueue_boundary(c, c->C);
+    enqueue_stage(c, c->C, c->A, c->A, kCombine, 2.0 / 3.0, true);
+  }
+  launch_step_end(s, c->n_tanks, c->dt, c->vmax, c->time, c->disp,
+                  c->dp.skin_half, c->counter + 1);
+}
+
+tit_status check_launch(tit_hip_ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    return fail(c, TIT_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  }
+  return TIT_OK;
+}
+
+tit_status read_flags(tit_hip_ctx* c) {
+  uint32_t flags = 0;
+  TIT_HIP_TRY(c, hipMemcpyAsync(&flags, c->counter + 1, sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, c->stream));
+  TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (flags & 2u) {
+    return fail(c, TIT_E_NUMERIC, "non-finite particle position");
+  }
+  if (flags & 1u) {
+    return fail(c, TIT_E_NUMERIC,
+                "particles moved farther than half the neighbour skin within "
+                "one step; raise tit_hip_params.skin");
+  }
+  return TIT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tit_status tit_hip_create(int device, const tit_hip_params* params,
+                          tit_hip_ctx** out) {
+  if (!out || !params) return TIT_E_ARG;
+  *out = nullptr;
+  auto* c = new (std::nothrow) tit_hip_ctx();
+  if (!c) return TIT_E_OOM;
+  tit_status st = fill_params(c, *params);
+  if (st != TIT_OK) {
+    delete c;
+    return st;
+  }
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    delete c;
+    return TIT_E_HIP;
+  }
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return TIT_E_HIP;
+  }
+  *out = c;
+  return TIT_OK;
+}
+
+void tit_hip_destroy(tit_hip_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  c->prof.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* tit_hip_last_error(const tit_hip_ctx* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+tit_status tit_hip_upload(tit_hip_ctx* c, const tit_hip_host_view* view,
+                          const tit_hip_tank* tank) {
+  return tit_hip_upload_batch(c, view, tank, 1);
+}
+
+tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
+                                const tit_hip_tank* tanks, size_t n_tanks) {
+  if (!c) return TIT_E_ARG;
+  if (!views || !tanks || n_tanks == 0 || n_tanks > (1u << 20)) {
+    return fail(c, TIT_E_ARG, "need at least one tank");
+  }
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  free_all(c);
+  const int dim = c->prm.dim;
+
+  // Sizes and tank grids.
+  size_t n_total = 0, n_fixed_total = 0;
+  for (size_t t = 0; t < n_tanks; ++t) {
+    const tit_hip_host_view& v = views[t];
+    if (!v.r || !v.rho) return fail(c, TIT_E_ARG, "r and rho are required");
+    if (v.n_fixed > 0 && !v.normal) {
+      return fail(c, TIT_E_ARG, "fixed particles need wall normals");
+    }
+    n_total += v.n_fluid + v.n_fixed;
+    n_fixed_total += v.n_fixed;
+  }
+  if (n_total >= (1ull << 31)) return fail(c, TIT_E_ARG, "too many particles");
+  const double csz = cell_size(c);
+  std::vector<DevTank> ht(n_tanks);
+  uint64_t cells = 0;
+  for (size_t t = 0; t < n_tanks; ++t) {
+    DevTank& d = ht[t];
+    uint64_t tc = 1;
+    for (int k = 0; k < 3; ++k) {
+      d.lo[k] = 0.0;
+      d.inv_c[k] = 0.0;
+      d.nc[k] = 1;
+    }
+    for (int k = 0; k < dim; ++k) {
+      const double lo = tanks[t].lo[k], hi = tanks[t].hi[k];
+      if (!(hi > lo)) return fail(c, TIT_E_ARG, "empty tank box");
+      const double nf = std::ceil((hi - lo) / csz);
+      if (nf > 1e7) return fail(c, TIT_E_ARG, "tank box too large for h");
+      d.nc[k] = std::max(1, static_cast<int>(nf));
+      d.lo[k] = lo;
+      d.inv_c[k] = 1.0 / csz;
+      tc *= static_cast<uint64_t>(d.nc[k]);
+    }
+    d.cell_off = static_cast<uint32_t>(cells);
+    d.n_cells = static_cast<uint32_t>(tc);
+    cells += tc;
+    if (cells >= (1ull << 31)) return fail(c, TIT_E_ARG, "too many cells");
+  }
+  c->n = static_cast<uint32_t>(n_total);
+  c->n_fixed = static_cast<uint32_t>(n_fixed_total);
+  c->n_tanks = static_cast<int>(n_tanks);
+  c->dp.n_tanks = c->n_tanks;
+  c->n_cells = static_cast<uint32_t>(cells);
+  c->key_bits = 1;
+  while ((1ull << c->key_bits) < cells) ++c->key_bits;
+
+  // Host staging in reference layout.
+  std::vector<double> hr(n_total * dim), hv(n_total * dim, 0.0), hrho(n_total),
+      hn(n_total * dim, 0.0), hdv;
+  std::vector<uint32_t> htag(n_total);
+  bool any_dv = false;
+  for (size_t t = 0; t < n_tanks; ++t) any_dv = any_dv || views[t].dv_dt;
+  if (any_dv) hdv.assign(n_total * dim, 0.0);
+  size_t o = 0;
+  for (size_t t = 0; t < n_tanks; ++t) {
+    const tit_hip_host_view& v = views[t];
+    const size_t nt = v.n_fluid + v.n_fixed;
+    std::memcpy(&hr[o * dim], v.r, nt * dim * sizeof(double));
+    if (v.v) std::memcpy(&hv[o * dim], v.v, nt * dim * sizeof(double));
+    std::memcpy(&hrho[o], v.rho, nt * sizeof(double));
+    if (v.dv_dt) std::memcpy(&hdv[o * dim], v.dv_dt, nt * dim * sizeof(double));
+    if (v.n_fixed) {
+      std::memcpy(&hn[(o + v.n_fluid) * dim], v.normal,
+                  v.n_fixed * dim * sizeof(double));
+    }
+    for (size_t a = 0; a < nt; ++a) {
+      htag[o + a] = static_cast<uint32_t>(t << 1) | (a >= v.n_fluid ? 1u : 0u);
+      // fixed particles start with zero velocity (setup_boundary sets v = 0)
+      if (a >= v.n_fluid) {
+        for (int k = 0; k < dim; ++k) hv[(o + a) * dim + k] = 0.0;
+      }
+    }
+    o += nt;
+  }
+  for (double x : hr) {
+    if (!std::isfinite(x)) return fail(c, TIT_E_ARG, "non-finite position");
+  }
+  for (double x : hrho) {
+    if (!(x > 0.0) || !std::isfinite(x)) {
+      return fail(c, TIT_E_ARG, "density must be positive and finite");
+    }
+  }
+
+  // Device buffers.
+  const uint32_t n = c->n;
+  TIT_HIP_TRY(c, alloc_state(c, c->A, n, dim));
+  TIT_HIP_TRY(c, alloc_state(c, c->T, n, dim));
+  TIT_HIP_TRY(c, alloc_state(c, c->B, n, dim));
+  TIT_HIP_TRY(c, alloc_state(c, c->C, n, dim));
+  TIT_HIP_TRY(c, dalloc(c, &c->key, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->key_alt, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->val, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->val_alt, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->id, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->id_alt, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->tag, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->tag_alt, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->cell_off, static_cast<size_t>(c->n_cells) + 1));
+  TIT_HIP_TRY(c, dalloc(c, &c->fixed_list, std::max<uint32_t>(c->n_fixed, 1)));
+  TIT_HIP_TRY(c, dalloc(c, &c->counter, 4));
+  c->sort.max_tiles = radix_sort_tiles(n);
+  TIT_HIP_TRY(c, dalloc(c, &c->sort.hist, 256ull * std::max<uint32_t>(c->sort.max_tiles, 1)));
+  TIT_HIP_TRY(c, dalloc(c, &c->sort.tot, 256));
+  TIT_HIP_TRY(c, dalloc(c, &c->tanks, n_tanks));
+  TIT_HIP_TRY(c, dalloc(c, &c->normal, static_cast<size_t>(n) * dim));
+  TIT_HIP_TRY(c, dalloc(c, &c->dt, n_tanks));
+  TIT_HIP_TRY(c, dalloc(c, &c->time, n_tanks));
+  TIT_HIP_TRY(c, dalloc(c, &c->disp, n_tanks));
+  TIT_HIP_TRY(c, dalloc(c, &c->dt_min, n_tanks));
+  TIT_HIP_TRY(c, dalloc(c, &c->vmax, n_tanks));
+  TIT_HIP_TRY(c, dalloc(c, &c->stage_r, static_cast<size_t>(n) * dim));
+  TIT_HIP_TRY(c, dalloc(c, &c->stage_v, static_cast<size_t>(n) * dim));
+  TIT_HIP_TRY(c, dalloc(c, &c->stage_rho, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->stage_p, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->rhs_drho, n));
+  for (int k = 0; k < dim; ++k) TIT_HIP_TRY(c, dalloc(c, &c->rhs_dv[k], n));
+
+  hipStream_t s = c->stream;
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->stage_r, hr.data(), hr.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->stage_v, hv.data(), hv.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->stage_rho, hrho.data(), hrho.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->normal, hn.data(), hn.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->tag, htag.data(), htag.size() * sizeof(uint32_t),
+                                hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->tanks, ht.data(), ht.size() * sizeof(DevTank),
+                                hipMemcpyHostToDevice, s));
+  launch_import(s, dim, n, c->stage_r, c->stage_v, c->stage_rho, c->A, c->id);
+  std::vector<double> zeros(n_tanks, 0.0);
+  std::vector<unsigned long long> inf(n_tanks, ~0ull);
+  std::vector<uint32_t> z32(4, 0u);
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->time, zeros.data(), n_tanks * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->disp, zeros.data(), n_tanks * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->dt, zeros.data(), n_tanks * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->dt_min, inf.data(), n_tanks * sizeof(unsigned long long),
+                                hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->counter, z32.data(), 4 * sizeof(uint32_t),
+                                hipMemcpyHostToDevice, s));
+  // First step size from the uploaded state and dv/dt (zero by default).
+  double* d_dv = nullptr;
+  if (any_dv) {
+    TIT_HIP_TRY(c, hipMalloc(&d_dv, hdv.size() * sizeof(double)));
+    TIT_HIP_TRY(c, hipMemcpyAsync(d_dv, hdv.data(), hdv.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, s));
+  }
+  launch_time_step(s, c->dp, n, c->A, c->tag, c->id, d_dv, c->dt_min);
+  tit_status st = check_launch(c, "upload");
+  TIT_HIP_TRY(c, hipStreamSynchronize(s));
+  if (d_dv) (void)hipFree(d_dv);
+  if (st != TIT_OK) return st;
+  c->uploaded = true;
+  c->rebuilds = 0;
+  return TIT_OK;
+}
+
+tit_status tit_hip_step(tit_hip_ctx* c, int n_steps, double* dt_last) {
+  if (!c) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "step before upload");
+  if (n_steps < 0) return fail(c, TIT_E_ARG, "negative step count");
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  int done = 0;
+  if (!c->prof.enabled && n_steps >= 2) {
+    // Every rebuild swaps buffer pointers; a graph captured at the other
+    // parity would read the wrong buffers.
+    if (c->graph && c->graph_parity != (c->rebuilds & 1u)) drop_graph(c);
+    if (!c->graph) {
+      c->graph_parity = c->rebuilds & 1u;
+      TIT_HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_step(c);
+      enqueue_step(c);
+      hipError_t e = hipStreamEndCapture(c->stream, &c->graph_def);
+      c->rebuilds -= 2;  // captured, not executed
+      if (e != hipSuccess) {
+        c->graph_def = nullptr;
+        return fail(c, TIT_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      }
+      TIT_HIP_TRY(c, hipGraphInstantiate(&c->graph, c->graph_def, nullptr, nullptr, 0));
+    }
+    for (; done + 2 <= n_steps; done += 2) {
+      TIT_HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+      c->rebuilds += 2;
+    }
+  }
+  for (; done < n_steps; ++done) enqueue_step(c);
+  tit_status st = check_launch(c, "step");
+  if (st != TIT_OK) return st;
+  if (c->prof.enabled) c->prof.collect();
+  st = read_flags(c);
+  if (dt_last) {
+    TIT_HIP_TRY(c, hipMemcpy(dt_last, c->dt, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return st;
+}
+
+tit_status tit_hip_download(tit_hip_ctx* c, double* r, double* v, double* rho,
+                            double* p) {
+  if (!c) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "download before upload");
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  const int dim = c->prm.dim;
+  const size_t n = c->n;
+  launch_export(c->stream, c->dp, c->n, c->A, c->id, c->stage_r, c->stage_v,
+                c->stage_rho, c->stage_p);
+  tit_status st = check_launch(c, "download");
+  if (st != TIT_OK) return st;
+  if (r) TIT_HIP_TRY(c, hipMemcpyAsync(r, c->stage_r, n * dim * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (v) TIT_HIP_TRY(c, hipMemcpyAsync(v, c->stage_v, n * dim * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (rho) TIT_HIP_TRY(c, hipMemcpyAsync(rho, c->stage_rho, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (p) TIT_HIP_TRY(c, hipMemcpyAsync(p, c->stage_p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return TIT_OK;
+}
+
+tit_status tit_hip_tank_clock(tit_hip_ctx* c, double* time, double* dt_next) {
+  if (!c) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "no state");
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (time) {
+    TIT_HIP_TRY(c, hipMemcpy(time, c->time, c->n_tanks * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (dt_next) {
+    std::vector<unsigned long long> b(c->n_tanks);
+    TIT_HIP_TRY(c, hipMemcpy(b.data(), c->dt_min, b.size() * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+    for (int t = 0; t < c->n_tanks; ++t) {
+      double x = 0.0;
+      if (b[t] != ~0ull) std::memcpy(&x, &b[t], sizeof(double));
+      dt_next[t] = x;
+    }
+  }
+  return TIT_OK;
+}
+
+tit_status tit_hip_compute_time_step(tit_hip_ctx* c, double* dt) {
+  return tit_hip_tank_clock(c, nullptr, dt);
+}
+
+tit_status tit_hip_prepare(tit_hip_ctx* c) {
+  if (!c) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "prepare before upload");
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  enqueue_rebuild(c);
+  enqueue_boundary(c, c->A);
+  tit_status st = check_launch(c, "prepare");
+  if (st != TIT_OK) return st;
+  if (c->prof.enabled) c->prof.collect();
+  return read_flags(c);
+}
+
+tit_status tit_hip_compute_rhs(tit_hip_ctx* c, double* drho_dt, double* dv_dt) {
+  if (!c) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "compute_rhs before upload");
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  enqueue_rebuild(c);
+  enqueue_stage(c, c->A, c->A, c->A, kRhsOnly, 1.0, false);
+  // reuse the staging buffers: drho -> stage_rho, dv -> stage_v
+  launch_export_rhs(c->stream, c->prm.dim, c->n, c->id, c->rhs_drho, c->rhs_dv,
+                    c->stage_rho, c->stage_v);
+  tit_status st = check_launch(c, "compute_rhs");
+  if (st != TIT_OK) return st;
+  const size_t n = c->n;
+  if (drho_dt) TIT_HIP_TRY(c, hipMemcpyAsync(drho_dt, c->stage_rho, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (dv_dt) TIT_HIP_TRY(c, hipMemcpyAsync(dv_dt, c->stage_v, n * c->prm.dim * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (c->prof.enabled) c->prof.collect();
+  return read_flags(c);
+}
+
+tit_status tit_hip_neighbors(tit_hip_ctx* c, uint64_t* off, uint32_t* idx,
+                             size_t cap, size_t* total) {
+  if (!c || !off) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "neighbors before upload");
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  enqueue_rebuild(c);
+  const size_t n = c->n;
+  uint64_t* d_cnt = nullptr;
+  TIT_HIP_TRY(c, hipMalloc(&d_cnt, (n + 1) * sizeof(uint64_t)));
+  launch_neighbor_count(c->stream, c->dp, c->tanks, c->n, c->A, c->key, c->tag,
+                        c->id, c->cell_off, d_cnt);
+  std::vector<uint64_t> cnt(n + 1, 0);
+  hipError_t e = hipMemcpyAsync(cnt.data(), d_cnt, n * sizeof(uint64_t),
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(d_cnt);
+    return fail(c, TIT_E_HIP, hipGetErrorString(e));
+  }
+  off[0] = 0;
+  for (size_t a = 0; a < n; ++a) off[a + 1] = off[a] + cnt[a];
+  if (total) *total = off[n];
+  if (off[n] > cap || !idx) {
+    (void)hipFree(d_cnt);
+    return fail(c, TIT_E_ARG, "neighbour capacity too small");
+  }
+  uint32_t* d_idx = nullptr;
+  e = hipMemcpyAsync(d_cnt, off, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                     c->stream);
+  if (e == hipSuccess) e = hipMalloc(&d_idx, std::max<uint64_t>(off[n], 1) * sizeof(uint32_t));
+  if (e == hipSuccess) {
+    launch_neighbor_fill(c->stream, c->dp, c->tanks, c->n, c->A, c->key, c->tag,
+                         c->id, c->cell_off, d_cnt, d_idx);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    e = hipMemcpyAsync(idx, d_idx, off[n] * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_cnt);
+  if (d_idx) (void)hipFree(d_idx);
+  if (e != hipSuccess) return fail(c, TIT_E_HIP, hipGetErrorString(e));
+  return read_flags(c);
+}
+
+tit_status tit_hip_set_profiling(tit_hip_ctx* c, int enable) {
+  if (!c) return TIT_E_ARG;
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->prof.clear();
+  c->prof.enabled = enable != 0;
+  return TIT_OK;
+}
+
+tit_status tit_hip_kernel_stats(tit_hip_ctx* c, tit_hip_kernel_stat* out, int cap,
+                                int* n_out) {
+  if (!c) return TIT_E_ARG;
+  c->prof.collect();
+  const int n = static_cast<int>(c->prof.totals.size());
+  if (n_out) *n_out = n;
+  for (int k = 0; k < n && k < cap && out; ++k) {
+    std::memset(out[k].name, 0, sizeof(out[k].name));
+    std::strncpy(out[k].name, c->prof.totals[k].name.c_str(), sizeof(out[k].name) - 1);
+    out[k].launches = c->prof.totals[k].launches;
+    out[k].total_ms = c->prof.totals[k].total_ms;
+  }
+  return TIT_OK;
+}
+
+tit_status tit_hip_counts(tit_hip_ctx* c, size_t* n_particles, uint64_t* n_rebuilds) {
+  if (!c) return TIT_E_ARG;
+  if (n_particles) *n_particles = c->n;
+  if (n_rebuilds) *n_rebuilds = c->rebuilds;
+  return TIT_OK;
+}
+
+}  // extern "C"

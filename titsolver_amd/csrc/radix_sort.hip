@@ -1,0 +1,195 @@
+// Stable LSD radix sort of (cell key, particle slot) pairs, 8-bit digits.
+//
+// Each pass is three launches:
+//   hist    one workgroup per 4096-key tile counts its digits in LDS;
+//   scan    one workgroup per digit turns the digit-major tile counts into
+//           exclusive tile offsets and the digit total;
+//   scatter each tile ranks its keys stably in LDS and writes them out.
+// Ranks inside a wave come from matching digits with eight ballots (64-lane
+// masks), so equal digits of a nearly sorted key stream cost one LDS update
+// per wave instead of one atomic per lane.  Within a tile the order is wave,
+// then round, then lane, which is the input order: the sort is stable, so
+// particles of one cell keep their previous relative order and the result is
+// deterministic.
+#include "internal.hpp"
+
+namespace tit_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;
+constexpr int kWaveSpan = 64 * kItems;
+
+__device__ __forceinline__ uint64_t match_digit8(uint32_t d, uint64_t valid) {
+  uint64_t peers = valid;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  return peers;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) {
+  return (1ull << lane) - 1ull;
+}
+
+__global__ __launch_bounds__(kThreads) void k_radix_hist(
+    const uint32_t* __restrict__ keys, uint32_t n, int shift,
+    uint32_t* __restrict__ hist, uint32_t ntiles) {
+  __shared__ uint32_t h[256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  h[tid] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kTile + wave * kWaveSpan;
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * 64 + lane;
+    const bool ok = i < n;
+    const uint64_t valid = __ballot(ok);
+    if (valid == 0) break;  // wave-uniform
+    const uint32_t d = ok ? (keys[i] >> shift) & 255u : 0u;
+    const uint64_t peers = match_digit8(d, valid);
+    if (ok && (peers & lanemask_lt(lane)) == 0) {
+      atomicAdd(&h[d], static_cast<uint32_t>(__popcll(peers)));
+    }
+  }
+  __syncthreads();
+  hist[tid * ntiles + blockIdx.x] = h[tid];
+}
+
+__device__ __forceinline__ uint32_t block_inclusive_scan256(uint32_t v,
+                                                            uint32_t* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t o = tid >= off ? sh[tid - off] : 0u;
+    __syncthreads();
+    sh[tid] += o;
+    __syncthreads();
+  }
+  return sh[tid];
+}
+
+__global__ __launch_bounds__(256) void k_radix_digit_scan(
+    uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t* __restrict__ tot) {
+  __shared__ uint32_t sh[256];
+  uint32_t* row = hist + static_cast<size_t>(blockIdx.x) * ntiles;
+  const int tid = threadIdx.x;
+  const uint32_t per = (ntiles + 255u) / 256u;
+  const uint32_t b0 = min(tid * per, ntiles), b1 = min(b0 + per, ntiles);
+  uint32_t s = 0;
+  for (uint32_t k = b0; k < b1; ++k) s += row[k];
+  const uint32_t incl = block_inclusive_scan256(s, sh);
+  uint32_t excl = incl - s;
+  for (uint32_t k = b0; k < b1; ++k) {
+    const uint32_t t = row[k];
+    row[k] = excl;
+    excl += t;
+  }
+  if (tid == 255) tot[blockIdx.x] = incl;
+}
+
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+    uint32_t n, int shift, const uint32_t* __restrict__ hist,
+    const uint32_t* __restrict__ tot, uint32_t ntiles) {
+  __shared__ uint32_t wcnt[kWaves][256];
+  __shared__ uint32_t sh[256];
+  __shared__ uint32_t tile_base[256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
+  const uint32_t t_d = tot[tid];
+  const uint32_t dig_incl = block_inclusive_scan256(t_d, sh);
+  tile_base[tid] = dig_incl - t_d + hist[tid * ntiles + blockIdx.x];
+  __syncthreads();
+
+  uint32_t key[kItems], rank[kItems];
+  const uint32_t base = blockIdx.x * kTile + wave * kWaveSpan;
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * 64 + lane;
+    const bool ok = i < n;
+    const uint64_t valid = __ballot(ok);
+    const uint32_t k = ok ? keys_in[i] : 0u;
+    const uint32_t d = (k >> shift) & 255u;
+    const uint64_t peers = match_digit8(d, valid);
+    const uint64_t below = peers & lanemask_lt(lane);
+    const uint32_t prior = wcnt[wave][d];
+    rank[r] = prior + static_cast<uint32_t>(__popcll(below));
+    if (ok && below == 0) {
+      wcnt[wave][d] = prior + static_cast<uint32_t>(__popcll(peers));
+    }
+    key[r] = k;
+  }
+  __syncthreads();
+  {
+    uint32_t s = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t t = wcnt[w][tid];
+      wcnt[w][tid] = s;
+      s += t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * 64 + lane;
+    if (i < n) {
+      const uint32_t d = (key[r] >> shift) & 255u;
+      const uint32_t pos = tile_base[d] + wcnt[wave][d] + rank[r];
+      keys_out[pos] = key[r];
+      vals_out[pos] = vals_in ? vals_in[i] : i;
+    }
+  }
+}
+
+}  // namespace
+
+uint32_t radix_sort_tiles(uint32_t n) {
+  return (n + kTile - 1) / kTile;
+}
+
+void radix_sort_pairs(hipStream_t stream, Profiler& prof, uint32_t n,
+                      int key_bits, uint32_t** keys, uint32_t** keys_alt,
+                      uint32_t** vals, uint32_t** vals_alt,
+                      const SortScratch& scratch) {
+  if (n == 0) return;
+  const uint32_t ntiles = radix_sort_tiles(n);
+  int passes = (key_bits + 7) / 8;
+  if (passes < 1) passes = 1;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    {
+      ProfScope ps(prof, stream, "radix_hist");
+      k_radix_hist<<<ntiles, kThreads, 0, stream>>>(*keys, n, shift,
+                                                     scratch.hist, ntiles);
+    }
+    {
+      ProfScope ps(prof, stream, "radix_digit_scan");
+      k_radix_digit_scan<<<256, 256, 0, stream>>>(scratch.hist, ntiles,
+                                                  scratch.tot);
+    }
+    {
+      ProfScope ps(prof, stream, "radix_scatter");
+      k_radix_scatter<<<ntiles, kThreads, 0, stream>>>(
+          *keys, p == 0 ? nullptr : *vals, *keys_alt, *vals_alt, n, shift,
+          scratch.hist, scratch.tot, ntiles);
+    }
+    uint32_t* t = *keys;
+    *keys = *keys_alt;
+    *keys_alt = t;
+    t = *vals;
+    *vals = *vals_alt;
+    *vals_alt = t;
+  }
+}
+
+}  // namespace tit_amd

@@ -1,0 +1,787 @@
+// WCSPH step kernels for gfx950.
+//
+// Particles live in cell order (sorted by global cell key).  A fluid target
+// gathers over the 5^d-cell stencil around the cell it was sorted into; with
+// the last axis fastest in the cell key (source/tit/geom/grid.hpp:120-128),
+// every stencil row along that axis is one contiguous particle range.  The
+// inclusive FP64 predicate decides membership, so the neighbour set is the
+// reference's exactly, independent of how the cells were built.
+#include "sph_kernels.hpp"
+
+namespace tit_amd {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline uint32_t grid_for(uint32_t n, int block = kBlock) {
+  return (n + block - 1) / block;
+}
+
+// ---- stencil traversal ------------------------------------------------------
+
+// Calls f(j, d, s) for every particle j != i with s = norm2(d) <= (2h)^2,
+// d = r_j - r_i, in stencil order.
+template <int DIM, class F>
+__device__ __forceinline__ void for_each_neighbor(
+    const DevParams& P, const DevTank& T, const DevState& S,
+    const uint32_t* __restrict__ coff, uint32_t key, uint32_t i,
+    const double* xa, F&& f) {
+  const uint32_t local = key - T.cell_off;
+  int c[3];
+  if constexpr (DIM == 2) {
+    c[0] = static_cast<int>(local / T.nc[1]);
+    c[1] = static_cast<int>(local % T.nc[1]);
+  } else {
+    const uint32_t t = local / T.nc[2];
+    c[2] = static_cast<int>(local % T.nc[2]);
+    c[1] = static_cast<int>(t % T.nc[1]);
+    c[0] = static_cast<int>(t / T.nc[1]);
+  }
+  constexpr int L = DIM - 1;
+  const int zlo = max(c[L] - kStencil, 0);
+  const int zhi = min(c[L] + kStencil, T.nc[L] - 1);
+  const int x0 = max(c[0] - kStencil, 0), x1 = min(c[0] + kStencil, T.nc[0] - 1);
+  int y0 = 0, y1 = 0;
+  if constexpr (DIM == 3) {
+    y0 = max(c[1] - kStencil, 0);
+    y1 = min(c[1] + kStencil, T.nc[1] - 1);
+  }
+  for (int ix = x0; ix <= x1; ++ix) {
+    for (int iy = y0; iy <= y1; ++iy) {
+      uint32_t row;
+      if constexpr (DIM == 2) {
+        row = T.cell_off + static_cast<uint32_t>(ix) * T.nc[1];
+      } else {
+        row = T.cell_off +
+              (static_cast<uint32_t>(ix) * T.nc[1] + static_cast<uint32_t>(iy)) *
+                  T.nc[2];
+      }
+      const uint32_t b = coff[row + zlo], e = coff[row + zhi + 1];
+      for (uint32_t j = b; j < e; ++j) {
+        double d[DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) d[k] = S.x[k][j] - xa[k];
+        double s = d[0] * d[0];
+#pragma unroll
+        for (int k = 1; k < DIM; ++k) s = s + d[k] * d[k];
+        if (s <= P.R2 && j != i) f(j, d, s);
+      }
+    }
+  }
+}
+
+// ---- wave-level reductions to per-tank slots ----------------------------------
+
+__device__ __forceinline__ void wave_atomic_min_u64(unsigned long long* base,
+                                                    uint32_t t,
+                                                    unsigned long long v,
+                                                    bool active) {
+  const uint64_t act = __ballot(active);
+  if (act == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll(static_cast<long long>(act)) - 1;
+  const uint32_t t0 = __shfl(t, leader);
+  const bool same = __ballot(active && t == t0) == act;
+  if (same) {
+    unsigned long long m = active ? v : ~0ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(m, off);
+      m = o < m ? o : m;
+    }
+    if (lane == leader) atomicMin(base + t0, m);
+  } else if (active) {
+    atomicMin(base + t, v);
+  }
+}
+
+__device__ __forceinline__ void wave_atomic_max_u32(uint32_t* base, uint32_t t,
+                                                    uint32_t v, bool active) {
+  const uint64_t act = __ballot(active);
+  if (act == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll(static_cast<long long>(act)) - 1;
+  const uint32_t t0 = __shfl(t, leader);
+  const bool same = __ballot(active && t == t0) == act;
+  if (same) {
+    uint32_t m = active ? v : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint32_t o = __shfl_xor(m, off);
+      m = o > m ? o : m;
+    }
+    if (lane == leader) atomicMax(base + t0, m);
+  } else if (active) {
+    atomicMax(base + t, v);
+  }
+}
+
+// FluidEquations::compute_time_step for one fluid particle
+// (source/tit/sph/fluid_equations.hpp:199-222).
+template <int DIM>
+__device__ __forceinline__ double particle_time_step(const DevParams& P,
+                                                     double rho,
+                                                     const double* v,
+                                                     const double* dv) {
+  double v2 = v[0] * v[0];
+  double a2 = dv[0] * dv[0];
+#pragma unroll
+  for (int k = 1; k < DIM; ++k) {
+    v2 = v2 + v[k] * v[k];
+    a2 = a2 + dv[k] * dv[k];
+  }
+  double dt = P.cfl * P.h / (eos_sound_speed(P, rho) + sqrt(v2));
+  if (P.mu > 0.0) dt = fmin(dt, P.c_visc * (P.h * P.h) * rho / P.mu);
+  const double an = fmax(sqrt(a2), P.g);
+  dt = fmin(dt, P.c_force * sqrt(P.h / an));
+  return dt;
+}
+
+// ---- import / keys / permutation --------------------------------------------
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_import(
+    uint32_t n, const double* __restrict__ r, const double* __restrict__ v,
+    const double* __restrict__ rho, DevState st, uint32_t* __restrict__ id) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    st.x[k][i] = r[static_cast<size_t>(i) * DIM + k];
+    st.v[k][i] = v ? v[static_cast<size_t>(i) * DIM + k] : 0.0;
+  }
+  st.rho[i] = rho[i];
+  id[i] = i;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_cell_keys(
+    DevParams P, const DevTank* __restrict__ tanks, uint32_t n, DevState st,
+    const uint32_t* __restrict__ tag, uint32_t* __restrict__ key,
+    uint32_t* __restrict__ flags) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const DevTank& T = tanks[tag[i] >> 1];
+  uint32_t flat = 0;
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    const double x = st.x[k][i];
+    const double f = (x - T.lo[k]) * T.inv_c[k];
+    int c;
+    if (!(f == f)) {
+      bad = true;
+      c = 0;
+    } else if (f <= 0.0) {
+      c = 0;
+    } else if (f >= static_cast<double>(T.nc[k] - 1)) {
+      c = T.nc[k] - 1;
+    } else {
+      c = static_cast<int>(f);
+    }
+    flat = flat * static_cast<uint32_t>(T.nc[k]) + static_cast<uint32_t>(c);
+  }
+  key[i] = T.cell_off + flat;
+  if (bad) atomicOr(flags, 2u);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_permute(
+    uint32_t n, const uint32_t* __restrict__ perm, DevState src, DevState dst,
+    const uint32_t* __restrict__ id_src, uint32_t* __restrict__ id_dst,
+    const uint32_t* __restrict__ tag_src, uint32_t* __restrict__ tag_dst) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = perm[i];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    dst.x[k][i] = src.x[k][s];
+    dst.v[k][i] = src.v[k][s];
+  }
+  dst.rho[i] = src.rho[s];
+  id_dst[i] = id_src[s];
+  tag_dst[i] = tag_src[s];
+}
+
+// CSR offsets: off[c] = first sorted position with key >= c.
+__global__ __launch_bounds__(kBlock) void k_cell_offsets(
+    uint32_t n_cells, uint32_t n, const uint32_t* __restrict__ key,
+    uint32_t* __restrict__ off) {
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c > n_cells) return;
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (key[mid] < c) {
+      lo = mid + 1;
+    } else {
+      hi = mid;
+    }
+  }
+  off[c] = lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_fixed(
+    uint32_t n, const uint32_t* __restrict__ tag,
+    uint32_t* __restrict__ list, uint32_t* __restrict__ counter) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool fixed = i < n && (tag[i] & 1u);
+  const uint64_t m = __ballot(fixed);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll(static_cast<long long>(m)) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(m)));
+  base = __shfl(base, leader);
+  if (fixed) {
+    list[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+  }
+}
+
+// ---- boundary extrapolation ---------------------------------------------------
+// FluidEquations::setup_boundary (source/tit/sph/fluid_equations.hpp:122-164):
+// Shepard average of H(rho_b) + g (r_be . n_e) n_e[1] over fluid neighbours,
+// solved for rho_e; v_e = 0.  FP64 throughout (fixed particles are few).
+
+template <int DIM, int KERNEL>
+__global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= A.n_fixed) return;
+  const DevParams& P = A.P;
+  const uint32_t i = A.fixed_list[k];
+  const DevTank T = A.tanks[A.tag[i] >> 1];
+  double xe[DIM], ne[DIM];
+  const double* nrm = A.normal + static_cast<size_t>(A.id[i]) * DIM;
+  double n2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    xe[d] = A.s.x[d][i];
+    ne[d] = nrm[d];
+  }
+  n2 = ne[0] * ne[0];
+#pragma unroll
+  for (int d = 1; d < DIM; ++d) n2 = n2 + ne[d] * ne[d];
+  const double tiny = 6.0554544523933395e-06;  // cbrt(DBL_EPSILON)
+  if (n2 >= tiny * tiny) {
+    const double s = sqrt(n2);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) ne[d] = ne[d] / s;
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) ne[d] = 0.0;
+  }
+  // W_be = omega / h^d * w(q), omega of the kernel in this dimension.
+  const double omega = KERNEL == kKernelC2
+                           ? (DIM == 2 ? 7.0 / (4.0 * M_PI) : 21.0 / (16.0 * M_PI))
+                           : (DIM == 2 ? 9.0 / (4.0 * M_PI) : 495.0 / (256.0 * M_PI));
+  const double h_inv = 1.0 / P.h;
+  double w0 = omega;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) w0 *= h_inv;
+  double S = 0.0, H = 0.0;
+  for_each_neighbor<DIM>(P, T, A.s, A.cell_off, A.key[i], i, xe,
+                         [&](uint32_t j, const double* dd, double s2) {
+                           if (A.tag[j] & 1u) return;
+                           const double rho_b = A.s.rho[j];
+                           const double V_b = P.m / rho_b;
+                           const double W =
+                               w0 * kernel_value_poly_f64<KERNEL>(h_inv * sqrt(s2));
+                           double rn = dd[0] * ne[0];
+#pragma unroll
+                           for (int d = 1; d < DIM; ++d) rn = rn + dd[d] * ne[d];
+                           const double H_b = eos_potential(P, rho_b);
+                           S += V_b * W;
+                           H += V_b * (H_b + P.g * rn * ne[1]) * W;
+                         });
+  const double Hs = fabs(S) <= tiny ? 0.0 : H / S;
+  A.s.rho[i] = eos_density_from_potential(P, Hs);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) A.s.v[d][i] = 0.0;
+}
+
+// ---- fused stage ----------------------------------------------------------------
+// Continuity + momentum right-hand sides (fluid_equations.hpp:232-305, wall
+// terms absent) gathered per fluid particle, then the SSPRK substep update
+// and Shu-Osher combination (time_integrator.hpp:188-223) in FP64.  The last
+// stage of a step also folds the next step's CFL candidates (the reference
+// computes them from this state and this stage's dv/dt at the start of the
+// next step, fluid_equations.hpp:199-222).
+
+template <int DIM, int KERNEL, int FLAVOUR, int KIND, bool FINAL>
+__global__ __launch_bounds__(kBlock) void k_stage(StageArgs A) {
+  const DevParams& P = A.P;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool in_range = i < A.n;
+  const uint32_t tag = in_range ? A.tag[i] : 1u;
+  const uint32_t t = tag >> 1;
+  const bool fluid = in_range && !(tag & 1u);
+
+  double dt_cand = 0.0;
+  float vin = 0.0f;
+  if (in_range && !fluid) {
+    // Fixed particles keep r; v = 0 and rho come from the extrapolation.
+    if constexpr (KIND == kRhsOnly) {
+      A.rhs_drho[i] = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) A.rhs_dv[k][i] = 0.0;
+    } else {
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        A.out.x[k][i] = A.in.x[k][i];
+        A.out.v[k][i] = 0.0;
+      }
+      A.out.rho[i] = A.in.rho[i];
+    }
+  }
+  if (fluid) {
+    const DevTank T = A.tanks[t];
+    double xa[DIM], va[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      xa[k] = A.in.x[k][i];
+      va[k] = A.in.v[k][i];
+    }
+    const double rho_a = A.in.rho[i];
+    const double pa = eos_pressure(P, rho_a);
+    const float Pa = static_cast<float>(pa / (rho_a * rho_a));
+    const double csa64 = eos_sound_speed(P, rho_a);
+    const float csa = static_cast<float>(csa64);
+    const float rho_af = static_cast<float>(rho_a);
+
+    Kahan acc_rho, acc_v[DIM];
+    acc_rho.init();
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) acc_v[k].init();
+
+    for_each_neighbor<DIM>(
+        P, T, A.in, A.cell_off, A.key[i], i, xa,
+        [&](uint32_t j, const double* d, double s2) {
+          float x[DIM], vab[DIM];
+          float vr = 0.0f;
+#pragma unroll
+          for (int k = 0; k < DIM; ++k) {
+            x[k] = static_cast<float>(-d[k]);
+            vab[k] = static_cast<float>(va[k] - A.in.v[k][j]);
+            vr = __builtin_fmaf(vab[k], x[k], vr);
+          }
+          const float r2 = static_cast<float>(s2);
+          const float r = sqrtf(r2);
+          const float q = r * P.inv_h;
+          const float G = P.gsc * kernel_grad_poly<KERNEL>(q);
+          const double rho_b = A.in.rho[j];
+          const double pb = eos_pressure(P, rho_b);
+          const float Pb = static_cast<float>(pb / (rho_b * rho_b));
+          const float rho_bf = static_cast<float>(rho_b);
+          const float drab = static_cast<float>(rho_a - rho_b);
+          // Continuity: m_b v_ab . grad W_ab plus density diffusion.
+          float cont = P.m_f * G * vr;
+          float Pi;
+          if constexpr (FLAVOUR == kFlavourCfg) {
+            // delta h c0 V_b psi_ab . grad W_ab, psi_ab = 2 rho_ab r_ab/|r_ab|^2
+            cont = __builtin_fmaf(P.dhc2 * (P.m_f / rho_bf), drab * G, cont);
+            Pi = 0.0f;
+            if (vr < 0.0f) {
+              const float csb = static_cast<float>(eos_sound_speed(P, rho_b));
+              const float cbar = 0.5f * (csa + csb);
+              const float rbar = 0.5f * (rho_af + rho_bf);
+              Pi = P.ahc * cbar * vr / (rbar * (r2 + P.eta2));
+            }
+          } else {
+            // Ferrari: m_b max(cs_a, cs_b) rho_ab / rho_b (r_ab . grad W)/|r_ab|
+            const float csb = static_cast<float>(eos_sound_speed(P, rho_b));
+            const float csab = fmaxf(csa, csb);
+            cont = __builtin_fmaf(P.m_f * csab * drab / rho_bf, G * r, cont);
+            Pi = P.two_mu * vr / (rho_af * rho_bf * r2);
+          }
+          acc_rho.add(cont);
+          const float coef = P.m_f * (Pi - (Pa + Pb)) * G;
+#pragma unroll
+          for (int k = 0; k < DIM; ++k) acc_v[k].add(coef * x[k]);
+        });
+
+    double drho = static_cast<double>(acc_rho.value());
+    double dv[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) dv[k] = static_cast<double>(acc_v[k].value());
+    dv[1] = dv[1] - P.g;
+
+    if constexpr (KIND == kRhsOnly) {
+      A.rhs_drho[i] = drho;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) A.rhs_dv[k][i] = dv[k];
+    } else {
+      double v2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) v2 = v2 + va[k] * va[k];
+      vin = static_cast<float>(sqrt(v2));
+      const double h = A.dt[t];
+      double rn[DIM], vn[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        rn[k] = xa[k] + h * va[k];
+        vn[k] = va[k] + h * dv[k];
+      }
+      double rhon = rho_a + h * drho;
+      if constexpr (KIND == kCombine) {
+        const double w = A.w, w1 = 1.0 - A.w;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+          rn[k] = w1 * A.old.x[k][i] + w * rn[k];
+          vn[k] = w1 * A.old.v[k][i] + w * vn[k];
+        }
+        rhon = w1 * A.old.rho[i] + w * rhon;
+      }
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        A.out.x[k][i] = rn[k];
+        A.out.v[k][i] = vn[k];
+      }
+      A.out.rho[i] = rhon;
+      if constexpr (FINAL) dt_cand = particle_time_step<DIM>(P, rhon, vn, dv);
+    }
+  }
+  if constexpr (KIND != kRhsOnly) {
+    wave_atomic_max_u32(A.vmax, t, __float_as_uint(vin), fluid);
+    if constexpr (FINAL) {
+      wave_atomic_min_u64(A.dt_min, t, dbl_bits(dt_cand), fluid);
+    }
+  }
+}
+
+// Time step of the current state with a given dv/dt (reference order) or
+// with dv/dt = 0.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_time_step(
+    DevParams P, uint32_t n, DevState st, const uint32_t* __restrict__ tag,
+    const uint32_t* __restrict__ id, const double* __restrict__ dv_ref,
+    unsigned long long* __restrict__ dt_min) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool in_range = i < n;
+  const uint32_t tg = in_range ? tag[i] : 1u;
+  const bool fluid = in_range && !(tg & 1u);
+  double dt = 0.0;
+  if (fluid) {
+    double v[DIM], dv[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      v[k] = st.v[k][i];
+      dv[k] = dv_ref ? dv_ref[static_cast<size_t>(id[i]) * DIM + k] : 0.0;
+    }
+    dt = particle_time_step<DIM>(P, st.rho[i], v, dv);
+  }
+  wave_atomic_min_u64(dt_min, tg >> 1, dbl_bits(dt), fluid);
+}
+
+__global__ void k_step_begin(int n_tanks, unsigned long long* dt_min,
+                             double* dt, uint32_t* vmax) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tanks) return;
+  const unsigned long long b = dt_min[t];
+  // A tank without fluid keeps dt = 0 (nothing moves).
+  dt[t] = b == ~0ull ? 0.0 : __longlong_as_double(static_cast<long long>(b));
+  dt_min[t] = ~0ull;
+  vmax[t] = 0u;
+}
+
+__global__ void k_step_end(int n_tanks, const double* dt, const uint32_t* vmax,
+                           double* time, double* disp, double skin_half,
+                           uint32_t* flags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tanks) return;
+  time[t] += dt[t];
+  const double d = disp[t] + dt[t] * static_cast<double>(__uint_as_float(vmax[t]));
+  disp[t] = d;
+  if (!(d <= skin_half)) atomicOr(flags, 1u);
+}
+
+__global__ void k_reset_disp(int n_tanks, double* disp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_tanks) disp[t] = 0.0;
+}
+
+// ---- export ---------------------------------------------------------------------
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_export(
+    DevParams P, uint32_t n, DevState st, const uint32_t* __restrict__ id,
+    double* __restrict__ r, double* __restrict__ v, double* __restrict__ rho,
+    double* __restrict__ p) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const size_t o = id[i];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    r[o * DIM + k] = st.x[k][i];
+    v[o * DIM + k] = st.v[k][i];
+  }
+  rho[o] = st.rho[i];
+  p[o] = eos_pressure(P, st.rho[i]);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_export_rhs(
+    uint32_t n, const uint32_t* __restrict__ id, const double* __restrict__ drho,
+    const double* __restrict__ dv0, const double* __restrict__ dv1,
+    const double* __restrict__ dv2, double* __restrict__ drho_ref,
+    double* __restrict__ dv_ref) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const size_t o = id[i];
+  drho_ref[o] = drho[i];
+  dv_ref[o * DIM + 0] = dv0[i];
+  dv_ref[o * DIM + 1] = dv1[i];
+  if constexpr (DIM == 3) dv_ref[o * DIM + 2] = dv2[i];
+}
+
+// ---- neighbour lists (test hook) ----------------------------------------------
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_neighbor_count(
+    DevParams P, const DevTank* __restrict__ tanks, uint32_t n, DevState st,
+    const uint32_t* __restrict__ key, const uint32_t* __restrict__ tag,
+    const uint32_t* __restrict__ id, const uint32_t* __restrict__ coff,
+    uint64_t* __restrict__ count_ref) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const DevTank T = tanks[tag[i] >> 1];
+  double xa[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) xa[k] = st.x[k][i];
+  uint64_t c = 1;  // self
+  for_each_neighbor<DIM>(P, T, st, coff, key[i], i, xa,
+                         [&](uint32_t, const double*, double) { ++c; });
+  count_ref[id[i]] = c;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_neighbor_fill(
+    DevParams P, const DevTank* __restrict__ tanks, uint32_t n, DevState st,
+    const uint32_t* __restrict__ key, const uint32_t* __restrict__ tag,
+    const uint32_t* __restrict__ id, const uint32_t* __restrict__ coff,
+    const uint64_t* __restrict__ off_ref, uint32_t* __restrict__ idx_ref) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const DevTank T = tanks[tag[i] >> 1];
+  double xa[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) xa[k] = st.x[k][i];
+  const uint32_t self = id[i];
+  uint32_t* out = idx_ref + off_ref[self];
+  uint32_t c = 0;
+  out[c++] = self;
+  for_each_neighbor<DIM>(P, T, st, coff, key[i], i, xa,
+                         [&](uint32_t j, const double*, double) { out[c++] = id[j]; });
+  // ascending order (particle_mesh.hpp:146)
+  for (uint32_t a = 1; a < c; ++a) {
+    const uint32_t v = out[a];
+    uint32_t b = a;
+    while (b > 0 && out[b - 1] > v) {
+      out[b] = out[b - 1];
+      --b;
+    }
+    out[b] = v;
+  }
+}
+
+// ---- dispatch helpers --------------------------------------------------------------
+
+template <int DIM, int KERNEL, int FLAVOUR>
+void stage_dispatch(hipStream_t s, const StageArgs& a, int kind, bool final) {
+  const uint32_t g = grid_for(a.n);
+  if (kind == kRhsOnly) {
+    k_stage<DIM, KERNEL, FLAVOUR, kRhsOnly, false><<<g, kBlock, 0, s>>>(a);
+  } else if (kind == kEuler) {
+    if (final) {
+      k_stage<DIM, KERNEL, FLAVOUR, kEuler, true><<<g, kBlock, 0, s>>>(a);
+    } else {
+      k_stage<DIM, KERNEL, FLAVOUR, kEuler, false><<<g, kBlock, 0, s>>>(a);
+    }
+  } else {
+    if (final) {
+      k_stage<DIM, KERNEL, FLAVOUR, kCombine, true><<<g, kBlock, 0, s>>>(a);
+    } else {
+      k_stage<DIM, KERNEL, FLAVOUR, kCombine, false><<<g, kBlock, 0, s>>>(a);
+    }
+  }
+}
+
+template <int DIM, int KERNEL>
+void stage_dispatch_flavour(hipStream_t s, const StageArgs& a, int kind,
+                            bool final) {
+  if (a.P.flavour == kFlavourCfg) {
+    stage_dispatch<DIM, KERNEL, kFlavourCfg>(s, a, kind, final);
+  } else {
+    stage_dispatch<DIM, KERNEL, kFlavourRef>(s, a, kind, final);
+  }
+}
+
+template <int DIM>
+void stage_dispatch_kernel(hipStream_t s, const StageArgs& a, int kind,
+                           bool final) {
+  if (a.P.kernel == kKernelC4) {
+    stage_dispatch_flavour<DIM, kKernelC4>(s, a, kind, final);
+  } else {
+    stage_dispatch_flavour<DIM, kKernelC2>(s, a, kind, final);
+  }
+}
+
+}  // namespace
+
+void launch_import(hipStream_t s, int dim, uint32_t n, const double* r,
+                   const double* v, const double* rho, DevState st,
+                   uint32_t* id) {
+  if (n == 0) return;
+  if (dim == 2) {
+    k_import<2><<<grid_for(n), kBlock, 0, s>>>(n, r, v, rho, st, id);
+  } else {
+    k_import<3><<<grid_for(n), kBlock, 0, s>>>(n, r, v, rho, st, id);
+  }
+}
+
+void launch_cell_keys(hipStream_t s, const DevParams& P, const DevTank* tanks,
+                      uint32_t n, DevState st, const uint32_t* tag,
+                      uint32_t* key, uint32_t* flags) {
+  if (n == 0) return;
+  if (P.dim == 2) {
+    k_cell_keys<2><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, tag, key, flags);
+  } else {
+    k_cell_keys<3><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, tag, key, flags);
+  }
+}
+
+void launch_permute(hipStream_t s, int dim, uint32_t n, const uint32_t* perm,
+                    DevState src, DevState dst, const uint32_t* id_src,
+                    uint32_t* id_dst, const uint32_t* tag_src,
+                    uint32_t* tag_dst) {
+  if (n == 0) return;
+  if (dim == 2) {
+    k_permute<2><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst, id_src,
+                                                id_dst, tag_src, tag_dst);
+  } else {
+    k_permute<3><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst, id_src,
+                                                id_dst, tag_src, tag_dst);
+  }
+}
+
+void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
+                         const uint32_t* key, uint32_t* cell_off) {
+  k_cell_offsets<<<grid_for(n_cells + 1), kBlock, 0, s>>>(n_cells, n, key,
+                                                          cell_off);
+}
+
+void launch_compact_fixed(hipStream_t s, uint32_t n, const uint32_t* tag,
+                          uint32_t* fixed_list, uint32_t* counter) {
+  if (n == 0) return;
+  k_compact_fixed<<<grid_for(n), kBlock, 0, s>>>(n, tag, fixed_list, counter);
+}
+
+void launch_boundary(hipStream_t s, const BoundaryArgs& a) {
+  if (a.n_fixed == 0) return;
+  const uint32_t g = grid_for(a.n_fixed);
+  if (a.P.dim == 2) {
+    if (a.P.kernel == kKernelC4) {
+      k_boundary<2, kKernelC4><<<g, kBlock, 0, s>>>(a);
+    } else {
+      k_boundary<2, kKernelC2><<<g, kBlock, 0, s>>>(a);
+    }
+  } else {
+    if (a.P.kernel == kKernelC4) {
+      k_boundary<3, kKernelC4><<<g, kBlock, 0, s>>>(a);
+    } else {
+      k_boundary<3, kKernelC2><<<g, kBlock, 0, s>>>(a);
+    }
+  }
+}
+
+void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final) {
+  if (a.n == 0) return;
+  if (a.P.dim == 2) {
+    stage_dispatch_kernel<2>(s, a, kind, final);
+  } else {
+    stage_dispatch_kernel<3>(s, a, kind, final);
+  }
+}
+
+void launch_time_step(hipStream_t s, const DevParams& P, uint32_t n,
+                      DevState st, const uint32_t* tag, const uint32_t* id,
+                      const double* dv_ref, unsigned long long* dt_min) {
+  if (n == 0) return;
+  if (P.dim == 2) {
+    k_time_step<2><<<grid_for(n), kBlock, 0, s>>>(P, n, st, tag, id, dv_ref, dt_min);
+  } else {
+    k_time_step<3><<<grid_for(n), kBlock, 0, s>>>(P, n, st, tag, id, dv_ref, dt_min);
+  }
+}
+
+void launch_step_begin(hipStream_t s, int n_tanks, unsigned long long* dt_min,
+                       double* dt, uint32_t* vmax) {
+  k_step_begin<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, dt_min, dt, vmax);
+}
+
+void launch_step_end(hipStream_t s, int n_tanks, const double* dt,
+                     const uint32_t* vmax, double* time, double* disp,
+                     double skin_half, uint32_t* flags) {
+  k_step_end<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, dt, vmax, time, disp,
+                                                   skin_half, flags);
+}
+
+void launch_reset_disp(hipStream_t s, int n_tanks, double* disp) {
+  k_reset_disp<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, disp);
+}
+
+void launch_export(hipStream_t s, const DevParams& P, uint32_t n, DevState st,
+                   const uint32_t* id, double* r, double* v, double* rho,
+                   double* p) {
+  if (n == 0) return;
+  if (P.dim == 2) {
+    k_export<2><<<grid_for(n), kBlock, 0, s>>>(P, n, st, id, r, v, rho, p);
+  } else {
+    k_export<3><<<grid_for(n), kBlock, 0, s>>>(P, n, st, id, r, v, rho, p);
+  }
+}
+
+void launch_export_rhs(hipStream_t s, int dim, uint32_t n, const uint32_t* id,
+                       const double* drho, double* const* dv, double* drho_ref,
+                       double* dv_ref) {
+  if (n == 0) return;
+  if (dim == 2) {
+    k_export_rhs<2><<<grid_for(n), kBlock, 0, s>>>(n, id, drho, dv[0], dv[1],
+                                                   nullptr, drho_ref, dv_ref);
+  } else {
+    k_export_rhs<3><<<grid_for(n), kBlock, 0, s>>>(n, id, drho, dv[0], dv[1],
+                                                   dv[2], drho_ref, dv_ref);
+  }
+}
+
+void launch_neighbor_count(hipStream_t s, const DevParams& P,
+                           const DevTank* tanks, uint32_t n, DevState st,
+                           const uint32_t* key, const uint32_t* tag,
+                           const uint32_t* id, const uint32_t* cell_off,
+                           uint64_t* count_ref) {
+  if (n == 0) return;
+  if (P.dim == 2) {
+    k_neighbor_count<2><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, key, tag,
+                                                       id, cell_off, count_ref);
+  } else {
+    k_neighbor_count<3><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, key, tag,
+                                                       id, cell_off, count_ref);
+  }
+}
+
+void launch_neighbor_fill(hipStream_t s, const DevParams& P,
+                          const DevTank* tanks, uint32_t n, DevState st,
+                          const uint32_t* key, const uint32_t* tag,
+                          const uint32_t* id, const uint32_t* cell_off,
+                          const uint64_t* off_ref, uint32_t* idx_ref) {
+  if (n == 0) return;
+  if (P.dim == 2) {
+    k_neighbor_fill<2><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, key, tag,
+                                                      id, cell_off, off_ref, idx_ref);
+  } else {
+    k_neighbor_fill<3><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, key, tag,
+                                                      id, cell_off, off_ref, idx_ref);
+  }
+}
+
+}  // namespace tit_amd

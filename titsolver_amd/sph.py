@@ -1,0 +1,282 @@
+"""Host-side mirror of TitSolver's tit::sph objects over the C ABI.
+
+`FluidEquations` carries the constructor arguments of the reference's
+FluidEquations + TaitEquationOfState + kernel (source/tit/sph/
+fluid_equations.hpp:61-69, equation_of_state.hpp:27-34); `ParticleArray`
+holds particles in the reference layout (fluid first, then fixed;
+particle_array.hpp:138-293); `SSPRKIntegrator.step` advances a device-resident
+array the way SSPRKIntegrator::step does (time_integrator.hpp:159-184) and
+returns the step size.  Every computation runs in libtit_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from ._abi import (FLAVOUR_CFG, FLAVOUR_REF, KERNEL_WENDLAND_C2,
+                   KERNEL_WENDLAND_C4)
+
+
+class TitError(RuntimeError):
+    """Raised for a non-OK tit_status (the role of tit::Exception)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_abi.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def _dp(a):
+    if a is None:
+        return C.cast(None, C.POINTER(C.c_double))
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class FluidEquations:
+    """Physical parameters of the WCSPH right-hand sides."""
+
+    dim: int = 2
+    kernel: int = KERNEL_WENDLAND_C2
+    flavour: int = FLAVOUR_CFG
+    h: float = 0.04
+    m: float = 0.4
+    rho0: float = 1000.0
+    cs0: float = 62.64
+    xi: float = 7.0
+    g: float = 9.81
+    mu: float = 0.0
+    alpha: float = 0.02
+    delta: float = 0.1
+    cfl: float = 0.25
+    c_force: float = 0.25
+    c_visc: float = 0.125
+
+    def to_params(self, order: int = 3, skin: float = 0.0) -> _abi.Params:
+        p = _abi.Params()
+        p.dim, p.kernel, p.flavour, p.ssprk_order = (self.dim, self.kernel,
+                                                    self.flavour, order)
+        for name in ("h", "m", "rho0", "cs0", "xi", "g", "mu", "alpha",
+                     "delta", "cfl", "c_force", "c_visc"):
+            setattr(p, name, float(getattr(self, name)))
+        p.skin = float(skin)
+        return p
+
+
+@dataclass
+class ParticleArray:
+    """Particles of one tank in the reference layout (fluid, then fixed)."""
+
+    dim: int
+    r: np.ndarray            # (n, dim)
+    rho: np.ndarray          # (n,)
+    n_fluid: int
+    normal: np.ndarray       # (n_fixed, dim) wall normals of fixed particles
+    box_lo: tuple = ()
+    box_hi: tuple = ()
+    v: np.ndarray | None = None
+    dv_dt: np.ndarray | None = None
+
+    @property
+    def n(self) -> int:
+        return self.r.shape[0]
+
+    @property
+    def n_fixed(self) -> int:
+        return self.n - self.n_fluid
+
+    def fluid(self) -> slice:
+        return slice(0, self.n_fluid)
+
+    def fixed(self) -> slice:
+        return slice(self.n_fluid, self.n)
+
+    def copy(self) -> "ParticleArray":
+        return ParticleArray(self.dim, self.r.copy(), self.rho.copy(), self.n_fluid,
+                             self.normal.copy(), self.box_lo, self.box_hi,
+                             None if self.v is None else self.v.copy(),
+                             None if self.dv_dt is None else self.dv_dt.copy())
+
+
+def dam_break(dim: int, dr: float, tank, column, layers: int = 3,
+              rho0: float = 1000.0, g: float = 9.81, cs0: float = 62.64,
+              margin: float | None = None) -> ParticleArray:
+    """Lattice dam break built by libtit_hip's host-side set-up."""
+    lib = _abi.load()
+    tank_a = np.asarray(tank, dtype=np.float64)
+    col_a = np.asarray(column, dtype=np.float64)
+    nf, nx = C.c_size_t(), C.c_size_t()
+    st = lib.tit_dam_break_count(dim, dr, _dp(tank_a), _dp(col_a), layers,
+                                 C.byref(nf), C.byref(nx))
+    if st != 0:
+        raise TitError(st, "dam break set-up rejected its arguments")
+    n = nf.value + nx.value
+    r = np.zeros((n, dim))
+    rho = np.zeros(n)
+    normal = np.zeros((max(nx.value, 1), dim))
+    st = lib.tit_dam_break_fill(dim, dr, _dp(tank_a), _dp(col_a), layers, rho0,
+                                g, cs0, _dp(r), _dp(rho), _dp(normal))
+    if st != 0:
+        raise TitError(st, "dam break fill failed")
+    if margin is None:
+        margin = 4.0 * dr
+    lo = [-(layers * dr) - margin] * dim
+    hi = [float(tank_a[k]) + layers * dr + margin for k in range(dim)]
+    return ParticleArray(dim, r, rho, nf.value, normal[: nx.value], tuple(lo),
+                         tuple(hi))
+
+
+class HipContext:
+    """One tit_hip_ctx: device buffers, stream and captured step graph."""
+
+    def __init__(self, equations: FluidEquations, order: int = 3,
+                 device: int = 0, skin: float = 0.0):
+        self.lib = _abi.load()
+        self.eq = equations
+        self.params = equations.to_params(order, skin)
+        h = C.c_void_p()
+        st = self.lib.tit_hip_create(device, C.byref(self.params), C.byref(h))
+        if st != 0:
+            raise TitError(st, "tit_hip_create failed (is a GPU visible?)")
+        self.h = h
+        self.sizes: list[tuple[int, int]] = []
+
+    def _check(self, st: int):
+        if st != 0:
+            raise TitError(st, self.lib.tit_hip_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.tit_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, arrays: list[ParticleArray]):
+        views = (_abi.HostView * len(arrays))()
+        tanks = (_abi.Tank * len(arrays))()
+        keep = []
+        for t, pa in enumerate(arrays):
+            assert pa.dim == self.eq.dim
+            r = np.ascontiguousarray(pa.r, dtype=np.float64)
+            rho = np.ascontiguousarray(pa.rho, dtype=np.float64)
+            v = None if pa.v is None else np.ascontiguousarray(pa.v, dtype=np.float64)
+            dv = None if pa.dv_dt is None else np.ascontiguousarray(pa.dv_dt, dtype=np.float64)
+            nrm = np.ascontiguousarray(pa.normal, dtype=np.float64)
+            keep += [r, rho, v, dv, nrm]
+            views[t].r, views[t].rho, views[t].v = _dp(r), _dp(rho), _dp(v)
+            views[t].dv_dt, views[t].normal = _dp(dv), _dp(nrm)
+            views[t].n_fluid, views[t].n_fixed = pa.n_fluid, pa.n_fixed
+            for k in range(pa.dim):
+                tanks[t].lo[k] = pa.box_lo[k]
+                tanks[t].hi[k] = pa.box_hi[k]
+        self._check(self.lib.tit_hip_upload_batch(self.h, views, tanks, len(arrays)))
+        self.sizes = [(pa.n_fluid, pa.n_fixed) for pa in arrays]
+        self.n = sum(a + b for a, b in self.sizes)
+
+    def step(self, n_steps: int = 1) -> float:
+        dt = C.c_double()
+        self._check(self.lib.tit_hip_step(self.h, n_steps, C.byref(dt)))
+        return dt.value
+
+    def download(self) -> dict:
+        d = self.eq.dim
+        r, v = np.empty((self.n, d)), np.empty((self.n, d))
+        rho, p = np.empty(self.n), np.empty(self.n)
+        self._check(self.lib.tit_hip_download(self.h, _dp(r), _dp(v), _dp(rho), _dp(p)))
+        return {"r": r, "v": v, "rho": rho, "p": p}
+
+    def tank_clock(self):
+        t = np.empty(len(self.sizes))
+        dt = np.empty(len(self.sizes))
+        self._check(self.lib.tit_hip_tank_clock(self.h, _dp(t), _dp(dt)))
+        return t, dt
+
+    def prepare(self):
+        self._check(self.lib.tit_hip_prepare(self.h))
+
+    def compute_time_step(self) -> np.ndarray:
+        dt = np.empty(len(self.sizes))
+        self._check(self.lib.tit_hip_compute_time_step(self.h, _dp(dt)))
+        return dt
+
+    def compute_rhs(self):
+        drho = np.empty(self.n)
+        dv = np.empty((self.n, self.eq.dim))
+        self._check(self.lib.tit_hip_compute_rhs(self.h, _dp(drho), _dp(dv)))
+        return drho, dv
+
+    def neighbors(self):
+        off = np.zeros(self.n + 1, dtype=np.uint64)
+        total = C.c_size_t()
+        st = self.lib.tit_hip_neighbors(
+            self.h, off.ctypes.data_as(C.POINTER(C.c_uint64)),
+            C.cast(None, C.POINTER(C.c_uint32)), 0, C.byref(total))
+        if st not in (0, _abi.TIT_E_ARG):
+            self._check(st)
+        idx = np.zeros(max(total.value, 1), dtype=np.uint32)
+        self._check(self.lib.tit_hip_neighbors(
+            self.h, off.ctypes.data_as(C.POINTER(C.c_uint64)),
+            idx.ctypes.data_as(C.POINTER(C.c_uint32)), idx.size, C.byref(total)))
+        return off, idx[: total.value]
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.tit_hip_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_stats(self) -> dict:
+        cap = 64
+        arr = (_abi.KernelStat * cap)()
+        n = C.c_int()
+        self._check(self.lib.tit_hip_kernel_stats(self.h, arr, cap, C.byref(n)))
+        return {arr[k].name.decode(): (arr[k].launches, arr[k].total_ms)
+                for k in range(min(n.value, cap))}
+
+    def counts(self):
+        n = C.c_size_t()
+        rb = C.c_uint64()
+        self._check(self.lib.tit_hip_counts(self.h, C.byref(n), C.byref(rb)))
+        return n.value, rb.value
+
+
+class SSPRKIntegrator:
+    """SSPRKIntegrator::step on device-resident particles."""
+
+    def __init__(self, equations: FluidEquations, order: int = 3):
+        self.equations = equations
+        self.order = order
+
+    def step(self, mesh: HipContext, n_steps: int = 1) -> float:
+        return mesh.step(n_steps)
+
+
+def cfg_equations(dim: int, dr: float, column_height: float,
+                  kernel: int = KERNEL_WENDLAND_C2, h_factor: float = 2.0,
+                  flavour: int = FLAVOUR_CFG) -> FluidEquations:
+    """Parameters of the dam-break configurations: Wendland C2, h = h_factor dr,
+    Tait rho0 = 1000, xi = 7, cs0 = 10 sqrt(2 g H), alpha = 0.02, delta = 0.1,
+    CFL 0.25."""
+    g = 9.81
+    return FluidEquations(dim=dim, kernel=kernel, flavour=flavour,
+                          h=h_factor * dr, m=1000.0 * dr ** dim, rho0=1000.0,
+                          cs0=10.0 * math.sqrt(2.0 * g * column_height), xi=7.0,
+                          g=g, mu=0.0, alpha=0.02, delta=0.1, cfl=0.25,
+                          c_force=0.25, c_visc=0.125)
+
+
+def ref_equations(dim: int, dr: float, column_height: float) -> FluidEquations:
+    """Parameters of titwcsph (wcsph.cpp:47-52, :87-89): Wendland C4, h = 2 dr,
+    cs0 = 20 sqrt(g H), mu = 1e-3, Ferrari diffusion, CFL 0.4."""
+    g = 9.81
+    return FluidEquations(dim=dim, kernel=KERNEL_WENDLAND_C4, flavour=FLAVOUR_REF,
+                          h=2.0 * dr, m=1000.0 * dr ** dim, rho0=1000.0,
+                          cs0=20.0 * math.sqrt(g * column_height), xi=7.0, g=g,
+                          mu=1e-3, alpha=0.0, delta=0.0, cfl=0.4, c_force=0.25,
+                          c_visc=0.125)
