@@ -158,7 +158,8 @@ def cpu_baseline(pa, eq, steps):
     g = ctx.download()
     ctx.close()
     rho_err = float(np.max(np.abs(g["rho"][:nf] - st.rho[:nf]) / st.rho[:nf]))
-    p_err = float(np.max(np.abs(g["p"][:nf] - st.p[:nf])) / np.abs(st.p[:nf]).max())
+    p_o = oracle.pressure(prm, st.rho[:nf])
+    p_err = float(np.max(np.abs(g["p"][:nf] - p_o)) / np.abs(p_o).max())
     front_g = float(g["r"][:nf, 0].max())
     front_o = float(st.r[:nf, 0].max())
     parity = {"steps": steps, "rho_max_rel_err": rho_err,

@@ -78,6 +78,7 @@ def load():
         "orc_ssprk_generic": (C.c_int, [C.c_int, C.c_double, C.c_size_t, _DP,
                                         C.c_void_p, C.c_void_p]),
         "orc_set_mutation": (None, [C.c_int]),
+        "orc_eos_pressure_array": (None, [P, C.c_size_t, _DP, _DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -99,6 +100,14 @@ def params(eq) -> OrcParams:
     for n in ("h", "m", "rho0", "cs0", "xi", "g", "mu", "alpha", "delta", "cfl",
               "c_force", "c_visc"):
         setattr(p, n, float(getattr(eq, n)))
+    return p
+
+
+def pressure(prm, rho):
+    """Tait pressure of each density (FP64)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    p = np.empty_like(rho)
+    load().orc_eos_pressure_array(C.byref(prm), rho.size, _dp(rho), _dp(p))
     return p
 
 

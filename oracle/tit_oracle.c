@@ -166,6 +166,11 @@ double orc_eos_density_from_potential(const orc_params* prm, double H) {
   return prm->rho0 * pow(1.0 + xi_1 * H / pow2(prm->cs0), 1.0 / xi_1);
 }
 
+void orc_eos_pressure_array(const orc_params* prm, size_t n, const double* rho,
+                            double* p) {
+  for (size_t i = 0; i < n; ++i) p[i] = orc_eos_pressure(prm, rho[i]);
+}
+
 /* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
 /* Uniform grid index (geom/search/grid_search.hpp:73-130, geom/grid.hpp:86-169,
  * geom/bbox.hpp:76-119, geom/point_range.hpp:82-87, geom/bsphere.hpp:47-54). */

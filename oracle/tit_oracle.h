@@ -83,6 +83,9 @@ double orc_eos_density_from_pressure(const orc_params* prm, double p);
 double orc_eos_sound_speed(const orc_params* prm, double rho);
 double orc_eos_potential(const orc_params* prm, double rho);
 double orc_eos_density_from_potential(const orc_params* prm, double H);
+/* p[i] = pressure(rho[i]) for i < n. */
+void orc_eos_pressure_array(const orc_params* prm, size_t n, const double* rho,
+                            double* p);
 
 /* ---- neighbour search ---------------------------------------------------- */
 /* Reference GridIndex + ParticleMesh::search_: per point, the ascending list of
