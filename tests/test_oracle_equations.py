@@ -1,0 +1,294 @@
+"""Pins of the oracle's right-hand sides, boundary extrapolation, time step
+and integrator (CPU).
+
+Nothing in the reference's test suite pins FluidEquations or the SSPRK
+integrator numerically (only source/tit/sph/kernel.test.cpp tests the sph
+library, source/tit/sph/CMakeLists.txt:48-56), so each
+check here is a property the mathematics fixes, chosen so that a plausible
+mistake fails it; the planted-mistake tests flip such a mistake on inside the
+oracle (orc_set_mutation) and show that the property catches it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import titsolver_amd as ta
+from helpers import perturbed_dam_break
+
+G = 9.81
+
+
+def _eq(dim, dr, flavour=ta.FLAVOUR_CFG, kernel=ta.KERNEL_WENDLAND_C2):
+    eq = ta.cfg_equations(dim, dr, 0.6, kernel=kernel, flavour=flavour)
+    if flavour == ta.FLAVOUR_REF:
+        eq.mu = 1e-3
+    return eq
+
+
+def _rhs(eq, r, rho, n_fluid, v=None, normal=None, boundary=False):
+    prm = oracle.params(eq)
+    dim = r.shape[1]
+    if normal is None:
+        normal = np.zeros((r.shape[0] - n_fluid, dim))
+    st = oracle.State(r, rho, n_fluid, normal, v)
+    off, idx = oracle.adjacency(st, prm)
+    if boundary:
+        oracle.setup_boundary(prm, st, off, idx)
+    oracle.compute_continuity(prm, st, off, idx)
+    oracle.compute_momentum(prm, st, off, idx)
+    return st
+
+
+def _lattice(dim, n, dr):
+    g = np.meshgrid(*[np.arange(n)] * dim, indexing="ij")
+    return np.stack([dr * (gi.ravel() + 1.0) for gi in g], axis=1)
+
+
+def _interior(r, dr, n, margin=5):
+    lo, hi = dr * (1 + margin), dr * (n - margin)
+    return np.all((r >= lo) & (r <= hi), axis=1)
+
+
+# ---- conservation and invariance ---------------------------------------------
+
+def _momentum_residual(st, eq):
+    dv_pair = st.dv_dt.copy()
+    dv_pair[: st.n_fluid, 1] += eq.g  # remove gravity of the fluid particles
+    return np.abs(dv_pair.sum(0)).max() / np.abs(dv_pair).sum()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("flavour", [ta.FLAVOUR_CFG, ta.FLAVOUR_REF])
+@pytest.mark.parametrize("kernel", [ta.KERNEL_WENDLAND_C2, ta.KERNEL_WENDLAND_C4])
+def test_pair_forces_conserve_momentum(dim, flavour, kernel):
+    dr = 0.02 if dim == 2 else 0.03
+    pa = perturbed_dam_break(dim, dr, seed=2)
+    eq = _eq(dim, dr, flavour, kernel)
+    st = _rhs(eq, pa.r, pa.rho, pa.n_fluid, pa.v)
+    assert _momentum_residual(st, eq) < 1e-13
+
+
+def test_planted_pressure_sign_breaks_momentum_conservation():
+    pa = perturbed_dam_break(2, 0.02, seed=2)
+    eq = _eq(2, 0.02)
+    oracle.set_mutation(1)
+    st = _rhs(eq, pa.r, pa.rho, pa.n_fluid, pa.v)
+    assert _momentum_residual(st, eq) > 1e-3
+
+
+@pytest.mark.parametrize("flavour", [ta.FLAVOUR_CFG, ta.FLAVOUR_REF])
+def test_galilean_invariance(flavour):
+    pa = perturbed_dam_break(2, 0.02, seed=4)
+    eq = _eq(2, 0.02, flavour)
+    a = _rhs(eq, pa.r, pa.rho, pa.n_fluid, pa.v)
+    b = _rhs(eq, pa.r, pa.rho, pa.n_fluid, pa.v + np.array([0.7, -1.3]))
+    np.testing.assert_allclose(b.drho_dt, a.drho_dt, rtol=0, atol=1e-9 * np.abs(a.drho_dt).max())
+    np.testing.assert_allclose(b.dv_dt, a.dv_dt, rtol=0, atol=1e-9 * np.abs(a.dv_dt).max())
+
+
+@pytest.mark.parametrize("flavour", [ta.FLAVOUR_CFG, ta.FLAVOUR_REF])
+def test_rotation_covariance(flavour):
+    pa = perturbed_dam_break(2, 0.02, seed=5)
+    eq = _eq(2, 0.02, flavour)
+    th = 0.37
+    Rm = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+    a = _rhs(eq, pa.r, pa.rho, pa.n_fluid, pa.v)
+    b = _rhs(eq, pa.r @ Rm.T, pa.rho, pa.n_fluid, pa.v @ Rm.T)
+    nf = pa.n_fluid
+    pa_dv = a.dv_dt.copy()
+    pa_dv[:nf, 1] += eq.g
+    pb_dv = b.dv_dt.copy()
+    pb_dv[:nf, 1] += eq.g
+    np.testing.assert_allclose(b.drho_dt, a.drho_dt, rtol=0, atol=1e-9 * np.abs(a.drho_dt).max())
+    np.testing.assert_allclose(pb_dv, pa_dv @ Rm.T, rtol=0, atol=1e-9 * np.abs(pa_dv).max())
+
+
+# ---- consistency on lattices ---------------------------------------------------
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_rest_state_is_exact(dim):
+    dr, n = 0.01, (24 if dim == 2 else 12)
+    r = _lattice(dim, n, dr)
+    eq = _eq(dim, dr)
+    st = _rhs(eq, r, np.full(len(r), 1000.0), len(r))
+    assert np.all(st.drho_dt == 0.0)
+    want = np.zeros(dim)
+    want[1] = -eq.g
+    assert np.all(st.dv_dt == want)
+
+
+@pytest.mark.parametrize("flavour", [ta.FLAVOUR_CFG, ta.FLAVOUR_REF])
+def test_continuity_of_linear_velocity_field(flavour):
+    dr, n = 0.01, 40
+    r = _lattice(2, n, dr)
+    A = np.array([[0.3, -0.8], [0.5, 0.9]])
+    v = r @ A.T
+    eq = _eq(2, dr, flavour)
+    st = _rhs(eq, r, np.full(len(r), 1000.0), len(r), v)
+    inner = _interior(r, dr, n)
+    want = -1000.0 * np.trace(A)
+    np.testing.assert_allclose(st.drho_dt[inner], want, rtol=1e-2)
+
+
+@pytest.mark.parametrize("mutate", [0, 1])
+def test_hydrostatic_lattice_has_no_interior_acceleration(mutate):
+    dr, n = 0.01, 40
+    r = _lattice(2, n, dr)
+    eq = _eq(2, dr)
+    prm = oracle.params(eq)
+    top = dr * (n + 1)
+    lib = oracle.load()
+    import ctypes as C
+    rho = np.array([lib.orc_eos_density_from_potential(C.byref(prm), eq.g * (top - y))
+                    for y in r[:, 1]])
+    oracle.set_mutation(mutate)
+    st = _rhs(eq, r, rho, len(r))
+    inner = _interior(r, dr, n)
+    acc = np.abs(st.dv_dt[inner]).max()
+    if mutate:
+        assert acc > 1.0  # a sign slip in the pressure term is caught
+    else:
+        assert acc < 0.02 * eq.g
+
+
+@pytest.mark.parametrize("mutate", [0, 2])
+def test_delta_sph_term_is_a_laplacian(mutate):
+    dr, n = 0.01, 40
+    r = _lattice(2, n, dr)
+    eq = _eq(2, dr)
+    c = 30.0
+    x0 = r.mean(0)
+    rho = 1000.0 + c * ((r - x0) ** 2).sum(1)  # laplacian = 4c
+    oracle.set_mutation(mutate)
+    st = _rhs(eq, r, rho, len(r))
+    inner = _interior(r, dr, n)
+    want = eq.delta * eq.h * eq.cs0 * 4.0 * c
+    got = st.drho_dt[inner]
+    if mutate:
+        assert np.all(got < 0)  # anti-diffusive sign is caught
+    else:
+        np.testing.assert_allclose(got, want, rtol=3e-2)
+
+
+@pytest.mark.parametrize("mutate", [0, 3])
+def test_artificial_viscosity_only_on_approaching_pairs(mutate):
+    dr, n = 0.01, 20
+    r = _lattice(2, n, dr)
+    rho = np.full(len(r), 1000.0)
+    eq = _eq(2, dr)
+    eq0 = _eq(2, dr)
+    eq0.alpha = 0.0
+    oracle.set_mutation(mutate)
+    expand = _rhs(eq, r, rho, len(r), 2.0 * (r - r.mean(0)))
+    expand0 = _rhs(eq0, r, rho, len(r), 2.0 * (r - r.mean(0)))
+    compress = _rhs(eq, r, rho, len(r), -2.0 * (r - r.mean(0)))
+    compress0 = _rhs(eq0, r, rho, len(r), -2.0 * (r - r.mean(0)))
+    assert np.abs(compress.dv_dt - compress0.dv_dt).max() > 1e-3
+    diff = np.abs(expand.dv_dt - expand0.dv_dt).max()
+    if mutate:
+        assert diff > 1e-3
+    else:
+        assert diff == 0.0
+
+
+# ---- boundary extrapolation ---------------------------------------------------------
+
+@pytest.mark.parametrize("kernel", [ta.KERNEL_WENDLAND_C2, ta.KERNEL_WENDLAND_C4])
+def test_boundary_extrapolation_continues_hydrostatic_law(kernel):
+    """For a hydrostatic fluid H(rho) + g y is constant, so every term of the
+    Shepard sum of setup_boundary equals C - g y_e and the wall density is the
+    hydrostatic density at the wall particle, whatever the weights."""
+    import ctypes as C
+    dr, nx, ny = 0.01, 30, 20
+    fl = _lattice(2, max(nx, ny), dr)
+    fl = fl[(fl[:, 0] <= nx * dr) & (fl[:, 1] <= ny * dr)]
+    xs = dr * np.arange(-2, nx + 3)
+    fx = np.array([[x, -k * dr] for k in range(3) for x in xs])
+    r = np.vstack([fl, fx])
+    eq = _eq(2, dr, kernel=kernel)
+    prm = oracle.params(eq)
+    lib = oracle.load()
+    Cst = eq.g * (ny + 1) * dr
+    rho_of_y = lambda y: lib.orc_eos_density_from_potential(C.byref(prm), Cst - eq.g * y)
+    rho = np.array([rho_of_y(y) for y in fl[:, 1]] + [1000.0] * len(fx))
+    normal = np.tile([0.0, 1.0], (len(fx), 1))
+    st = _rhs(eq, r, rho, len(fl), normal=normal, boundary=True)
+    want = np.array([rho_of_y(y) for y in fx[:, 1]])
+    got = st.rho[len(fl):]
+    under = (fx[:, 0] >= dr) & (fx[:, 0] <= nx * dr)
+    np.testing.assert_allclose(got[under], want[under], rtol=1e-12)
+    # beyond the fluid's ends the Shepard sum may vanish (is_tiny -> H = 0,
+    # rho = rho0, fluid_equations.hpp:162)
+    ok = np.isclose(got, want, rtol=1e-12, atol=0) | (got == 1000.0)
+    assert np.all(ok[~under])
+    assert np.all(st.v[len(fl):] == 0.0)
+
+
+# ---- time step -----------------------------------------------------------------------
+
+def test_time_step_known_values():
+    eq = _eq(2, 0.01)
+    eq.mu = 0.0
+    r = np.array([[0.0, 0.0]])
+    v = np.array([[3.0, 4.0]])
+    prm = oracle.params(eq)
+    st = oracle.State(r, np.array([1000.0]), 1, np.zeros((0, 2)), v)
+    # acoustic limit: cfl h / (cs0 + |v|)
+    assert oracle.compute_time_step(prm, st) == pytest.approx(eq.cfl * eq.h / (eq.cs0 + 5.0), rel=1e-15)
+    # force limit with |dv/dt| below g uses g
+    eq2 = _eq(2, 0.01)
+    eq2.mu, eq2.cfl = 0.0, 1e6
+    st2 = oracle.State(r, np.array([1000.0]), 1, np.zeros((0, 2)), v, np.array([[1.0, 0.0]]))
+    assert oracle.compute_time_step(oracle.params(eq2), st2) == pytest.approx(
+        eq2.c_force * math.sqrt(eq2.h / eq2.g), rel=1e-15)
+    st3 = oracle.State(r, np.array([1000.0]), 1, np.zeros((0, 2)), v, np.array([[300.0, 400.0]]))
+    assert oracle.compute_time_step(oracle.params(eq2), st3) == pytest.approx(
+        eq2.c_force * math.sqrt(eq2.h / 500.0), rel=1e-15)
+    # viscous limit c_visc h^2 rho / mu
+    eq4 = _eq(2, 0.01, ta.FLAVOUR_REF)
+    eq4.mu, eq4.cfl, eq4.c_force = 1e3, 1e6, 1e6
+    assert oracle.compute_time_step(oracle.params(eq4), st) == pytest.approx(
+        eq4.c_visc * eq4.h ** 2 * 1000.0 / 1e3, rel=1e-15)
+
+
+# ---- integrator ----------------------------------------------------------------------
+
+@pytest.mark.parametrize("order,mutate", [(2, 0), (3, 0), (3, 4)])
+def test_ssprk_amplification_polynomial(order, mutate):
+    """One step on u' = lam u equals the Taylor polynomial of exp(z) of the
+    method's order (any s-stage order-s explicit RK, s <= 3)."""
+    oracle.set_mutation(mutate)
+    for z in (-0.5, -0.1, 0.2, 0.9):
+        u = oracle.ssprk_generic(order, z, np.array([1.0]), lambda u: u)[0]
+        want = sum(z ** k / math.factorial(k) for k in range(order + 1))
+        if mutate:
+            assert abs(u - want) > 1e-4
+        else:
+            assert u == pytest.approx(want, rel=1e-14)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_ssprk_convergence_order(order):
+    # u' = -u^2, u(0) = 1 -> u(1) = 1/2
+    errs = []
+    for n in (20, 40, 80):
+        u = np.array([1.0])
+        for _ in range(n):
+            u = oracle.ssprk_generic(order, 1.0 / n, u, lambda x: -x * x)
+        errs.append(abs(u[0] - 0.5))
+    rates = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(abs(rt - order) < 0.15 for rt in rates), rates
+
+
+def test_sph_step_integrates_free_fall_exactly():
+    """A lone fluid particle under gravity: SSPRK3 is exact for quadratics."""
+    eq = _eq(2, 0.01)
+    prm = oracle.params(eq)
+    st = oracle.State(np.array([[0.3, 0.7]]), np.array([1000.0]), 1, np.zeros((0, 2)),
+                      np.array([[0.5, 1.5]]))
+    dt = oracle.ssprk_step(prm, st, 3)
+    assert st.r[0, 0] == pytest.approx(0.3 + 0.5 * dt, rel=1e-15)
+    assert st.r[0, 1] == pytest.approx(0.7 + 1.5 * dt - 0.5 * eq.g * dt * dt, rel=1e-14)
+    assert st.v[0, 1] == pytest.approx(1.5 - eq.g * dt, rel=1e-14)
