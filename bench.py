@@ -55,7 +55,32 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=512,
+                    help="device-to-device copy of this many MiB (attainable "
+                         "HBM bandwidth; reference kernel for counter scaling)")
     return ap.parse_args()
+
+
+def copy_probe(mib: int, reps: int = 5) -> dict:
+    """Times hipMemcpy device-to-device of `mib` MiB (read + write traffic)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        return {"error": "hipMalloc failed"}
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        best = min(best, time.perf_counter() - t0)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"bytes_read": n, "bytes_written": n, "best_s": best,
+            "achieved_GBs": 2 * n / best / 1e9}
 
 
 def main():
@@ -128,6 +153,8 @@ def main():
     }
     ctx.close()
 
+    if args.copy_probe_mb > 0:
+        out["copy_probe"] = copy_probe(args.copy_probe_mb)
     if not args.no_cpu and args.cpu_steps > 0:
         out["cpu_baseline"], out["parity"] = cpu_baseline(pa, eq, args.cpu_steps)
     print(json.dumps(out), flush=True)

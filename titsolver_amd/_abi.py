@@ -47,6 +47,11 @@ class HostView(C.Structure):
     ]
 
 
+class Schedule(C.Structure):
+    _fields_ = [("strip_cells", C.c_int), ("lds_capacity", C.c_int),
+                ("occupancy", C.c_double)]
+
+
 class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64),
                 ("total_ms", C.c_double)]
@@ -71,6 +76,8 @@ SIGNATURES = {
     "tit_hip_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "tit_hip_kernel_stats": (C.c_int, [C.c_void_p, C.POINTER(KernelStat), C.c_int,
                                        C.POINTER(C.c_int)]),
+    "tit_hip_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
+    "tit_hip_set_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
     "tit_hip_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t),
                                  C.POINTER(C.c_uint64)]),
     "tit_hip_last_error": (C.c_char_p, [C.c_void_p]),

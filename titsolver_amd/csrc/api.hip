@@ -115,6 +115,15 @@ struct tit_hip_ctx {
   double* rhs_drho = nullptr;
   double* rhs_dv[3] = {nullptr, nullptr, nullptr};
 
+  // strip schedule of the stage kernel
+  std::vector<DevTank> host_tanks;
+  uint4* work = nullptr;
+  size_t work_cap = 0;
+  uint32_t n_work = 0;
+  uint32_t strip_len = 16;
+  uint32_t lds_cap = 0;
+  double occupancy = 1.0;  // mean particles per occupied cell at upload
+
   // graph of two steps, valid while the rebuild parity matches
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
@@ -242,6 +251,8 @@ tit_status fill_params(tit_hip_ctx* c, const tit_hip_params& p) {
   d.ahc = static_cast<float>(p.alpha * p.h);
   d.eta2 = static_cast<float>(0.01 * p.h * p.h);
   d.two_mu = static_cast<float>(2.0 * p.mu);
+  d.inv_rho0_f = static_cast<float>(1.0 / p.rho0);
+  d.cs0_f = static_cast<float>(p.cs0);
   return TIT_OK;
 }
 
@@ -318,8 +329,64 @@ void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
   launch_boundary(c->stream, a);
 }
 
+// Strip work list: every column of every tank cut into strips of strip_len
+// cells along the fast axis.
+tit_status build_work(tit_hip_ctx* c) {
+  const int dim = c->prm.dim;
+  std::vector<uint4> items;
+  const uint32_t S = c->strip_len;
+  for (size_t t = 0; t < c->host_tanks.size(); ++t) {
+    const DevTank& T = c->host_tanks[t];
+    const int nz = T.nc[dim - 1];
+    const int ny = dim == 3 ? T.nc[1] : 1;
+    for (int ix = 0; ix < T.nc[0]; ++ix) {
+      for (int iy = 0; iy < ny; ++iy) {
+        for (int z = 0; z < nz; z += static_cast<int>(S)) {
+          items.push_back(make_uint4(static_cast<uint32_t>(t), ix, iy, z));
+        }
+      }
+    }
+  }
+  if (items.size() >= (1ull << 31)) return fail(c, TIT_E_ARG, "too many strips");
+  if (items.size() > c->work_cap) {
+    if (c->work) {
+      auto it = std::find(c->allocs.begin(), c->allocs.end(), static_cast<void*>(c->work));
+      if (it != c->allocs.end()) c->allocs.erase(it);
+      (void)hipFree(c->work);
+      c->work = nullptr;
+    }
+    TIT_HIP_TRY(c, dalloc(c, &c->work, items.size()));
+    c->work_cap = items.size();
+  }
+  TIT_HIP_TRY(c, hipMemcpy(c->work, items.data(), items.size() * sizeof(uint4),
+                           hipMemcpyHostToDevice));
+  c->n_work = static_cast<uint32_t>(items.size());
+  drop_graph(c);
+  return TIT_OK;
+}
+
+// Default strip length: about 0.9 * threads targets per strip at the upload
+// occupancy, and (2D) an LDS candidate estimate of at most 85% capacity.
+uint32_t default_strip_len(const tit_hip_ctx* c) {
+  const double occ = std::max(c->occupancy, 0.25);
+  const int nt = strip_threads(c->prm.dim);
+  int S = static_cast<int>(std::floor(0.9 * nt / occ));
+  S = std::max(1, std::min(S, kStripMax));
+  if (c->prm.dim == 2) {
+    while (S > 1 && (2 * kStencil + 1) * (S + 2 * kStencil) * occ >
+                        0.85 * strip_lds_capacity(2)) {
+      --S;
+    }
+  }
+  return static_cast<uint32_t>(S);
+}
+
 StageArgs stage_args(tit_hip_ctx* c) {
   StageArgs a{};
+  a.work = c->work;
+  a.n_work = c->n_work;
+  a.strip_len = c->strip_len;
+  a.lds_cap = c->lds_cap;
   a.P = c->dp;
   a.tanks = c->tanks;
   a.key = c->key;
@@ -531,6 +598,28 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
     }
   }
 
+  // Mean occupancy of the occupied cells (sets the default strip length).
+  {
+    std::vector<uint64_t> keys(n_total);
+    size_t q = 0;
+    for (size_t t = 0; t < n_tanks; ++t) {
+      const size_t nt = views[t].n_fluid + views[t].n_fixed;
+      for (size_t a = 0; a < nt; ++a, ++q) {
+        uint64_t flat = 0;
+        for (int k = 0; k < dim; ++k) {
+          const double f = (hr[q * dim + k] - ht[t].lo[k]) * ht[t].inv_c[k];
+          int cc = f <= 0.0 ? 0 : (f >= ht[t].nc[k] - 1 ? ht[t].nc[k] - 1 : static_cast<int>(f));
+          flat = flat * static_cast<uint64_t>(ht[t].nc[k]) + static_cast<uint64_t>(cc);
+        }
+        keys[q] = ht[t].cell_off + flat;
+      }
+    }
+    std::sort(keys.begin(), keys.end());
+    const size_t occupied = static_cast<size_t>(std::unique(keys.begin(), keys.end()) - keys.begin());
+    c->occupancy = occupied ? static_cast<double>(n_total) / occupied : 1.0;
+  }
+  c->host_tanks = ht;
+
   // Device buffers.
   const uint32_t n = c->n;
   TIT_HIP_TRY(c, alloc_state(c, c->A, n, dim));
@@ -603,6 +692,12 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   tit_status st = check_launch(c, "upload");
   TIT_HIP_TRY(c, hipStreamSynchronize(s));
   if (d_dv) (void)hipFree(d_dv);
+  if (st != TIT_OK) return st;
+  c->work = nullptr;
+  c->work_cap = 0;
+  c->strip_len = default_strip_len(c);
+  c->lds_cap = strip_lds_capacity(dim);
+  st = build_work(c);
   if (st != TIT_OK) return st;
   c->uploaded = true;
   c->rebuilds = 0;
@@ -791,6 +886,29 @@ tit_status tit_hip_kernel_stats(tit_hip_ctx* c, tit_hip_kernel_stat* out, int ca
     out[k].total_ms = c->prof.totals[k].total_ms;
   }
   return TIT_OK;
+}
+
+tit_status tit_hip_get_schedule(tit_hip_ctx* c, tit_hip_schedule* out) {
+  if (!c || !out) return TIT_E_ARG;
+  out->strip_cells = static_cast<int>(c->strip_len);
+  out->lds_capacity = static_cast<int>(c->lds_cap);
+  out->occupancy = c->occupancy;
+  return TIT_OK;
+}
+
+tit_status tit_hip_set_schedule(tit_hip_ctx* c, const tit_hip_schedule* sch) {
+  if (!c || !sch) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "schedule before upload");
+  if (sch->strip_cells < 0 || sch->strip_cells > kStripMax || sch->lds_capacity < 0 ||
+      sch->lds_capacity > static_cast<int>(strip_lds_capacity(c->prm.dim))) {
+    return fail(c, TIT_E_ARG, "strip_cells must be in [0, 64] and lds_capacity "
+                              "within the kernel's LDS capacity");
+  }
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (sch->strip_cells > 0) c->strip_len = static_cast<uint32_t>(sch->strip_cells);
+  c->lds_cap = static_cast<uint32_t>(sch->lds_capacity);
+  return build_work(c);
 }
 
 tit_status tit_hip_counts(tit_hip_ctx* c, size_t* n_particles, uint64_t* n_rebuilds) {

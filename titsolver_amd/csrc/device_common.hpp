@@ -39,6 +39,8 @@ struct DevParams {
   float ahc;        // alpha h         (Monaghan viscosity)
   float eta2;       // 0.01 h^2
   float two_mu;     // 2 mu            (physical viscosity)
+  float inv_rho0_f; // 1 / rho0
+  float cs0_f;      // cs0
 };
 
 // Per-tank cell grid.  Cell size is at least (support radius + skin) / 2, so

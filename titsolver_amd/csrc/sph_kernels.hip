@@ -96,27 +96,6 @@ __device__ __forceinline__ void wave_atomic_min_u64(unsigned long long* base,
   }
 }
 
-__device__ __forceinline__ void wave_atomic_max_u32(uint32_t* base, uint32_t t,
-                                                    uint32_t v, bool active) {
-  const uint64_t act = __ballot(active);
-  if (act == 0) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll(static_cast<long long>(act)) - 1;
-  const uint32_t t0 = __shfl(t, leader);
-  const bool same = __ballot(active && t == t0) == act;
-  if (same) {
-    uint32_t m = active ? v : 0u;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const uint32_t o = __shfl_xor(m, off);
-      m = o > m ? o : m;
-    }
-    if (lane == leader) atomicMax(base + t0, m);
-  } else if (active) {
-    atomicMax(base + t, v);
-  }
-}
-
 // FluidEquations::compute_time_step for one fluid particle
 // (source/tit/sph/fluid_equations.hpp:199-222).
 template <int DIM>
@@ -242,24 +221,27 @@ __global__ __launch_bounds__(kBlock) void k_compact_fixed(
 // ---- boundary extrapolation ---------------------------------------------------
 // FluidEquations::setup_boundary (source/tit/sph/fluid_equations.hpp:122-164):
 // Shepard average of H(rho_b) + g (r_be . n_e) n_e[1] over fluid neighbours,
-// solved for rho_e; v_e = 0.  FP64 throughout (fixed particles are few).
+// solved for rho_e; v_e = 0.  FP64 throughout.  One wave per fixed particle:
+// the lanes split the stencil's candidates and the two sums are reduced with
+// wave shuffles.
 
 template <int DIM, int KERNEL>
 __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= A.n_fixed) return;
+  constexpr int NCOL = DIM == 2 ? 2 * kStencil + 1 : (2 * kStencil + 1) * (2 * kStencil + 1);
+  const uint32_t k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= A.n_fixed) return;  // wave-uniform
   const DevParams& P = A.P;
   const uint32_t i = A.fixed_list[k];
   const DevTank T = A.tanks[A.tag[i] >> 1];
   double xe[DIM], ne[DIM];
   const double* nrm = A.normal + static_cast<size_t>(A.id[i]) * DIM;
-  double n2 = 0.0;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     xe[d] = A.s.x[d][i];
     ne[d] = nrm[d];
   }
-  n2 = ne[0] * ne[0];
+  double n2 = ne[0] * ne[0];
 #pragma unroll
   for (int d = 1; d < DIM; ++d) n2 = n2 + ne[d] * ne[d];
   const double tiny = 6.0554544523933395e-06;  // cbrt(DBL_EPSILON)
@@ -271,7 +253,6 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) ne[d] = 0.0;
   }
-  // W_be = omega / h^d * w(q), omega of the kernel in this dimension.
   const double omega = KERNEL == kKernelC2
                            ? (DIM == 2 ? 7.0 / (4.0 * M_PI) : 21.0 / (16.0 * M_PI))
                            : (DIM == 2 ? 9.0 / (4.0 * M_PI) : 495.0 / (256.0 * M_PI));
@@ -279,173 +260,80 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   double w0 = omega;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) w0 *= h_inv;
+
+  // Stencil column ranges (wave-uniform).
+  const uint32_t local = A.key[i] - T.cell_off;
+  int c[3];
+  if constexpr (DIM == 2) {
+    c[0] = static_cast<int>(local / T.nc[1]);
+    c[1] = static_cast<int>(local % T.nc[1]);
+  } else {
+    const uint32_t q = local / T.nc[2];
+    c[2] = static_cast<int>(local % T.nc[2]);
+    c[1] = static_cast<int>(q % T.nc[1]);
+    c[0] = static_cast<int>(q / T.nc[1]);
+  }
+  constexpr int L = DIM - 1;
+  const int zlo = max(c[L] - kStencil, 0), zhi = min(c[L] + kStencil, T.nc[L] - 1);
+  uint32_t rb[NCOL], rl[NCOL + 1];
+  rl[0] = 0;
+#pragma unroll
+  for (int q = 0; q < NCOL; ++q) {
+    int ix, iy = 0;
+    bool ok;
+    if constexpr (DIM == 2) {
+      ix = c[0] - kStencil + q;
+      ok = ix >= 0 && ix < T.nc[0];
+    } else {
+      ix = c[0] - kStencil + q / (2 * kStencil + 1);
+      iy = c[1] - kStencil + q % (2 * kStencil + 1);
+      ok = ix >= 0 && ix < T.nc[0] && iy >= 0 && iy < T.nc[1];
+    }
+    uint32_t b = 0, e = 0;
+    if (ok) {
+      const uint32_t row = DIM == 2
+                               ? T.cell_off + static_cast<uint32_t>(ix) * T.nc[1]
+                               : T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] +
+                                               static_cast<uint32_t>(iy)) * T.nc[2];
+      b = A.cell_off[row + zlo];
+      e = A.cell_off[row + zhi + 1];
+    }
+    rb[q] = b;
+    rl[q + 1] = rl[q] + (e - b);
+  }
   double S = 0.0, H = 0.0;
-  for_each_neighbor<DIM>(P, T, A.s, A.cell_off, A.key[i], i, xe,
-                         [&](uint32_t j, const double* dd, double s2) {
-                           if (A.tag[j] & 1u) return;
-                           const double rho_b = A.s.rho[j];
-                           const double V_b = P.m / rho_b;
-                           const double W =
-                               w0 * kernel_value_poly_f64<KERNEL>(h_inv * sqrt(s2));
-                           double rn = dd[0] * ne[0];
+  for (uint32_t t = lane; t < rl[NCOL]; t += 64) {
+    int q = 0;
+    while (t >= rl[q + 1]) ++q;
+    const uint32_t j = rb[q] + (t - rl[q]);
+    if (A.tag[j] & 1u) continue;  // fluid neighbours only (self is fixed)
+    double dd[DIM];
 #pragma unroll
-                           for (int d = 1; d < DIM; ++d) rn = rn + dd[d] * ne[d];
-                           const double H_b = eos_potential(P, rho_b);
-                           S += V_b * W;
-                           H += V_b * (H_b + P.g * rn * ne[1]) * W;
-                         });
-  const double Hs = fabs(S) <= tiny ? 0.0 : H / S;
-  A.s.rho[i] = eos_density_from_potential(P, Hs);
+    for (int d = 0; d < DIM; ++d) dd[d] = A.s.x[d][j] - xe[d];
+    double s2 = dd[0] * dd[0];
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) A.s.v[d][i] = 0.0;
-}
-
-// ---- fused stage ----------------------------------------------------------------
-// Continuity + momentum right-hand sides (fluid_equations.hpp:232-305, wall
-// terms absent) gathered per fluid particle, then the SSPRK substep update
-// and Shu-Osher combination (time_integrator.hpp:188-223) in FP64.  The last
-// stage of a step also folds the next step's CFL candidates (the reference
-// computes them from this state and this stage's dv/dt at the start of the
-// next step, fluid_equations.hpp:199-222).
-
-template <int DIM, int KERNEL, int FLAVOUR, int KIND, bool FINAL>
-__global__ __launch_bounds__(kBlock) void k_stage(StageArgs A) {
-  const DevParams& P = A.P;
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const bool in_range = i < A.n;
-  const uint32_t tag = in_range ? A.tag[i] : 1u;
-  const uint32_t t = tag >> 1;
-  const bool fluid = in_range && !(tag & 1u);
-
-  double dt_cand = 0.0;
-  float vin = 0.0f;
-  if (in_range && !fluid) {
-    // Fixed particles keep r; v = 0 and rho come from the extrapolation.
-    if constexpr (KIND == kRhsOnly) {
-      A.rhs_drho[i] = 0.0;
+    for (int d = 1; d < DIM; ++d) s2 = s2 + dd[d] * dd[d];
+    if (!(s2 <= P.R2)) continue;
+    const double rho_b = A.s.rho[j];
+    const double V_b = P.m / rho_b;
+    const double W = w0 * kernel_value_poly_f64<KERNEL>(h_inv * sqrt(s2));
+    double rn = dd[0] * ne[0];
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) A.rhs_dv[k][i] = 0.0;
-    } else {
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        A.out.x[k][i] = A.in.x[k][i];
-        A.out.v[k][i] = 0.0;
-      }
-      A.out.rho[i] = A.in.rho[i];
-    }
+    for (int d = 1; d < DIM; ++d) rn = rn + dd[d] * ne[d];
+    const double H_b = eos_potential(P, rho_b);
+    S += V_b * W;
+    H += V_b * (H_b + P.g * rn * ne[1]) * W;
   }
-  if (fluid) {
-    const DevTank T = A.tanks[t];
-    double xa[DIM], va[DIM];
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) {
-      xa[k] = A.in.x[k][i];
-      va[k] = A.in.v[k][i];
-    }
-    const double rho_a = A.in.rho[i];
-    const double pa = eos_pressure(P, rho_a);
-    const float Pa = static_cast<float>(pa / (rho_a * rho_a));
-    const double csa64 = eos_sound_speed(P, rho_a);
-    const float csa = static_cast<float>(csa64);
-    const float rho_af = static_cast<float>(rho_a);
-
-    Kahan acc_rho, acc_v[DIM];
-    acc_rho.init();
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) acc_v[k].init();
-
-    for_each_neighbor<DIM>(
-        P, T, A.in, A.cell_off, A.key[i], i, xa,
-        [&](uint32_t j, const double* d, double s2) {
-          float x[DIM], vab[DIM];
-          float vr = 0.0f;
-#pragma unroll
-          for (int k = 0; k < DIM; ++k) {
-            x[k] = static_cast<float>(-d[k]);
-            vab[k] = static_cast<float>(va[k] - A.in.v[k][j]);
-            vr = __builtin_fmaf(vab[k], x[k], vr);
-          }
-          const float r2 = static_cast<float>(s2);
-          const float r = sqrtf(r2);
-          const float q = r * P.inv_h;
-          const float G = P.gsc * kernel_grad_poly<KERNEL>(q);
-          const double rho_b = A.in.rho[j];
-          const double pb = eos_pressure(P, rho_b);
-          const float Pb = static_cast<float>(pb / (rho_b * rho_b));
-          const float rho_bf = static_cast<float>(rho_b);
-          const float drab = static_cast<float>(rho_a - rho_b);
-          // Continuity: m_b v_ab . grad W_ab plus density diffusion.
-          float cont = P.m_f * G * vr;
-          float Pi;
-          if constexpr (FLAVOUR == kFlavourCfg) {
-            // delta h c0 V_b psi_ab . grad W_ab, psi_ab = 2 rho_ab r_ab/|r_ab|^2
-            cont = __builtin_fmaf(P.dhc2 * (P.m_f / rho_bf), drab * G, cont);
-            Pi = 0.0f;
-            if (vr < 0.0f) {
-              const float csb = static_cast<float>(eos_sound_speed(P, rho_b));
-              const float cbar = 0.5f * (csa + csb);
-              const float rbar = 0.5f * (rho_af + rho_bf);
-              Pi = P.ahc * cbar * vr / (rbar * (r2 + P.eta2));
-            }
-          } else {
-            // Ferrari: m_b max(cs_a, cs_b) rho_ab / rho_b (r_ab . grad W)/|r_ab|
-            const float csb = static_cast<float>(eos_sound_speed(P, rho_b));
-            const float csab = fmaxf(csa, csb);
-            cont = __builtin_fmaf(P.m_f * csab * drab / rho_bf, G * r, cont);
-            Pi = P.two_mu * vr / (rho_af * rho_bf * r2);
-          }
-          acc_rho.add(cont);
-          const float coef = P.m_f * (Pi - (Pa + Pb)) * G;
-#pragma unroll
-          for (int k = 0; k < DIM; ++k) acc_v[k].add(coef * x[k]);
-        });
-
-    double drho = static_cast<double>(acc_rho.value());
-    double dv[DIM];
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) dv[k] = static_cast<double>(acc_v[k].value());
-    dv[1] = dv[1] - P.g;
-
-    if constexpr (KIND == kRhsOnly) {
-      A.rhs_drho[i] = drho;
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) A.rhs_dv[k][i] = dv[k];
-    } else {
-      double v2 = 0.0;
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) v2 = v2 + va[k] * va[k];
-      vin = static_cast<float>(sqrt(v2));
-      const double h = A.dt[t];
-      double rn[DIM], vn[DIM];
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        rn[k] = xa[k] + h * va[k];
-        vn[k] = va[k] + h * dv[k];
-      }
-      double rhon = rho_a + h * drho;
-      if constexpr (KIND == kCombine) {
-        const double w = A.w, w1 = 1.0 - A.w;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-          rn[k] = w1 * A.old.x[k][i] + w * rn[k];
-          vn[k] = w1 * A.old.v[k][i] + w * vn[k];
-        }
-        rhon = w1 * A.old.rho[i] + w * rhon;
-      }
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        A.out.x[k][i] = rn[k];
-        A.out.v[k][i] = vn[k];
-      }
-      A.out.rho[i] = rhon;
-      if constexpr (FINAL) dt_cand = particle_time_step<DIM>(P, rhon, vn, dv);
-    }
+  for (int off = 32; off > 0; off >>= 1) {
+    S += __shfl_xor(S, off);
+    H += __shfl_xor(H, off);
   }
-  if constexpr (KIND != kRhsOnly) {
-    wave_atomic_max_u32(A.vmax, t, __float_as_uint(vin), fluid);
-    if constexpr (FINAL) {
-      wave_atomic_min_u64(A.dt_min, t, dbl_bits(dt_cand), fluid);
-    }
+  if (lane == 0) {
+    const double Hs = fabs(S) <= tiny ? 0.0 : H / S;
+    A.s.rho[i] = eos_density_from_potential(P, Hs);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) A.s.v[d][i] = 0.0;
   }
 }
 
@@ -586,46 +474,6 @@ __global__ __launch_bounds__(kBlock) void k_neighbor_fill(
 
 // ---- dispatch helpers --------------------------------------------------------------
 
-template <int DIM, int KERNEL, int FLAVOUR>
-void stage_dispatch(hipStream_t s, const StageArgs& a, int kind, bool final) {
-  const uint32_t g = grid_for(a.n);
-  if (kind == kRhsOnly) {
-    k_stage<DIM, KERNEL, FLAVOUR, kRhsOnly, false><<<g, kBlock, 0, s>>>(a);
-  } else if (kind == kEuler) {
-    if (final) {
-      k_stage<DIM, KERNEL, FLAVOUR, kEuler, true><<<g, kBlock, 0, s>>>(a);
-    } else {
-      k_stage<DIM, KERNEL, FLAVOUR, kEuler, false><<<g, kBlock, 0, s>>>(a);
-    }
-  } else {
-    if (final) {
-      k_stage<DIM, KERNEL, FLAVOUR, kCombine, true><<<g, kBlock, 0, s>>>(a);
-    } else {
-      k_stage<DIM, KERNEL, FLAVOUR, kCombine, false><<<g, kBlock, 0, s>>>(a);
-    }
-  }
-}
-
-template <int DIM, int KERNEL>
-void stage_dispatch_flavour(hipStream_t s, const StageArgs& a, int kind,
-                            bool final) {
-  if (a.P.flavour == kFlavourCfg) {
-    stage_dispatch<DIM, KERNEL, kFlavourCfg>(s, a, kind, final);
-  } else {
-    stage_dispatch<DIM, KERNEL, kFlavourRef>(s, a, kind, final);
-  }
-}
-
-template <int DIM>
-void stage_dispatch_kernel(hipStream_t s, const StageArgs& a, int kind,
-                           bool final) {
-  if (a.P.kernel == kKernelC4) {
-    stage_dispatch_flavour<DIM, kKernelC4>(s, a, kind, final);
-  } else {
-    stage_dispatch_flavour<DIM, kKernelC2>(s, a, kind, final);
-  }
-}
-
 }  // namespace
 
 void launch_import(hipStream_t s, int dim, uint32_t n, const double* r,
@@ -678,7 +526,7 @@ void launch_compact_fixed(hipStream_t s, uint32_t n, const uint32_t* tag,
 
 void launch_boundary(hipStream_t s, const BoundaryArgs& a) {
   if (a.n_fixed == 0) return;
-  const uint32_t g = grid_for(a.n_fixed);
+  const uint32_t g = grid_for(a.n_fixed, kBlock / 64);
   if (a.P.dim == 2) {
     if (a.P.kernel == kKernelC4) {
       k_boundary<2, kKernelC4><<<g, kBlock, 0, s>>>(a);
@@ -691,15 +539,6 @@ void launch_boundary(hipStream_t s, const BoundaryArgs& a) {
     } else {
       k_boundary<3, kKernelC2><<<g, kBlock, 0, s>>>(a);
     }
-  }
-}
-
-void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final) {
-  if (a.n == 0) return;
-  if (a.P.dim == 2) {
-    stage_dispatch_kernel<2>(s, a, kind, final);
-  } else {
-    stage_dispatch_kernel<3>(s, a, kind, final);
   }
 }
 

@@ -5,6 +5,11 @@
 
 namespace tit_amd {
 
+// Longest strip (cells along the fast axis) and LDS candidate capacity of
+// the 2D strip kernel.
+constexpr int kStripMax = 64;
+constexpr int kStripLdsCap = 768;
+
 struct StageArgs {
   DevParams P;
   const DevTank* tanks;
@@ -23,6 +28,11 @@ struct StageArgs {
   // reductions
   unsigned long long* dt_min;  // per tank, FP64 bits
   uint32_t* vmax;              // per tank, FP32 bits
+  // strip schedule: work items (tank, column x, column y, first cell z)
+  const uint4* work;
+  uint32_t n_work;
+  uint32_t strip_len;          // cells per strip along the fast axis
+  uint32_t lds_cap;            // candidates staged in LDS at most
 };
 
 struct BoundaryArgs {
@@ -57,6 +67,8 @@ void launch_compact_fixed(hipStream_t s, uint32_t n, const uint32_t* tag,
                           uint32_t* fixed_list, uint32_t* counter);
 void launch_boundary(hipStream_t s, const BoundaryArgs& a);
 void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final);
+int strip_threads(int dim);
+uint32_t strip_lds_capacity(int dim);
 void launch_time_step(hipStream_t s, const DevParams& P, uint32_t n,
                       DevState st, const uint32_t* tag, const uint32_t* id,
                       const double* dv_ref, unsigned long long* dt_min);
