@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="stage-kernel targets per workgroup chunk (0: default)")
+    ap.add_argument("--lds-cap", type=int, default=-1,
+                    help="LDS candidate capacity (0: global-memory path)")
     ap.add_argument("--copy-probe-mb", type=int, default=512,
                     help="device-to-device copy of this many MiB (attainable "
                          "HBM bandwidth; reference kernel for counter scaling)")
@@ -98,6 +102,9 @@ def main():
     nf = pa.n_fluid
     ctx = ta.HipContext(eq, order=3)
     ctx.upload([pa])
+    if args.chunk or args.lds_cap >= 0:
+        ctx.set_schedule(args.chunk, None if args.lds_cap < 0 else args.lds_cap)
+    sched = ctx.get_schedule()
     print(f"bench: {nf} fluid + {pa.n_fixed} fixed particles uploaded; "
           f"warmup {args.warmup}", file=sys.stderr, flush=True)
     ctx.step(args.warmup)
@@ -143,7 +150,8 @@ def main():
                                f"{pa.n_fixed} fixed particles, Wendland C2 h=2dr, "
                                "delta-SPH 0.1, alpha 0.02, SSPRK3 CFL 0.25",
                    "fluid_particles": nf, "fixed_particles": pa.n_fixed,
-                   "sim_time_s": float(t_step[0]), "dt_next_s": float(dt_next[0])},
+                   "sim_time_s": float(t_step[0]), "dt_next_s": float(dt_next[0]),
+                   "schedule": sched},
         "roofline": {"bound": "hbm", "kernel": "k_stage (fused pair sums + SSPRK update)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

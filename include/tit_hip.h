@@ -150,20 +150,20 @@ tit_status tit_hip_set_profiling(tit_hip_ctx* ctx, int enable);
 tit_status tit_hip_kernel_stats(tit_hip_ctx* ctx, tit_hip_kernel_stat* out,
                                 int cap, int* n_out);
 
-/* Schedule of the fused stage kernel: one workgroup per strip of
- * strip_cells cells along the fast axis of one cell column; in 2D up to
- * lds_capacity candidates of a strip are staged in LDS (0 reads them from
- * global memory; results are bitwise identical either way).  occupancy is
- * the mean number of particles per occupied cell at upload.  No reference
- * counterpart (the reference partitions pairs for TBB threads instead,
- * particle_mesh.hpp:164-241). */
+/* Schedule of the fused stage kernel: each workgroup takes a chunk of at
+ * most chunk_targets consecutive particles of one cell column (one per lane);
+ * in 2D up to lds_capacity candidate positions of a chunk are staged in LDS
+ * (0 reads them from global memory; results are bitwise identical either
+ * way).  occupancy is the mean number of particles per occupied cell at
+ * upload.  No reference counterpart (the reference partitions pairs for TBB
+ * threads instead, particle_mesh.hpp:164-241). */
 typedef struct tit_hip_schedule {
-  int strip_cells;
+  int chunk_targets;
   int lds_capacity;
   double occupancy;
 } tit_hip_schedule;
 tit_status tit_hip_get_schedule(tit_hip_ctx* ctx, tit_hip_schedule* out);
-/* strip_cells = 0 keeps the current strip length. */
+/* chunk_targets = 0 keeps the current chunk length. */
 tit_status tit_hip_set_schedule(tit_hip_ctx* ctx, const tit_hip_schedule* s);
 
 /* Number of particles (all tanks) and of neighbour-index rebuilds so far. */

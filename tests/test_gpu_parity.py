@@ -199,22 +199,20 @@ def test_rebuild_interleaved_with_hooks():
 
 @pytest.mark.parametrize("dim", [2, 3])
 def test_schedules_are_bitwise_identical(dim):
-    """Any strip length, and LDS staging on or off, gives the same bits: each
+    """Any chunk length, and LDS staging on or off, gives the same bits: each
     target walks its own stencil in the same order on every path."""
     dr = 0.02 if dim == 2 else 0.03
     pa = perturbed_dam_break(dim, dr, seed=9)
     eq = _eq(dim, dr)
     outs = []
     base = _ctx(eq, [pa]).get_schedule()
-    for strip, lds in [(0, None), (1, None), (7, None), (0, 0), (64, 0)]:
-        if dim == 3 and lds is None and strip == 7:
-            continue
+    for chunk, lds in [(0, None), (1, None), (37, None), (0, 0), (64, 0), (0, 60)]:
         ctx = _ctx(eq, [pa])
-        ctx.set_schedule(strip, lds)
+        ctx.set_schedule(chunk, lds)
         ctx.step(4)
         outs.append(ctx.download())
         ctx.close()
     for o in outs[1:]:
         for f in ("r", "v", "rho"):
             assert np.array_equal(o[f], outs[0][f]), f
-    assert base["strip_cells"] >= 1 and base["occupancy"] > 1.0
+    assert base["chunk_targets"] >= 1 and base["occupancy"] > 1.0

@@ -48,7 +48,7 @@ class HostView(C.Structure):
 
 
 class Schedule(C.Structure):
-    _fields_ = [("strip_cells", C.c_int), ("lds_capacity", C.c_int),
+    _fields_ = [("chunk_targets", C.c_int), ("lds_capacity", C.c_int),
                 ("occupancy", C.c_double)]
 
 

@@ -115,13 +115,19 @@ struct tit_hip_ctx {
   double* rhs_drho = nullptr;
   double* rhs_dv[3] = {nullptr, nullptr, nullptr};
 
-  // strip schedule of the stage kernel
+  // chunk schedule of the stage kernel
   std::vector<DevTank> host_tanks;
-  uint4* work = nullptr;
-  size_t work_cap = 0;
-  uint32_t n_work = 0;
-  uint32_t strip_len = 16;
+  uint4* cols = nullptr;
+  uint32_t n_cols = 0;
+  uint4* chunks = nullptr;
+  uint32_t* overflow = nullptr;
+  uint32_t* block_tot = nullptr;
+  uint32_t* chunk_counts = nullptr;
+  size_t chunk_cap = 0;
+  uint32_t chunk_targets = kChunkMax;
   uint32_t lds_cap = 0;
+  uint32_t grid_lds = 0, grid_global = 0;
+  int n_cu = 0;
   double occupancy = 1.0;  // mean particles per occupied cell at upload
 
   // graph of two steps, valid while the rebuild parity matches
@@ -181,6 +187,7 @@ hipError_t alloc_state(tit_hip_ctx* c, DevState& s, uint32_t n, int dim) {
     if (e == hipSuccess) e = dalloc(c, &s.v[k], n);
   }
   if (e == hipSuccess) e = dalloc(c, &s.rho, n);
+  if (e == hipSuccess) e = dalloc(c, &s.aux, n);
   return e;
 }
 
@@ -283,6 +290,8 @@ __global__ void k_zero_u32(uint32_t* p, int n) {
   if (i < n) p[i] = 0u;
 }
 
+StageArgs stage_args(tit_hip_ctx* c);
+
 void enqueue_rebuild(tit_hip_ctx* c) {
   hipStream_t s = c->stream;
   {
@@ -294,7 +303,7 @@ void enqueue_rebuild(tit_hip_ctx* c) {
                    &c->val, &c->val_alt, c->sort);
   {
     ProfScope ps(c->prof, s, "permute");
-    launch_permute(s, c->dp.dim, c->n, c->val, c->A, c->T, c->id, c->id_alt,
+    launch_permute(s, c->dp, c->n, c->val, c->A, c->T, c->id, c->id_alt,
                    c->tag, c->tag_alt);
   }
   swap_state(c->A, c->T);
@@ -303,6 +312,11 @@ void enqueue_rebuild(tit_hip_ctx* c) {
   {
     ProfScope ps(c->prof, s, "cell_offsets");
     launch_cell_offsets(s, c->n_cells, c->n, c->key, c->cell_off);
+  }
+  {
+    ProfScope ps(c->prof, s, "build_chunks");
+    StageArgs a = stage_args(c);
+    launch_build_chunks(s, a, c->block_tot);
   }
   {
     ProfScope ps(c->prof, s, "compact_fixed");
@@ -329,64 +343,69 @@ void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
   launch_boundary(c->stream, a);
 }
 
-// Strip work list: every column of every tank cut into strips of strip_len
-// cells along the fast axis.
-tit_status build_work(tit_hip_ctx* c) {
+void forget(tit_hip_ctx* c, void* q) {
+  if (!q) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(q);
+}
+
+// Cell columns of every tank (2D: one per x cell, 3D: one per (x, y) cell)
+// and chunk buffers sized for the current chunk length.
+tit_status build_schedule(tit_hip_ctx* c) {
   const int dim = c->prm.dim;
-  std::vector<uint4> items;
-  const uint32_t S = c->strip_len;
+  std::vector<uint4> cols;
   for (size_t t = 0; t < c->host_tanks.size(); ++t) {
     const DevTank& T = c->host_tanks[t];
     const int nz = T.nc[dim - 1];
     const int ny = dim == 3 ? T.nc[1] : 1;
     for (int ix = 0; ix < T.nc[0]; ++ix) {
       for (int iy = 0; iy < ny; ++iy) {
-        for (int z = 0; z < nz; z += static_cast<int>(S)) {
-          items.push_back(make_uint4(static_cast<uint32_t>(t), ix, iy, z));
-        }
+        const uint32_t col_row =
+            T.cell_off + (static_cast<uint32_t>(ix) * ny + static_cast<uint32_t>(iy)) * nz;
+        cols.push_back(make_uint4(static_cast<uint32_t>(t), ix, iy, col_row));
       }
     }
   }
-  if (items.size() >= (1ull << 31)) return fail(c, TIT_E_ARG, "too many strips");
-  if (items.size() > c->work_cap) {
-    if (c->work) {
-      auto it = std::find(c->allocs.begin(), c->allocs.end(), static_cast<void*>(c->work));
-      if (it != c->allocs.end()) c->allocs.erase(it);
-      (void)hipFree(c->work);
-      c->work = nullptr;
-    }
-    TIT_HIP_TRY(c, dalloc(c, &c->work, items.size()));
-    c->work_cap = items.size();
+  if (cols.size() >= (1ull << 31)) return fail(c, TIT_E_ARG, "too many cell columns");
+  const size_t need = c->n / c->chunk_targets + cols.size() + 1;
+  if (!c->cols || cols.size() != c->n_cols) {
+    forget(c, c->cols);
+    forget(c, c->block_tot);
+    c->cols = nullptr;
+    c->block_tot = nullptr;
+    TIT_HIP_TRY(c, dalloc(c, &c->cols, cols.size()));
+    TIT_HIP_TRY(c, dalloc(c, &c->block_tot, cols.size() / 1024 + 2));
+    TIT_HIP_TRY(c, hipMemcpy(c->cols, cols.data(), cols.size() * sizeof(uint4),
+                             hipMemcpyHostToDevice));
+    c->n_cols = static_cast<uint32_t>(cols.size());
   }
-  TIT_HIP_TRY(c, hipMemcpy(c->work, items.data(), items.size() * sizeof(uint4),
-                           hipMemcpyHostToDevice));
-  c->n_work = static_cast<uint32_t>(items.size());
+  if (need > c->chunk_cap) {
+    forget(c, c->chunks);
+    forget(c, c->overflow);
+    c->chunks = nullptr;
+    c->overflow = nullptr;
+    TIT_HIP_TRY(c, dalloc(c, &c->chunks, need));
+    TIT_HIP_TRY(c, dalloc(c, &c->overflow, need));
+    c->chunk_cap = need;
+  }
+  if (!c->chunk_counts) TIT_HIP_TRY(c, dalloc(c, &c->chunk_counts, 4));
+  stage_grids(c->dp, c->n_cu, &c->grid_lds, &c->grid_global);
   drop_graph(c);
   return TIT_OK;
 }
 
-// Default strip length: about 0.9 * threads targets per strip at the upload
-// occupancy, and (2D) an LDS candidate estimate of at most 85% capacity.
-uint32_t default_strip_len(const tit_hip_ctx* c) {
-  const double occ = std::max(c->occupancy, 0.25);
-  const int nt = strip_threads(c->prm.dim);
-  int S = static_cast<int>(std::floor(0.9 * nt / occ));
-  S = std::max(1, std::min(S, kStripMax));
-  if (c->prm.dim == 2) {
-    while (S > 1 && (2 * kStencil + 1) * (S + 2 * kStencil) * occ >
-                        0.85 * strip_lds_capacity(2)) {
-      --S;
-    }
-  }
-  return static_cast<uint32_t>(S);
-}
-
 StageArgs stage_args(tit_hip_ctx* c) {
   StageArgs a{};
-  a.work = c->work;
-  a.n_work = c->n_work;
-  a.strip_len = c->strip_len;
+  a.cols = c->cols;
+  a.n_cols = c->n_cols;
+  a.chunk_targets = c->chunk_targets;
   a.lds_cap = c->lds_cap;
+  a.chunks = c->chunks;
+  a.overflow = c->overflow;
+  a.counts = c->chunk_counts;
+  a.grid_lds = c->grid_lds;
+  a.grid_global = c->grid_global;
   a.P = c->dp;
   a.tanks = c->tanks;
   a.key = c->key;
@@ -476,6 +495,12 @@ tit_status tit_hip_create(int device, const tit_hip_params* params,
     return TIT_E_HIP;
   }
   c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete c;
+    return TIT_E_HIP;
+  }
+  c->n_cu = prop.multiProcessorCount;
   if (hipSetDevice(device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -693,11 +718,15 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, hipStreamSynchronize(s));
   if (d_dv) (void)hipFree(d_dv);
   if (st != TIT_OK) return st;
-  c->work = nullptr;
-  c->work_cap = 0;
-  c->strip_len = default_strip_len(c);
-  c->lds_cap = strip_lds_capacity(dim);
-  st = build_work(c);
+  c->cols = nullptr;
+  c->chunks = nullptr;
+  c->overflow = nullptr;
+  c->block_tot = nullptr;
+  c->chunk_counts = nullptr;
+  c->n_cols = 0;
+  c->chunk_cap = 0;
+  c->lds_cap = stage_lds_capacity(dim);
+  st = build_schedule(c);
   if (st != TIT_OK) return st;
   c->uploaded = true;
   c->rebuilds = 0;
@@ -890,7 +919,7 @@ tit_status tit_hip_kernel_stats(tit_hip_ctx* c, tit_hip_kernel_stat* out, int ca
 
 tit_status tit_hip_get_schedule(tit_hip_ctx* c, tit_hip_schedule* out) {
   if (!c || !out) return TIT_E_ARG;
-  out->strip_cells = static_cast<int>(c->strip_len);
+  out->chunk_targets = static_cast<int>(c->chunk_targets);
   out->lds_capacity = static_cast<int>(c->lds_cap);
   out->occupancy = c->occupancy;
   return TIT_OK;
@@ -899,16 +928,16 @@ tit_status tit_hip_get_schedule(tit_hip_ctx* c, tit_hip_schedule* out) {
 tit_status tit_hip_set_schedule(tit_hip_ctx* c, const tit_hip_schedule* sch) {
   if (!c || !sch) return TIT_E_ARG;
   if (!c->uploaded) return fail(c, TIT_E_STATE, "schedule before upload");
-  if (sch->strip_cells < 0 || sch->strip_cells > kStripMax || sch->lds_capacity < 0 ||
-      sch->lds_capacity > static_cast<int>(strip_lds_capacity(c->prm.dim))) {
-    return fail(c, TIT_E_ARG, "strip_cells must be in [0, 64] and lds_capacity "
+  if (sch->chunk_targets < 0 || sch->chunk_targets > kChunkMax || sch->lds_capacity < 0 ||
+      sch->lds_capacity > static_cast<int>(stage_lds_capacity(c->prm.dim))) {
+    return fail(c, TIT_E_ARG, "chunk_targets must be in [0, 128] and lds_capacity "
                               "within the kernel's LDS capacity");
   }
   TIT_HIP_TRY(c, hipSetDevice(c->device));
   TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  if (sch->strip_cells > 0) c->strip_len = static_cast<uint32_t>(sch->strip_cells);
+  if (sch->chunk_targets > 0) c->chunk_targets = static_cast<uint32_t>(sch->chunk_targets);
   c->lds_cap = static_cast<uint32_t>(sch->lds_capacity);
-  return build_work(c);
+  return build_schedule(c);
 }
 
 tit_status tit_hip_counts(tit_hip_ctx* c, size_t* n_particles, uint64_t* n_rebuilds) {

@@ -58,10 +58,16 @@ __device__ __forceinline__ float pressure_over_rho2(const DevParams& P,
   return static_cast<float>(p / (rho * rho));
 }
 
+// Per-particle pair inputs kept beside rho: (p / rho^2, m / rho).  Every
+// kernel that writes a density writes this too.
+__device__ __forceinline__ float2 make_aux(const DevParams& P, double rho) {
+  return make_float2(pressure_over_rho2(P, rho), P.m_f / static_cast<float>(rho));
+}
+
 template <int DIM>
 __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
                                             const double* v, double rho,
-                                            Target<DIM>& a) {
+                                            float Pa, Target<DIM>& a) {
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
     a.x[k] = x[k];
@@ -69,7 +75,7 @@ __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
   }
   a.rho = rho;
   a.rho_f = static_cast<float>(rho);
-  a.Pa = pressure_over_rho2(P, rho);
+  a.Pa = Pa;
   a.csa = sound_speed_f32(P, a.rho_f);
 }
 
@@ -85,7 +91,9 @@ __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>&
   float r2 = x[0] * x[0];
 #pragma unroll
   for (int k = 1; k < DIM; ++k) r2 = __builtin_fmaf(x[k], x[k], r2);
-  const float r = __builtin_sqrtf(r2);
+  // hardware square root and reciprocal (about 1 ulp; the pair terms are
+  // FP32 and compared with the FP64 oracle at a stated tolerance)
+  const float r = __builtin_amdgcn_sqrtf(r2);
   const float q = r * P.inv_h;
   const float G = P.gsc * kernel_grad_poly<KERNEL>(q);
   float vr = 0.0f;
@@ -99,12 +107,12 @@ __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>&
     cont = G * __builtin_fmaf(P.m_f, vr, P.dhc2 * Vb * drab);
     const float cbar = 0.5f * (a.csa + csb);
     const float rbar = 0.5f * (a.rho_f + rbf);
-    const float visc = P.ahc * cbar * vr / (rbar * (r2 + P.eta2));
+    const float visc = P.ahc * cbar * vr * __builtin_amdgcn_rcpf(rbar * (r2 + P.eta2));
     Pi = vr < 0.0f ? visc : 0.0f;
   } else {
     const float csab = fmaxf(a.csa, csb);
     cont = G * __builtin_fmaf(P.m_f, vr, csab * drab * Vb * r);
-    Pi = P.two_mu * vr / (a.rho_f * rbf * r2);
+    Pi = P.two_mu * vr * __builtin_amdgcn_rcpf(a.rho_f * rbf * r2);
   }
   const float coef = P.m_f * (Pi - (a.Pa + Pb)) * G;
   s.rho.add(cont);

@@ -6,6 +6,7 @@
 // every stencil row along that axis is one contiguous particle range.  The
 // inclusive FP64 predicate decides membership, so the neighbour set is the
 // reference's exactly, independent of how the cells were built.
+#include "pair_math.hpp"
 #include "sph_kernels.hpp"
 
 namespace tit_amd {
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(
 
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_permute(
-    uint32_t n, const uint32_t* __restrict__ perm, DevState src, DevState dst,
+    DevParams P, uint32_t n, const uint32_t* __restrict__ perm, DevState src, DevState dst,
     const uint32_t* __restrict__ id_src, uint32_t* __restrict__ id_dst,
     const uint32_t* __restrict__ tag_src, uint32_t* __restrict__ tag_dst) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -178,7 +179,9 @@ __global__ __launch_bounds__(kBlock) void k_permute(
     dst.x[k][i] = src.x[k][s];
     dst.v[k][i] = src.v[k][s];
   }
-  dst.rho[i] = src.rho[s];
+  const double rho = src.rho[s];
+  dst.rho[i] = rho;
+  dst.aux[i] = make_aux(P, rho);
   id_dst[i] = id_src[s];
   tag_dst[i] = tag_src[s];
 }
@@ -331,7 +334,9 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   }
   if (lane == 0) {
     const double Hs = fabs(S) <= tiny ? 0.0 : H / S;
-    A.s.rho[i] = eos_density_from_potential(P, Hs);
+    const double rho_e = eos_density_from_potential(P, Hs);
+    A.s.rho[i] = rho_e;
+    A.s.aux[i] = make_aux(P, rho_e);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) A.s.v[d][i] = 0.0;
   }
@@ -498,16 +503,16 @@ void launch_cell_keys(hipStream_t s, const DevParams& P, const DevTank* tanks,
   }
 }
 
-void launch_permute(hipStream_t s, int dim, uint32_t n, const uint32_t* perm,
+void launch_permute(hipStream_t s, const DevParams& P, uint32_t n, const uint32_t* perm,
                     DevState src, DevState dst, const uint32_t* id_src,
                     uint32_t* id_dst, const uint32_t* tag_src,
                     uint32_t* tag_dst) {
   if (n == 0) return;
-  if (dim == 2) {
-    k_permute<2><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst, id_src,
+  if (P.dim == 2) {
+    k_permute<2><<<grid_for(n), kBlock, 0, s>>>(P, n, perm, src, dst, id_src,
                                                 id_dst, tag_src, tag_dst);
   } else {
-    k_permute<3><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst, id_src,
+    k_permute<3><<<grid_for(n), kBlock, 0, s>>>(P, n, perm, src, dst, id_src,
                                                 id_dst, tag_src, tag_dst);
   }
 }

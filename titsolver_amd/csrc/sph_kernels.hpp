@@ -5,10 +5,10 @@
 
 namespace tit_amd {
 
-// Longest strip (cells along the fast axis) and LDS candidate capacity of
-// the 2D strip kernel.
-constexpr int kStripMax = 64;
-constexpr int kStripLdsCap = 768;
+// Largest chunk (targets per workgroup) and LDS position capacity of the
+// 2D stage kernel.
+constexpr int kChunkMax = 128;
+constexpr int kStageLdsCap = 1024;
 
 struct StageArgs {
   DevParams P;
@@ -28,11 +28,17 @@ struct StageArgs {
   // reductions
   unsigned long long* dt_min;  // per tank, FP64 bits
   uint32_t* vmax;              // per tank, FP32 bits
-  // strip schedule: work items (tank, column x, column y, first cell z)
-  const uint4* work;
-  uint32_t n_work;
-  uint32_t strip_len;          // cells per strip along the fast axis
-  uint32_t lds_cap;            // candidates staged in LDS at most
+  // chunk schedule: cell columns (tank, ix, iy, first cell) and, rebuilt on
+  // the device after every cell rebuild, chunks (column, first particle,
+  // end particle) of at most chunk_targets consecutive particles of a column
+  const uint4* cols;
+  uint32_t n_cols;
+  uint32_t chunk_targets;
+  uint32_t lds_cap;            // candidate positions staged in LDS at most
+  uint4* chunks;
+  uint32_t* overflow;          // 2D chunks whose candidates exceed lds_cap
+  uint32_t* counts;            // [0] chunks, [1] overflow chunks
+  uint32_t grid_lds, grid_global;  // persistent grid sizes
 };
 
 struct BoundaryArgs {
@@ -57,7 +63,7 @@ void launch_import(hipStream_t s, int dim, uint32_t n, const double* r,
 void launch_cell_keys(hipStream_t s, const DevParams& P, const DevTank* tanks,
                       uint32_t n, DevState st, const uint32_t* tag,
                       uint32_t* key, uint32_t* flags);
-void launch_permute(hipStream_t s, int dim, uint32_t n, const uint32_t* perm,
+void launch_permute(hipStream_t s, const DevParams& P, uint32_t n, const uint32_t* perm,
                     DevState src, DevState dst, const uint32_t* id_src,
                     uint32_t* id_dst, const uint32_t* tag_src,
                     uint32_t* tag_dst);
@@ -67,8 +73,11 @@ void launch_compact_fixed(hipStream_t s, uint32_t n, const uint32_t* tag,
                           uint32_t* fixed_list, uint32_t* counter);
 void launch_boundary(hipStream_t s, const BoundaryArgs& a);
 void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final);
-int strip_threads(int dim);
-uint32_t strip_lds_capacity(int dim);
+uint32_t stage_lds_capacity(int dim);
+void stage_grids(const DevParams& P, int n_cu, uint32_t* grid_lds,
+                 uint32_t* grid_global);
+// Rebuilds the chunk list from the cell offsets (after a cell rebuild).
+void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot);
 void launch_time_step(hipStream_t s, const DevParams& P, uint32_t n,
                       DevState st, const uint32_t* tag, const uint32_t* id,
                       const double* dv_ref, unsigned long long* dt_min);

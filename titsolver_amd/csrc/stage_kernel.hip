@@ -1,21 +1,24 @@
 // Fused SSPRK stage kernel: continuity + momentum pair sums, EOS, update.
 //
-// Work is split into strips: one workgroup takes S consecutive cells along
-// the fast (last) axis of one cell column of one tank, and every particle in
-// those cells is a target.  In 2D the candidates of the whole strip (5
-// neighbouring columns, S + 4 rows each) are staged in LDS once: FP64
-// positions for the exact predicate, FP32 velocity, FP64 density and the
-// per-particle EOS terms p/rho^2 and m/rho.  Each target lane then walks its
-// own 5 x 5 cell stencil in LDS in two passes per column range: the FP64
-// predicate builds a 64-bit mask of accepted candidates, and the FP32 pair
-// terms run over the set bits only, so rejected candidates never occupy the
-// pair arithmetic.  A strip whose candidates exceed the LDS capacity, and 3D,
-// read the same candidates from global memory in the same order, so every
-// path produces bitwise identical sums.
+// Work unit: a chunk of at most chunk_targets consecutive (cell-sorted)
+// particles of one cell column of one tank, one particle per lane.  After
+// every cell rebuild the chunks are recut on the device so that each column's
+// particles split into equal chunks; a persistent grid then walks the chunk
+// list in XCD-major bands (launch index b runs on XCD b % 8 and takes chunks
+// from that XCD's eighth of the list, so each XCD's L2 holds a contiguous
+// band of columns).
 //
-// Workgroups are assigned XCD-major: launch index b runs on XCD b % 8, and
-// takes strip (b % 8) * ceil(n/8) + b / 8, so each XCD's L2 sees a contiguous
-// band of columns and the halo columns it stages are mostly its own.
+// 2D: the FP64 positions of the chunk's candidates (the 5 neighbouring
+// columns, the chunk's rows +- 2) are staged in LDS.  Each lane walks its own
+// 5 x 5 cell stencil one column range at a time: the exact FP64 predicate
+// (four LDS loads per batch) builds a 64-bit acceptance mask, and the FP32
+// pair terms run over the accepted candidates only, reading their velocity,
+// density and (p/rho^2, m/rho) from L2 one candidate ahead of the arithmetic.
+// Chunks whose candidates exceed the LDS capacity, and 3D, take positions
+// from global memory too.  Every path visits candidates in the same order and
+// rounds identically, so all schedules produce the same bits.
+#include <algorithm>
+
 #include "pair_math.hpp"
 #include "sph_kernels.hpp"
 
@@ -24,34 +27,49 @@ namespace tit_amd {
 namespace {
 
 constexpr int kCols2 = 2 * kStencil + 1;
+constexpr int kRowsMax = 128;  // LDS cell-offset rows of one chunk
 
-template <int CAP>
-struct StripLds {
-  double2 xy[CAP];
-  float2 v[CAP];
-  double rho[CAP];
-  float P[CAP];
-  float V[CAP];
+// Positions from LDS (2D).
+struct LdsPos {
+  const double2* xy;
+  const uint32_t (*coff)[kRowsMax + 1];  // [column][row - zlo]
+  const uint32_t* gb;                    // global index of each column's row zlo
+  const uint32_t* lb;                    // LDS index of each column's row zlo
+  int zlo, nz, ix, ncx;
+
+  __device__ __forceinline__ bool range(int k, int zc, uint32_t& b,
+                                        uint32_t& e) const {
+    const int ixk = ix - kStencil + k;
+    if (ixk < 0 || ixk >= ncx) return false;
+    const int z0 = max(zc - kStencil, 0), z1 = min(zc + kStencil, nz - 1);
+    b = coff[k][z0 - zlo];
+    e = coff[k][z1 + 1 - zlo];
+    return true;
+  }
+  __device__ __forceinline__ void pos(int k, uint32_t j, double* x) const {
+    const double2 p = xy[lb[k] + (j - gb[k])];
+    x[0] = p.x;
+    x[1] = p.y;
+  }
 };
 
-// Candidate source reading global memory (any dimension).
+// Positions from global memory (any dimension).
 template <int DIM>
-struct GlobalSrc {
-  const DevParams& P;
+struct GlobalPos {
   const DevTank& T;
   const DevState& S;
   const uint32_t* coff;
-  int c[3];  // target column (and the fast-axis cell is passed per call)
+  int cx, cy;  // target column
 
   __device__ __forceinline__ bool range(int k, int zc, uint32_t& b,
                                         uint32_t& e) const {
     constexpr int L = DIM - 1;
     int ix, iy = 0;
     if constexpr (DIM == 2) {
-      ix = c[0] - kStencil + k;
+      ix = cx - kStencil + k;
     } else {
-      ix = c[0] - kStencil + k / kCols2;
-      iy = c[1] - kStencil + k % kCols2;
+      ix = cx - kStencil + k / kCols2;
+      iy = cy - kStencil + k % kCols2;
       if (iy < 0 || iy >= T.nc[1]) return false;
     }
     if (ix < 0 || ix >= T.nc[0]) return false;
@@ -72,92 +90,98 @@ struct GlobalSrc {
 #pragma unroll
     for (int k = 0; k < DIM; ++k) x[k] = S.x[k][j];
   }
-  __device__ __forceinline__ void data(int, uint32_t j, float* v, double& rho,
-                                       float& Pb, float& Vb) const {
+};
+
+template <int DIM>
+struct PairIn {
+  double x[DIM];
+  float v[DIM];
+  double rho;
+  float2 aux;
+};
+
+template <int DIM, class Pos>
+__device__ __forceinline__ void load_pair(const Pos& ps, const DevState& S, int k,
+                                          uint32_t j, PairIn<DIM>& p) {
+  ps.pos(k, j, p.x);
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) v[k] = static_cast<float>(S.v[k][j]);
-    rho = S.rho[j];
-    Pb = pressure_over_rho2(P, rho);
-    Vb = P.m_f / static_cast<float>(rho);
-  }
-};
+  for (int q = 0; q < DIM; ++q) p.v[q] = static_cast<float>(S.v[q][j]);
+  p.rho = S.rho[j];
+  p.aux = S.aux[j];
+}
 
-// Candidate source reading a strip staged in LDS (2D).
-template <int CAP, int ROWS>
-struct LdsSrc {
-  const StripLds<CAP>& L;
-  const uint32_t (*coff)[ROWS];  // [column][row - zlo]
-  const uint32_t* gb;            // global index of each column's first row
-  const uint32_t* lb;            // LDS index of each column's first row
-  int zlo, nz;
-  bool valid[kCols2];
+template <int DIM, int KERNEL, int FLAVOUR>
+__device__ __forceinline__ void sum_pair(const DevParams& P, const Target<DIM>& a,
+                                         const PairIn<DIM>& p, Sums<DIM>& acc) {
+  double d[DIM];
+#pragma unroll
+  for (int q = 0; q < DIM; ++q) d[q] = p.x[q] - a.x[q];
+  pair_term<DIM, KERNEL, FLAVOUR>(P, a, d, p.v, p.rho, p.aux.x, p.aux.y, acc);
+}
 
-  __device__ __forceinline__ bool range(int k, int zc, uint32_t& b,
-                                        uint32_t& e) const {
-    if (!valid[k]) return false;
-    const int z0 = max(zc - kStencil, 0), z1 = min(zc + kStencil, nz - 1);
-    b = coff[k][z0 - zlo];
-    e = coff[k][z1 + 1 - zlo];
-    return true;
+template <int DIM>
+__device__ __forceinline__ bool within(const DevParams& P, const double* xb,
+                                       const double* xa) {
+  const double d0 = xb[0] - xa[0];
+  double s = d0 * d0;
+#pragma unroll
+  for (int q = 1; q < DIM; ++q) {
+    const double dq = xb[q] - xa[q];
+    s = s + dq * dq;
   }
-  __device__ __forceinline__ uint32_t lds(int k, uint32_t j) const {
-    return lb[k] + (j - gb[k]);
-  }
-  __device__ __forceinline__ void pos(int k, uint32_t j, double* x) const {
-    const double2 p = L.xy[lds(k, j)];
-    x[0] = p.x;
-    x[1] = p.y;
-  }
-  __device__ __forceinline__ void data(int k, uint32_t j, float* v, double& rho,
-                                       float& Pb, float& Vb) const {
-    const uint32_t l = lds(k, j);
-    const float2 vv = L.v[l];
-    v[0] = vv.x;
-    v[1] = vv.y;
-    rho = L.rho[l];
-    Pb = L.P[l];
-    Vb = L.V[l];
-  }
-};
+  return s <= P.R2;
+}
 
-// Pair sums of one target over its stencil, column range by column range:
-// an FP64 predicate pass builds the acceptance mask of up to 64 candidates,
-// then the pair terms run over the accepted ones.
-template <int DIM, int KERNEL, int FLAVOUR, int NCOL, class Src>
-__device__ __forceinline__ void gather(const DevParams& P, const Src& src,
-                                       const Target<DIM>& a, uint32_t i, int zc,
-                                       Sums<DIM>& acc) {
+__device__ __forceinline__ uint32_t pop_lowest(uint64_t& mask) {
+  const uint32_t t = static_cast<uint32_t>(__builtin_ctzll(mask));
+  mask &= mask - 1;
+  return t;
+}
+
+// Pair sums of one target over its stencil, column range by column range.
+template <int DIM, int KERNEL, int FLAVOUR, int NCOL, class Pos>
+__device__ __forceinline__ void gather(const DevParams& P, const Pos& ps,
+                                       const DevState& S, const Target<DIM>& a,
+                                       uint32_t i, int zc, Sums<DIM>& acc) {
   for (int k = 0; k < NCOL; ++k) {
     uint32_t b, e;
-    if (!src.range(k, zc, b, e)) continue;
+    if (!ps.range(k, zc, b, e)) continue;
     for (uint32_t c0 = b; c0 < e; c0 += 64) {
       const uint32_t len = min(e - c0, 64u);
       uint64_t mask = 0;
-      for (uint32_t t = 0; t < len; ++t) {
-        const uint32_t j = c0 + t;
-        double xb[DIM];
-        src.pos(k, j, xb);
-        double d0 = xb[0] - a.x[0];
-        double s = d0 * d0;
-#pragma unroll
-        for (int q = 1; q < DIM; ++q) {
-          const double dq = xb[q] - a.x[q];
-          s = s + dq * dq;
-        }
-        if (s <= P.R2 && j != i) mask |= 1ull << t;
+      uint32_t t = 0;
+      for (; t + 4 <= len; t += 4) {
+        double x0[DIM], x1[DIM], x2[DIM], x3[DIM];
+        ps.pos(k, c0 + t, x0);
+        ps.pos(k, c0 + t + 1, x1);
+        ps.pos(k, c0 + t + 2, x2);
+        ps.pos(k, c0 + t + 3, x3);
+        const uint64_t m = (within<DIM>(P, x0, a.x) ? 1ull : 0ull) |
+                           (within<DIM>(P, x1, a.x) ? 2ull : 0ull) |
+                           (within<DIM>(P, x2, a.x) ? 4ull : 0ull) |
+                           (within<DIM>(P, x3, a.x) ? 8ull : 0ull);
+        mask |= m << t;
       }
-      while (mask) {
-        const uint32_t t = static_cast<uint32_t>(__builtin_ctzll(mask));
-        mask &= mask - 1;
-        const uint32_t j = c0 + t;
-        double xb[DIM], d[DIM];
-        src.pos(k, j, xb);
-#pragma unroll
-        for (int q = 0; q < DIM; ++q) d[q] = xb[q] - a.x[q];
-        float vb[DIM], Pb, Vb;
-        double rho_b;
-        src.data(k, j, vb, rho_b, Pb, Vb);
-        pair_term<DIM, KERNEL, FLAVOUR>(P, a, d, vb, rho_b, Pb, Vb, acc);
+      for (; t < len; ++t) {
+        double x0[DIM];
+        ps.pos(k, c0 + t, x0);
+        if (within<DIM>(P, x0, a.x)) mask |= 1ull << t;
+      }
+      if (i >= c0 && i < c0 + len) mask &= ~(1ull << (i - c0));  // not self
+      if (!mask) continue;
+      // Two register sets alternate: the next accepted candidate's loads are
+      // in flight while the current one is summed.
+      PairIn<DIM> A, B;
+      load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), A);
+      while (true) {
+        const bool hb = mask != 0;
+        if (hb) load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), B);
+        sum_pair<DIM, KERNEL, FLAVOUR>(P, a, A, acc);
+        if (!hb) break;
+        const bool ha = mask != 0;
+        if (ha) load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), A);
+        sum_pair<DIM, KERNEL, FLAVOUR>(P, a, B, acc);
+        if (!ha) break;
       }
     }
   }
@@ -194,237 +218,377 @@ __device__ __forceinline__ double particle_dt(const DevParams& P, double rho,
   return fmin(dt, P.c_force * sqrt(P.h / an));
 }
 
-constexpr int kRowsMax = kStripMax + 2 * kStencil + 1;
+struct ChunkInfo {
+  uint32_t t, col_row, t0, t1;
+  int cx, cy, nz;
+  DevTank T;
+};
 
-template <int DIM, int KERNEL, int FLAVOUR, int KIND, bool FINAL, int WAVES,
-          int CAP>
-__global__ __launch_bounds__(64 * WAVES) void k_strip(StageArgs A) {
-  constexpr int NT = 64 * WAVES;
-  constexpr bool kLds = DIM == 2 && CAP > 0;
+template <int DIM>
+__device__ __forceinline__ ChunkInfo chunk_info(const StageArgs& A, uint32_t q) {
+  ChunkInfo c;
+  const uint4 ch = A.chunks[q];
+  const uint4 col = A.cols[ch.x];
+  c.t = col.x;
+  c.T = A.tanks[c.t];
+  c.cx = static_cast<int>(col.y);
+  c.cy = static_cast<int>(col.z);
+  c.col_row = col.w;
+  c.nz = c.T.nc[DIM - 1];
+  c.t0 = ch.y;
+  c.t1 = ch.z;
+  return c;
+}
+
+// Targets of one chunk (one per lane): pair sums, then the SSPRK substep /
+// combination of time_integrator.hpp:188-223 in FP64 and, on the last stage,
+// the next step's CFL candidate; per-wave reductions at the end.
+template <int DIM, int KERNEL, int FLAVOUR, int KIND, bool FINAL, class Pos>
+__device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInfo& c,
+                                              const Pos& ps) {
   constexpr int NCOL = DIM == 2 ? kCols2 : kCols2 * kCols2;
-  __shared__ StripLds<kLds ? CAP : 1> lds;
-  __shared__ uint32_t s_coff[kLds ? kCols2 : 1][kLds ? kRowsMax : 1];
-  __shared__ uint32_t s_gb[kCols2], s_lb[kCols2 + 1];
-  __shared__ int s_use_lds;
-
   const DevParams& P = A.P;
-  const uint32_t per = (A.n_work + 7) / 8;
-  const uint32_t w = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  if (w >= A.n_work) return;
-  const uint4 item = A.work[w];  // tank, column x, column y, first cell z
-  const uint32_t t = item.x;
-  const DevTank T = A.tanks[t];
-  constexpr int L = DIM - 1;
-  const int nz = T.nc[L];
-  const int z0 = static_cast<int>(item.w);
-  const int z1 = min(z0 + static_cast<int>(A.strip_len), nz);
-  uint32_t col_row;
-  if constexpr (DIM == 2) {
-    col_row = T.cell_off + item.y * static_cast<uint32_t>(nz);
-  } else {
-    col_row = T.cell_off + (item.y * T.nc[1] + item.z) * static_cast<uint32_t>(nz);
+  const uint32_t i = c.t0 + threadIdx.x;
+  const bool in_range = i < c.t1;
+  const uint32_t tag = in_range ? A.tag[i] : 1u;
+  const bool fluid = in_range && !(tag & 1u);
+  float vmax = 0.0f;
+  unsigned long long dtmin = ~0ull;
+  if (in_range && !fluid) {
+    if constexpr (KIND == kRhsOnly) {
+      A.rhs_drho[i] = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) A.rhs_dv[k][i] = 0.0;
+    } else {
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        A.out.x[k][i] = A.in.x[k][i];
+        A.out.v[k][i] = 0.0;
+      }
+      A.out.rho[i] = A.in.rho[i];
+      A.out.aux[i] = A.in.aux[i];
+    }
   }
-  const uint32_t t0 = A.cell_off[col_row + z0], t1 = A.cell_off[col_row + z1];
-  if (t0 == t1) return;
-  const int tid = threadIdx.x;
-  const int ix = static_cast<int>(item.y);
+  if (fluid) {
+    double xa[DIM], va[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      xa[k] = A.in.x[k][i];
+      va[k] = A.in.v[k][i];
+    }
+    const double rho_a = A.in.rho[i];
+    Target<DIM> a;
+    make_target<DIM>(P, xa, va, rho_a, A.in.aux[i].x, a);
+    const int zc = static_cast<int>(A.key[i] - c.col_row);
+    Sums<DIM> acc;
+    acc.init();
+    gather<DIM, KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, acc);
+    const double drho = static_cast<double>(acc.rho.value());
+    double dv[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) dv[k] = static_cast<double>(acc.v[k].value());
+    dv[1] = dv[1] - P.g;
+    if constexpr (KIND == kRhsOnly) {
+      A.rhs_drho[i] = drho;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) A.rhs_dv[k][i] = dv[k];
+    } else {
+      double v2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) v2 = v2 + va[k] * va[k];
+      vmax = static_cast<float>(sqrt(v2));
+      const double h = A.dt[c.t];
+      double rn[DIM], vn[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        rn[k] = xa[k] + h * va[k];
+        vn[k] = va[k] + h * dv[k];
+      }
+      double rhon = rho_a + h * drho;
+      if constexpr (KIND == kCombine) {
+        const double wn = A.w, wo = 1.0 - A.w;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+          rn[k] = wo * A.old.x[k][i] + wn * rn[k];
+          vn[k] = wo * A.old.v[k][i] + wn * vn[k];
+        }
+        rhon = wo * A.old.rho[i] + wn * rhon;
+      }
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        A.out.x[k][i] = rn[k];
+        A.out.v[k][i] = vn[k];
+      }
+      A.out.rho[i] = rhon;
+      A.out.aux[i] = make_aux(P, rhon);
+      if constexpr (FINAL) dtmin = dbl_bits(particle_dt<DIM>(P, rhon, vn, dv));
+    }
+  }
+  if constexpr (KIND != kRhsOnly) {
+    // all targets of a chunk belong to one tank: one atomic per wave
+    const float vm = wave_max_f32(vmax);
+    if ((threadIdx.x & 63) == 0 && vm > 0.0f) atomicMax(A.vmax + c.t, __float_as_uint(vm));
+    if constexpr (FINAL) {
+      const unsigned long long dm = wave_min_u64(dtmin);
+      if ((threadIdx.x & 63) == 0 && dm != ~0ull) atomicMin(A.dt_min + c.t, dm);
+    }
+  }
+}
 
-  bool use_lds = false;
-  const int zlo = max(z0 - kStencil, 0), zhi = min(z1 - 1 + kStencil, nz - 1);
-  if constexpr (kLds) {
+struct Band {
+  uint32_t lo, hi, step;
+};
+__device__ __forceinline__ Band xcd_band(uint32_t n) {
+  const uint32_t x = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+  Band b;
+  b.lo = static_cast<uint32_t>((static_cast<uint64_t>(n) * x) / 8) + slot;
+  b.hi = static_cast<uint32_t>((static_cast<uint64_t>(n) * (x + 1)) / 8);
+  b.step = gridDim.x >> 3;
+  return b;
+}
+
+// 2D chunks with LDS-staged positions; chunks that do not fit go to the
+// overflow list.
+template <int KERNEL, int FLAVOUR, int KIND, bool FINAL, int NT, int CAP>
+__global__ __launch_bounds__(NT) void k_stage_lds(StageArgs A) {
+  __shared__ double2 s_xy[CAP];
+  __shared__ uint32_t s_coff[kCols2][kRowsMax + 1];
+  __shared__ uint32_t s_gb[kCols2], s_lb[kCols2 + 1];
+  __shared__ int s_fits;
+  const int tid = threadIdx.x;
+  const Band band = xcd_band(A.counts[0]);
+  for (uint32_t q = band.lo; q < band.hi; q += band.step) {
+    const ChunkInfo c = chunk_info<2>(A, q);
+    const int zf = static_cast<int>(A.key[c.t0] - c.col_row);
+    const int zl = static_cast<int>(A.key[c.t1 - 1] - c.col_row);
+    const int zlo = max(zf - kStencil, 0), zhi = min(zl + kStencil, c.nz - 1);
     const int nrows = zhi - zlo + 1;
-    for (int q = tid; q < kCols2 * (nrows + 1); q += NT) {
-      const int k = q / (nrows + 1), r = q % (nrows + 1);
-      const int ixk = ix - kStencil + k;
-      s_coff[k][r] = (ixk >= 0 && ixk < T.nc[0])
-                         ? A.cell_off[T.cell_off + static_cast<uint32_t>(ixk) * nz + zlo + r]
-                         : 0u;
+    __syncthreads();  // the previous chunk's LDS reads are done
+    const bool rows_fit = nrows <= kRowsMax;
+    if (rows_fit) {
+      for (int r = tid; r < kCols2 * (nrows + 1); r += NT) {
+        const int k = r / (nrows + 1), row = r % (nrows + 1);
+        const int ixk = c.cx - kStencil + k;
+        s_coff[k][row] =
+            (ixk >= 0 && ixk < c.T.nc[0])
+                ? A.cell_off[c.T.cell_off + static_cast<uint32_t>(ixk) * c.nz + zlo + row]
+                : 0u;
+      }
     }
     __syncthreads();
     if (tid == 0) {
       uint32_t lb = 0;
-      for (int k = 0; k < kCols2; ++k) {
-        s_gb[k] = s_coff[k][0];
-        s_lb[k] = lb;
-        lb += s_coff[k][nrows] - s_coff[k][0];
+      if (rows_fit) {
+        for (int k = 0; k < kCols2; ++k) {
+          s_gb[k] = s_coff[k][0];
+          s_lb[k] = lb;
+          lb += s_coff[k][nrows] - s_coff[k][0];
+        }
+        s_lb[kCols2] = lb;
       }
-      s_lb[kCols2] = lb;
-      s_use_lds = lb <= A.lds_cap;
+      s_fits = rows_fit && lb <= A.lds_cap;
+      if (!s_fits) A.overflow[atomicAdd(A.counts + 1, 1u)] = q;
     }
     __syncthreads();
-    use_lds = s_use_lds;
-    if (use_lds) {
-      const uint32_t total = s_lb[kCols2];
-      for (uint32_t l = tid; l < total; l += NT) {
-        int k = 0;
-        while (l >= s_lb[k + 1]) ++k;
-        const uint32_t j = s_gb[k] + (l - s_lb[k]);
-        const double rho = A.in.rho[j];
-        lds.xy[l] = make_double2(A.in.x[0][j], A.in.x[1][j]);
-        lds.v[l] = make_float2(static_cast<float>(A.in.v[0][j]),
-                               static_cast<float>(A.in.v[1][j]));
-        lds.rho[l] = rho;
-        lds.P[l] = pressure_over_rho2(P, rho);
-        lds.V[l] = P.m_f / static_cast<float>(rho);
-      }
-      __syncthreads();
+    if (!s_fits) continue;
+    const uint32_t total = s_lb[kCols2];
+    for (uint32_t l = tid; l < total; l += NT) {
+      int k = 0;
+      while (l >= s_lb[k + 1]) ++k;
+      const uint32_t j = s_gb[k] + (l - s_lb[k]);
+      s_xy[l] = make_double2(A.in.x[0][j], A.in.x[1][j]);
     }
-  }
-
-  float vmax = 0.0f;
-  unsigned long long dtmin = ~0ull;
-  for (uint32_t base = t0; base < t1; base += NT) {
-    const uint32_t i = base + tid;
-    const bool in_range = i < t1;
-    const uint32_t tag = in_range ? A.tag[i] : 1u;
-    const bool fluid = in_range && !(tag & 1u);
-    if (in_range && !fluid) {
-      if constexpr (KIND == kRhsOnly) {
-        A.rhs_drho[i] = 0.0;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) A.rhs_dv[k][i] = 0.0;
-      } else {
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-          A.out.x[k][i] = A.in.x[k][i];
-          A.out.v[k][i] = 0.0;
-        }
-        A.out.rho[i] = A.in.rho[i];
-      }
-    }
-    if (fluid) {
-      double xa[DIM], va[DIM];
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        xa[k] = A.in.x[k][i];
-        va[k] = A.in.v[k][i];
-      }
-      const double rho_a = A.in.rho[i];
-      Target<DIM> a;
-      make_target<DIM>(P, xa, va, rho_a, a);
-      const int zc = static_cast<int>(A.key[i] - col_row);
-      Sums<DIM> acc;
-      acc.init();
-      if constexpr (kLds) {
-        if (use_lds) {
-          LdsSrc<CAP, kRowsMax> src{lds, s_coff, s_gb, s_lb, zlo, nz, {}};
-#pragma unroll
-          for (int k = 0; k < kCols2; ++k) {
-            const int ixk = ix - kStencil + k;
-            src.valid[k] = ixk >= 0 && ixk < T.nc[0];
-          }
-          gather<DIM, KERNEL, FLAVOUR, NCOL>(P, src, a, i, zc, acc);
-        }
-      }
-      if (!use_lds) {
-        GlobalSrc<DIM> src{P, T, A.in, A.cell_off, {ix, static_cast<int>(item.z), 0}};
-        gather<DIM, KERNEL, FLAVOUR, NCOL>(P, src, a, i, zc, acc);
-      }
-      const double drho = static_cast<double>(acc.rho.value());
-      double dv[DIM];
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) dv[k] = static_cast<double>(acc.v[k].value());
-      dv[1] = dv[1] - P.g;
-      if constexpr (KIND == kRhsOnly) {
-        A.rhs_drho[i] = drho;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) A.rhs_dv[k][i] = dv[k];
-      } else {
-        double v2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) v2 = v2 + va[k] * va[k];
-        vmax = fmaxf(vmax, static_cast<float>(sqrt(v2)));
-        const double h = A.dt[t];
-        double rn[DIM], vn[DIM];
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-          rn[k] = xa[k] + h * va[k];
-          vn[k] = va[k] + h * dv[k];
-        }
-        double rhon = rho_a + h * drho;
-        if constexpr (KIND == kCombine) {
-          const double wn = A.w, wo = 1.0 - A.w;
-#pragma unroll
-          for (int k = 0; k < DIM; ++k) {
-            rn[k] = wo * A.old.x[k][i] + wn * rn[k];
-            vn[k] = wo * A.old.v[k][i] + wn * vn[k];
-          }
-          rhon = wo * A.old.rho[i] + wn * rhon;
-        }
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-          A.out.x[k][i] = rn[k];
-          A.out.v[k][i] = vn[k];
-        }
-        A.out.rho[i] = rhon;
-        if constexpr (FINAL) {
-          const unsigned long long b = dbl_bits(particle_dt<DIM>(P, rhon, vn, dv));
-          dtmin = b < dtmin ? b : dtmin;
-        }
-      }
-    }
-  }
-  if constexpr (KIND != kRhsOnly) {
-    // every target of the strip belongs to tank t: one atomic per wave
-    vmax = wave_max_f32(vmax);
-    if ((tid & 63) == 0 && vmax > 0.0f) atomicMax(A.vmax + t, __float_as_uint(vmax));
-    if constexpr (FINAL) {
-      dtmin = wave_min_u64(dtmin);
-      if ((tid & 63) == 0 && dtmin != ~0ull) atomicMin(A.dt_min + t, dtmin);
-    }
+    __syncthreads();
+    const LdsPos ps{s_xy, s_coff, s_gb, s_lb, zlo, c.nz, c.cx, c.T.nc[0]};
+    chunk_targets<2, KERNEL, FLAVOUR, KIND, FINAL>(A, c, ps);
   }
 }
 
-template <int DIM, int KERNEL, int FLAVOUR, int WAVES, int CAP>
+// Chunks with positions from global memory: 3D, and 2D overflow chunks.
+template <int DIM, int KERNEL, int FLAVOUR, int KIND, bool FINAL, int NT>
+__global__ __launch_bounds__(NT) void k_stage_global(StageArgs A, const uint32_t* list,
+                                                     const uint32_t* count) {
+  const Band band = xcd_band(*count);
+  for (uint32_t q = band.lo; q < band.hi; q += band.step) {
+    const ChunkInfo c = chunk_info<DIM>(A, list ? list[q] : q);
+    const GlobalPos<DIM> ps{c.T, A.in, A.cell_off, c.cx, c.cy};
+    chunk_targets<DIM, KERNEL, FLAVOUR, KIND, FINAL>(A, c, ps);
+  }
+}
+
+// ---- chunk list (after every cell rebuild) ----------------------------------
+
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ uint32_t column_chunks(const StageArgs& A, uint32_t col_i,
+                                                  uint32_t* beg, uint32_t* n) {
+  const uint4 col = A.cols[col_i];
+  const DevTank& T = A.tanks[col.x];
+  const int nz = T.nc[A.P.dim - 1];
+  *beg = A.cell_off[col.w];
+  *n = A.cell_off[col.w + nz] - *beg;
+  return (*n + A.chunk_targets - 1) / A.chunk_targets;
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh,
+                                                    uint32_t* total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    const uint32_t o = tid >= off ? sh[tid - off] : 0u;
+    __syncthreads();
+    sh[tid] += o;
+    __syncthreads();
+  }
+  *total = sh[kScanBlock - 1];
+  return sh[tid] - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_chunk_count(StageArgs A,
+                                                            uint32_t* block_tot) {
+  __shared__ uint32_t sh[kScanBlock];
+  const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t beg, n, nch = 0;
+  if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
+  uint32_t tot;
+  (void)block_excl_scan(nch, sh, &tot);
+  if (threadIdx.x == 0) block_tot[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_chunk_scan(uint32_t n_blocks,
+                                                           uint32_t* block_tot,
+                                                           uint32_t* counts) {
+  __shared__ uint32_t sh[kScanBlock];
+  const uint32_t per = (n_blocks + kScanBlock - 1) / kScanBlock;
+  const uint32_t b0 = min(threadIdx.x * per, n_blocks), b1 = min(b0 + per, n_blocks);
+  uint32_t s = 0;
+  for (uint32_t k = b0; k < b1; ++k) s += block_tot[k];
+  uint32_t tot;
+  uint32_t excl = block_excl_scan(s, sh, &tot);
+  for (uint32_t k = b0; k < b1; ++k) {
+    const uint32_t v = block_tot[k];
+    block_tot[k] = excl;
+    excl += v;
+  }
+  if (threadIdx.x == 0) {
+    counts[0] = tot;  // chunks
+    counts[1] = 0;    // overflow chunks (appended by the stage kernel)
+  }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_chunk_emit(StageArgs A,
+                                                           const uint32_t* block_off) {
+  __shared__ uint32_t sh[kScanBlock];
+  const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t beg = 0, n = 0, nch = 0;
+  if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
+  uint32_t tot;
+  const uint32_t off = block_off[blockIdx.x] + block_excl_scan(nch, sh, &tot);
+  for (uint32_t k = 0; k < nch; ++k) {
+    const uint32_t p0 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * k) / nch);
+    const uint32_t p1 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * (k + 1)) / nch);
+    A.chunks[off + k] = make_uint4(c, p0, p1, 0u);
+  }
+}
+
+__global__ void k_reset_overflow(uint32_t* counts) { counts[1] = 0u; }
+
+// ---- dispatch -------------------------------------------------------------------
+
+constexpr int kThreads = kChunkMax;
+
+template <int DIM, int KERNEL, int FLAVOUR, int KIND, bool FINAL>
+void launch_kind(hipStream_t s, const StageArgs& a) {
+  if constexpr (DIM == 2) {
+    k_reset_overflow<<<1, 1, 0, s>>>(a.counts);
+    k_stage_lds<KERNEL, FLAVOUR, KIND, FINAL, kThreads, kStageLdsCap>
+        <<<a.grid_lds, kThreads, 0, s>>>(a);
+    k_stage_global<2, KERNEL, FLAVOUR, KIND, FINAL, kThreads>
+        <<<a.grid_global, kThreads, 0, s>>>(a, a.overflow, a.counts + 1);
+  } else {
+    k_stage_global<3, KERNEL, FLAVOUR, KIND, FINAL, kThreads>
+        <<<a.grid_global, kThreads, 0, s>>>(a, nullptr, a.counts);
+  }
+}
+
+template <int DIM, int KERNEL, int FLAVOUR>
 void dispatch_kind(hipStream_t s, const StageArgs& a, int kind, bool final) {
-  const uint32_t per = (a.n_work + 7) / 8;
-  const dim3 grid(8 * per), block(64 * WAVES);
   if (kind == kRhsOnly) {
-    k_strip<DIM, KERNEL, FLAVOUR, kRhsOnly, false, WAVES, CAP><<<grid, block, 0, s>>>(a);
+    launch_kind<DIM, KERNEL, FLAVOUR, kRhsOnly, false>(s, a);
   } else if (kind == kEuler) {
     if (final) {
-      k_strip<DIM, KERNEL, FLAVOUR, kEuler, true, WAVES, CAP><<<grid, block, 0, s>>>(a);
+      launch_kind<DIM, KERNEL, FLAVOUR, kEuler, true>(s, a);
     } else {
-      k_strip<DIM, KERNEL, FLAVOUR, kEuler, false, WAVES, CAP><<<grid, block, 0, s>>>(a);
+      launch_kind<DIM, KERNEL, FLAVOUR, kEuler, false>(s, a);
     }
   } else {
     if (final) {
-      k_strip<DIM, KERNEL, FLAVOUR, kCombine, true, WAVES, CAP><<<grid, block, 0, s>>>(a);
+      launch_kind<DIM, KERNEL, FLAVOUR, kCombine, true>(s, a);
     } else {
-      k_strip<DIM, KERNEL, FLAVOUR, kCombine, false, WAVES, CAP><<<grid, block, 0, s>>>(a);
+      launch_kind<DIM, KERNEL, FLAVOUR, kCombine, false>(s, a);
     }
   }
 }
 
-template <int DIM, int KERNEL, int WAVES, int CAP>
+template <int DIM, int KERNEL>
 void dispatch_flavour(hipStream_t s, const StageArgs& a, int kind, bool final) {
   if (a.P.flavour == kFlavourCfg) {
-    dispatch_kind<DIM, KERNEL, kFlavourCfg, WAVES, CAP>(s, a, kind, final);
+    dispatch_kind<DIM, KERNEL, kFlavourCfg>(s, a, kind, final);
   } else {
-    dispatch_kind<DIM, KERNEL, kFlavourRef, WAVES, CAP>(s, a, kind, final);
+    dispatch_kind<DIM, KERNEL, kFlavourRef>(s, a, kind, final);
   }
 }
 
-template <int DIM, int WAVES, int CAP>
+template <int DIM>
 void dispatch_kernel(hipStream_t s, const StageArgs& a, int kind, bool final) {
   if (a.P.kernel == kKernelC4) {
-    dispatch_flavour<DIM, kKernelC4, WAVES, CAP>(s, a, kind, final);
+    dispatch_flavour<DIM, kKernelC4>(s, a, kind, final);
   } else {
-    dispatch_flavour<DIM, kKernelC2, WAVES, CAP>(s, a, kind, final);
+    dispatch_flavour<DIM, kKernelC2>(s, a, kind, final);
   }
+}
+
+template <class F>
+int occupancy(F* f) {
+  int n = 0;
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(f),
+                                                     kThreads, 0);
+  return std::max(n, 1);
 }
 
 }  // namespace
 
-int strip_threads(int dim) { return dim == 2 ? 128 : 128; }
-uint32_t strip_lds_capacity(int dim) { return dim == 2 ? kStripLdsCap : 0u; }
+uint32_t stage_lds_capacity(int dim) { return dim == 2 ? kStageLdsCap : 0u; }
+
+void stage_grids(const DevParams& P, int n_cu, uint32_t* grid_lds,
+                 uint32_t* grid_global) {
+  int bl, bg;
+  if (P.dim == 2) {
+    bl = occupancy(k_stage_lds<kKernelC2, kFlavourCfg, kCombine, false, kThreads, kStageLdsCap>);
+    bg = occupancy(k_stage_global<2, kKernelC2, kFlavourCfg, kCombine, false, kThreads>);
+  } else {
+    bl = 1;
+    bg = occupancy(k_stage_global<3, kKernelC2, kFlavourCfg, kCombine, false, kThreads>);
+  }
+  // multiples of 8: every XCD owns the same number of slots
+  *grid_lds = 8u * static_cast<uint32_t>((bl * n_cu + 7) / 8);
+  *grid_global = 8u * static_cast<uint32_t>((bg * n_cu + 7) / 8);
+}
+
+void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot) {
+  const uint32_t nb = (a.n_cols + kScanBlock - 1) / kScanBlock;
+  k_chunk_count<<<nb, kScanBlock, 0, s>>>(a, block_tot);
+  k_chunk_scan<<<1, kScanBlock, 0, s>>>(nb, block_tot, a.counts);
+  k_chunk_emit<<<nb, kScanBlock, 0, s>>>(a, block_tot);
+}
 
 void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final) {
-  if (a.n == 0 || a.n_work == 0) return;
+  if (a.n == 0 || a.n_cols == 0) return;
   if (a.P.dim == 2) {
-    dispatch_kernel<2, 2, kStripLdsCap>(s, a, kind, final);
+    dispatch_kernel<2>(s, a, kind, final);
   } else {
-    dispatch_kernel<3, 2, 0>(s, a, kind, final);
+    dispatch_kernel<3>(s, a, kind, final);
   }
 }
 

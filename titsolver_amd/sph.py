@@ -242,12 +242,12 @@ class HipContext:
     def get_schedule(self) -> dict:
         sch = _abi.Schedule()
         self._check(self.lib.tit_hip_get_schedule(self.h, C.byref(sch)))
-        return {"strip_cells": sch.strip_cells, "lds_capacity": sch.lds_capacity,
+        return {"chunk_targets": sch.chunk_targets, "lds_capacity": sch.lds_capacity,
                 "occupancy": sch.occupancy}
 
-    def set_schedule(self, strip_cells: int = 0, lds_capacity: int | None = None):
+    def set_schedule(self, chunk_targets: int = 0, lds_capacity: int | None = None):
         cur = self.get_schedule()
-        sch = _abi.Schedule(strip_cells, cur["lds_capacity"] if lds_capacity is None
+        sch = _abi.Schedule(chunk_targets, cur["lds_capacity"] if lds_capacity is None
                             else lds_capacity, 0.0)
         self._check(self.lib.tit_hip_set_schedule(self.h, C.byref(sch)))
 
