@@ -285,10 +285,6 @@ void swap_ptr(T*& a, T*& b) {
   b = t;
 }
 
-__global__ void k_zero_u32(uint32_t* p, int n) {
-  const int i = threadIdx.x;
-  if (i < n) p[i] = 0u;
-}
 
 StageArgs stage_args(tit_hip_ctx* c);
 
@@ -304,7 +300,7 @@ void enqueue_rebuild(tit_hip_ctx* c) {
   {
     ProfScope ps(c->prof, s, "permute");
     launch_permute(s, c->dp, c->n, c->val, c->A, c->T, c->id, c->id_alt,
-                   c->tag, c->tag_alt);
+                   c->tag, c->tag_alt, c->fixed_list, c->counter);
   }
   swap_state(c->A, c->T);
   swap_ptr(c->id, c->id_alt);
@@ -317,11 +313,6 @@ void enqueue_rebuild(tit_hip_ctx* c) {
     ProfScope ps(c->prof, s, "build_chunks");
     StageArgs a = stage_args(c);
     launch_build_chunks(s, a, c->block_tot);
-  }
-  {
-    ProfScope ps(c->prof, s, "compact_fixed");
-    k_zero_u32<<<1, 64, 0, s>>>(c->counter, 1);
-    launch_compact_fixed(s, c->n, c->tag, c->fixed_list, c->counter);
   }
   launch_reset_disp(s, c->n_tanks, c->disp);
   c->rebuilds += 1;

@@ -56,7 +56,7 @@ struct DevState {
   double* x[3];
   double* v[3];
   double* rho;
-  float2* aux;  // per particle (p / rho^2, m / rho) of rho, FP32
+  float2* aux;  // per particle (p / rho^2, cs) of rho, FP32
 };
 
 // ---- radix sort (radix_sort.hip) -------------------------------------------

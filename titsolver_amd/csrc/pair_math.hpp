@@ -58,16 +58,17 @@ __device__ __forceinline__ float pressure_over_rho2(const DevParams& P,
   return static_cast<float>(p / (rho * rho));
 }
 
-// Per-particle pair inputs kept beside rho: (p / rho^2, m / rho).  Every
-// kernel that writes a density writes this too.
+// Per-particle pair inputs kept beside rho: (p / rho^2, cs).  Every kernel
+// that writes a density writes this too.
 __device__ __forceinline__ float2 make_aux(const DevParams& P, double rho) {
-  return make_float2(pressure_over_rho2(P, rho), P.m_f / static_cast<float>(rho));
+  return make_float2(pressure_over_rho2(P, rho),
+                     sound_speed_f32(P, static_cast<float>(rho)));
 }
 
 template <int DIM>
 __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
                                             const double* v, double rho,
-                                            float Pa, Target<DIM>& a) {
+                                            float2 aux, Target<DIM>& a) {
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
     a.x[k] = x[k];
@@ -75,15 +76,16 @@ __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
   }
   a.rho = rho;
   a.rho_f = static_cast<float>(rho);
-  a.Pa = Pa;
-  a.csa = sound_speed_f32(P, a.rho_f);
+  a.Pa = aux.x;
+  a.csa = aux.y;
 }
 
-// d = r_b - r_a in FP64 (as in the neighbour predicate).
+// d = r_b - r_a in FP64 (as in the neighbour predicate); Pb = p_b/rho_b^2,
+// csb = cs(rho_b) from the neighbour's aux.
 template <int DIM, int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>& a,
                                           const double* d, const float* vb,
-                                          double rho_b, float Pb, float Vb,
+                                          double rho_b, float Pb, float csb,
                                           Sums<DIM>& s) {
   float x[DIM];
 #pragma unroll
@@ -101,7 +103,7 @@ __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>&
   for (int k = 0; k < DIM; ++k) vr = __builtin_fmaf(a.v[k] - vb[k], x[k], vr);
   const float drab = static_cast<float>(a.rho - rho_b);
   const float rbf = static_cast<float>(rho_b);
-  const float csb = sound_speed_f32(P, rbf);
+  const float Vb = P.m_f * __builtin_amdgcn_rcpf(rbf);
   float cont, Pi;
   if constexpr (FLAVOUR == kFlavourCfg) {
     cont = G * __builtin_fmaf(P.m_f, vr, P.dhc2 * Vb * drab);

@@ -164,15 +164,30 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(
   }
   key[i] = T.cell_off + flat;
   if (bad) atomicOr(flags, 2u);
+  if (i == 0) flags[-1] = 0u;  // fixed-particle counter, filled by k_permute
 }
 
+// Gathers the state into cell order and lists the fixed particles (the list
+// order is irrelevant: each fixed particle is extrapolated on its own).
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_permute(
     DevParams P, uint32_t n, const uint32_t* __restrict__ perm, DevState src, DevState dst,
     const uint32_t* __restrict__ id_src, uint32_t* __restrict__ id_dst,
-    const uint32_t* __restrict__ tag_src, uint32_t* __restrict__ tag_dst) {
+    const uint32_t* __restrict__ tag_src, uint32_t* __restrict__ tag_dst,
+    uint32_t* __restrict__ fixed_list, uint32_t* __restrict__ fixed_count) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  const bool in = i < n;
+  const uint32_t tg = in ? tag_src[perm[i]] : 0u;
+  const uint64_t m = __ballot(in && (tg & 1u));
+  if (m) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(fixed_count, static_cast<uint32_t>(__popcll(m)));
+    base = __shfl(base, leader);
+    if (in && (tg & 1u)) fixed_list[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+  }
+  if (!in) return;
   const uint32_t s = perm[i];
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
@@ -183,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(
   dst.rho[i] = rho;
   dst.aux[i] = make_aux(P, rho);
   id_dst[i] = id_src[s];
-  tag_dst[i] = tag_src[s];
+  tag_dst[i] = tg;
 }
 
 // CSR offsets: off[c] = first sorted position with key >= c.
@@ -202,23 +217,6 @@ __global__ __launch_bounds__(kBlock) void k_cell_offsets(
     }
   }
   off[c] = lo;
-}
-
-__global__ __launch_bounds__(kBlock) void k_compact_fixed(
-    uint32_t n, const uint32_t* __restrict__ tag,
-    uint32_t* __restrict__ list, uint32_t* __restrict__ counter) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const bool fixed = i < n && (tag[i] & 1u);
-  const uint64_t m = __ballot(fixed);
-  if (m == 0) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll(static_cast<long long>(m)) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(m)));
-  base = __shfl(base, leader);
-  if (fixed) {
-    list[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
-  }
 }
 
 // ---- boundary extrapolation ---------------------------------------------------
@@ -305,6 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     rl[q + 1] = rl[q] + (e - b);
   }
   double S = 0.0, H = 0.0;
+  const double xi_1 = P.xi - 1.0, inv_xi_1 = 1.0 / xi_1, cs2 = P.cs0 * P.cs0;
   for (uint32_t t = lane; t < rl[NCOL]; t += 64) {
     int q = 0;
     while (t >= rl[q + 1]) ++q;
@@ -323,7 +322,10 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     double rn = dd[0] * ne[0];
 #pragma unroll
     for (int d = 1; d < DIM; ++d) rn = rn + dd[d] * ne[d];
-    const double H_b = eos_potential(P, rho_b);
+    // H(rho_b) = cs0^2 ((rho_b/rho0)^(xi-1) - 1) / (xi - 1)
+    const double ratio = rho_b * P.inv_rho0;
+    const double rp = P.xi_int ? ipow_f64(ratio, P.xi_int - 1) : pow(ratio, xi_1);
+    const double H_b = cs2 * (rp - 1.0) * inv_xi_1;
     S += V_b * W;
     H += V_b * (H_b + P.g * rn * ne[1]) * W;
   }
@@ -506,14 +508,16 @@ void launch_cell_keys(hipStream_t s, const DevParams& P, const DevTank* tanks,
 void launch_permute(hipStream_t s, const DevParams& P, uint32_t n, const uint32_t* perm,
                     DevState src, DevState dst, const uint32_t* id_src,
                     uint32_t* id_dst, const uint32_t* tag_src,
-                    uint32_t* tag_dst) {
+                    uint32_t* tag_dst, uint32_t* fixed_list, uint32_t* fixed_count) {
   if (n == 0) return;
   if (P.dim == 2) {
     k_permute<2><<<grid_for(n), kBlock, 0, s>>>(P, n, perm, src, dst, id_src,
-                                                id_dst, tag_src, tag_dst);
+                                                id_dst, tag_src, tag_dst, fixed_list,
+                                                fixed_count);
   } else {
     k_permute<3><<<grid_for(n), kBlock, 0, s>>>(P, n, perm, src, dst, id_src,
-                                                id_dst, tag_src, tag_dst);
+                                                id_dst, tag_src, tag_dst, fixed_list,
+                                                fixed_count);
   }
 }
 
@@ -521,12 +525,6 @@ void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
                          const uint32_t* key, uint32_t* cell_off) {
   k_cell_offsets<<<grid_for(n_cells + 1), kBlock, 0, s>>>(n_cells, n, key,
                                                           cell_off);
-}
-
-void launch_compact_fixed(hipStream_t s, uint32_t n, const uint32_t* tag,
-                          uint32_t* fixed_list, uint32_t* counter) {
-  if (n == 0) return;
-  k_compact_fixed<<<grid_for(n), kBlock, 0, s>>>(n, tag, fixed_list, counter);
 }
 
 void launch_boundary(hipStream_t s, const BoundaryArgs& a) {

@@ -66,11 +66,9 @@ void launch_cell_keys(hipStream_t s, const DevParams& P, const DevTank* tanks,
 void launch_permute(hipStream_t s, const DevParams& P, uint32_t n, const uint32_t* perm,
                     DevState src, DevState dst, const uint32_t* id_src,
                     uint32_t* id_dst, const uint32_t* tag_src,
-                    uint32_t* tag_dst);
+                    uint32_t* tag_dst, uint32_t* fixed_list, uint32_t* fixed_count);
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
                          const uint32_t* key, uint32_t* cell_off);
-void launch_compact_fixed(hipStream_t s, uint32_t n, const uint32_t* tag,
-                          uint32_t* fixed_list, uint32_t* counter);
 void launch_boundary(hipStream_t s, const BoundaryArgs& a);
 void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final);
 uint32_t stage_lds_capacity(int dim);

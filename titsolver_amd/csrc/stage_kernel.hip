@@ -13,7 +13,7 @@
 // 5 x 5 cell stencil one column range at a time: the exact FP64 predicate
 // (four LDS loads per batch) builds a 64-bit acceptance mask, and the FP32
 // pair terms run over the accepted candidates only, reading their velocity,
-// density and (p/rho^2, m/rho) from L2 one candidate ahead of the arithmetic.
+// density and (p/rho^2, cs) from L2 one candidate ahead of the arithmetic.
 // Chunks whose candidates exceed the LDS capacity, and 3D, take positions
 // from global memory too.  Every path visits candidates in the same order and
 // rounds identically, so all schedules produce the same bits.
@@ -278,7 +278,7 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
     }
     const double rho_a = A.in.rho[i];
     Target<DIM> a;
-    make_target<DIM>(P, xa, va, rho_a, A.in.aux[i].x, a);
+    make_target<DIM>(P, xa, va, rho_a, A.in.aux[i], a);
     const int zc = static_cast<int>(A.key[i] - c.col_row);
     Sums<DIM> acc;
     acc.init();
@@ -430,65 +430,56 @@ __device__ __forceinline__ uint32_t column_chunks(const StageArgs& A, uint32_t c
   return (*n + A.chunk_targets - 1) / A.chunk_targets;
 }
 
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh,
+// Exclusive scan over a 1024-thread block: wave shuffles, then the 16 wave
+// totals.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum,
                                                     uint32_t* total) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    const uint32_t o = tid >= off ? sh[tid - off] : 0u;
-    __syncthreads();
-    sh[tid] += o;
-    __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
   }
-  *total = sh[kScanBlock - 1];
-  return sh[tid] - v;
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+      const uint32_t t = wsum[w];
+      wsum[w] = s;
+      s += t;
+    }
+    wsum[kScanBlock / 64] = s;
+  }
+  __syncthreads();
+  const uint32_t r = wsum[wave] + x - v;
+  *total = wsum[kScanBlock / 64];
+  __syncthreads();
+  return r;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_chunk_count(StageArgs A,
-                                                            uint32_t* block_tot) {
-  __shared__ uint32_t sh[kScanBlock];
-  const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
-  uint32_t beg, n, nch = 0;
-  if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
-  uint32_t tot;
-  (void)block_excl_scan(nch, sh, &tot);
-  if (threadIdx.x == 0) block_tot[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(kScanBlock) void k_chunk_scan(uint32_t n_blocks,
-                                                           uint32_t* block_tot,
-                                                           uint32_t* counts) {
-  __shared__ uint32_t sh[kScanBlock];
-  const uint32_t per = (n_blocks + kScanBlock - 1) / kScanBlock;
-  const uint32_t b0 = min(threadIdx.x * per, n_blocks), b1 = min(b0 + per, n_blocks);
-  uint32_t s = 0;
-  for (uint32_t k = b0; k < b1; ++k) s += block_tot[k];
-  uint32_t tot;
-  uint32_t excl = block_excl_scan(s, sh, &tot);
-  for (uint32_t k = b0; k < b1; ++k) {
-    const uint32_t v = block_tot[k];
-    block_tot[k] = excl;
-    excl += v;
+// One workgroup recuts every column into balanced chunks, in column order.
+__global__ __launch_bounds__(kScanBlock) void k_build_chunks(StageArgs A) {
+  __shared__ uint32_t wsum[kScanBlock / 64 + 1];
+  uint32_t base_off = 0;
+  for (uint32_t base = 0; base < A.n_cols; base += kScanBlock) {
+    const uint32_t c = base + threadIdx.x;
+    uint32_t beg = 0, n = 0, nch = 0;
+    if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
+    uint32_t tot;
+    const uint32_t off = base_off + block_excl_scan(nch, wsum, &tot);
+    for (uint32_t k = 0; k < nch; ++k) {
+      const uint32_t p0 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * k) / nch);
+      const uint32_t p1 =
+          beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * (k + 1)) / nch);
+      A.chunks[off + k] = make_uint4(c, p0, p1, 0u);
+    }
+    base_off += tot;
   }
   if (threadIdx.x == 0) {
-    counts[0] = tot;  // chunks
-    counts[1] = 0;    // overflow chunks (appended by the stage kernel)
-  }
-}
-
-__global__ __launch_bounds__(kScanBlock) void k_chunk_emit(StageArgs A,
-                                                           const uint32_t* block_off) {
-  __shared__ uint32_t sh[kScanBlock];
-  const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
-  uint32_t beg = 0, n = 0, nch = 0;
-  if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
-  uint32_t tot;
-  const uint32_t off = block_off[blockIdx.x] + block_excl_scan(nch, sh, &tot);
-  for (uint32_t k = 0; k < nch; ++k) {
-    const uint32_t p0 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * k) / nch);
-    const uint32_t p1 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * (k + 1)) / nch);
-    A.chunks[off + k] = make_uint4(c, p0, p1, 0u);
+    A.counts[0] = base_off;  // chunks
+    A.counts[1] = 0;         // overflow chunks (appended by the stage kernel)
   }
 }
 
@@ -576,11 +567,8 @@ void stage_grids(const DevParams& P, int n_cu, uint32_t* grid_lds,
   *grid_global = 8u * static_cast<uint32_t>((bg * n_cu + 7) / 8);
 }
 
-void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot) {
-  const uint32_t nb = (a.n_cols + kScanBlock - 1) / kScanBlock;
-  k_chunk_count<<<nb, kScanBlock, 0, s>>>(a, block_tot);
-  k_chunk_scan<<<1, kScanBlock, 0, s>>>(nb, block_tot, a.counts);
-  k_chunk_emit<<<nb, kScanBlock, 0, s>>>(a, block_tot);
+void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t*) {
+  k_build_chunks<<<1, kScanBlock, 0, s>>>(a);
 }
 
 void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final) {
