@@ -43,6 +43,18 @@ def stage_bytes_per_particle(dim: int) -> float:
     return (2 * state + 3 * state + 3 * state) / 3.0
 
 
+def measured_traffic(nf: int):
+    """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (scaled by a 256 MiB copy), when they were taken on this
+    same workload; None otherwise."""
+    f = ROOT / "profiles" / "stage_kernel_2d_dam_break_2M_traffic.json"
+    try:
+        d = json.loads(f.read_text())
+    except (OSError, ValueError):
+        return None
+    return d["hbm_bytes_per_launch"] if d.get("fluid_particles") == nf else None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,7 +103,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                          "(sharding across GPUs is designed, not built)"}))
         sys.exit(2)
     import titsolver_amd as ta
 
@@ -154,7 +166,8 @@ def main():
                    "schedule": sched},
         "roofline": {"bound": "hbm", "kernel": "k_stage (fused pair sums + SSPRK update)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(nf),
                      "bytes_per_launch": bytes_launch,
                      "avg_launch_us": avg_stage_s * 1e6},
         "kernels": kernels,
