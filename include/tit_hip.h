@@ -173,6 +173,86 @@ tit_status tit_hip_counts(tit_hip_ctx* ctx, size_t* n_particles,
 const char* tit_hip_last_error(const tit_hip_ctx* ctx);
 void tit_hip_destroy(tit_hip_ctx* ctx);
 
+/* ---- learned step ----------------------------------------------------------
+ * No reference counterpart: a message-passing network (edge MLP 80 -> 128
+ * -> 128, node MLP 160 -> 128 -> 128 -> 16, SiLU, two rounds over the 8
+ * nearest neighbours within h) predicts each fluid particle's acceleration
+ * and density rate; one learned step advances 4 dt:
+ *   v' = v + 4dt a,  r' = r + 4dt (v + v') / 2,  rho' = rho + 4dt drho/dt.
+ * It stands in for the pair sums of FluidEquations (fluid_equations.hpp:
+ * 232-305) inside SSPRKIntegrator::step (time_integrator.hpp:159-184).
+ * Parameters are FP32 (kept in BF16 for the MFMA GEMMs). */
+
+/* Creates the network with seeded random weights (replaces any earlier
+ * one; recorded samples are dropped). */
+tit_status tit_hip_learned_init(tit_hip_ctx* ctx, uint64_t seed);
+/* Parameter vector (row-major weight blocks, per round W1 b1 W2 b2 W3 b3 W4
+ * b4 W5 b5).  With w = NULL only *n is written. */
+tit_status tit_hip_learned_params(tit_hip_ctx* ctx, float* w, size_t cap,
+                                  size_t* n);
+tit_status tit_hip_learned_set_params(tit_hip_ctx* ctx, const float* w,
+                                      size_t n);
+/* Scales: [v_s, rho_s, a_x, a_y, a_z, drho/dt].  Node features use v / v_s
+ * and (rho / rho0 - 1) / rho_s; outputs are multiplied by the last four. The
+ * output scales are set from the first recorded sample unless set here. */
+tit_status tit_hip_learned_scales(tit_hip_ctx* ctx, double* scales6);
+tit_status tit_hip_learned_set_scales(tit_hip_ctx* ctx,
+                                      const double* scales6);
+
+/* Training sample from the current state: its graph and features, then
+ * `plain_steps` plain steps (the context's state advances); the targets are
+ * (v_after - v) / T and (rho_after - rho) / T with T the time advanced. */
+tit_status tit_hip_learned_record(tit_hip_ctx* ctx, int plain_steps,
+                                  int* sample_index);
+tit_status tit_hip_learned_clear_samples(tit_hip_ctx* ctx);
+/* Sample export (test hook): n nodes; nbr n x 8 (sample-local indices, self
+ * for a missing edge), node_feat n x 16, edge_feat n x 8 x 16 (the BF16
+ * values as float), target n x 4, fluid n.  NULL pointers are skipped. */
+tit_status tit_hip_learned_sample(tit_hip_ctx* ctx, int index, size_t* n,
+                                  uint32_t* nbr, float* node_feat,
+                                  float* edge_feat, float* target,
+                                  uint8_t* fluid);
+/* Mean-squared loss over fluid particles and used output channels of one
+ * sample, and its gradient with respect to the parameters (test hook). */
+tit_status tit_hip_learned_loss_grad(tit_hip_ctx* ctx, int index,
+                                     double* loss, float* grad);
+/* Adam (beta 0.9 / 0.999) over the recorded samples in turn; losses[it]
+ * (may be NULL) receives each iteration's loss before its update. */
+tit_status tit_hip_learned_train(tit_hip_ctx* ctx, int iters, double lr,
+                                 double* losses);
+/* Graph and features of the current state, in reference order (test hook):
+ * nbr_ref n x 8 particle indices, node_feat n x 16, edge_feat n x 8 x 16. */
+tit_status tit_hip_learned_graph(tit_hip_ctx* ctx, uint32_t* nbr_ref,
+                                 float* node_feat, float* edge_feat);
+/* Network output (n x 16, reference order) on the current state. */
+tit_status tit_hip_learned_forward(tit_hip_ctx* ctx, float* out);
+
+typedef struct tit_hip_learned_report {
+  int learned_steps;    /* learned steps taken, judged ones included */
+  int judged;           /* comparisons with the plain step */
+  int rejected;         /* comparisons above the tolerance */
+  int plain_steps;      /* plain SSPRK steps (judge references, fallbacks) */
+  double max_rho_err;   /* largest max_i |rho_L - rho_P| / rho_P seen */
+  double last_rho_err;
+  double max_front_err; /* largest |front_L - front_P| / |front_P| seen */
+} tit_hip_learned_report;
+/* n_steps learned steps.  The last step of every group of judge_every is
+ * judged: it is taken from state S and compared with 4 plain steps from S
+ * (max relative density error over fluid particles); the plain result is
+ * kept.  Above tol, the group is discarded: the state goes back to the last
+ * judged state and the group's interval is covered by plain steps. */
+tit_status tit_hip_learned_run(tit_hip_ctx* ctx, int n_steps, int judge_every,
+                               double tol, tit_hip_learned_report* report);
+
+/* The network's BF16 MFMA GEMM on the current device (test hook, no
+ * context): A is M x K (K x M with transA), B is K x N (N x K with transB),
+ * row-major host FP32, rounded to BF16; C (M x N) = act(op(A) op(B) + bias)
+ * with act 0 (none) or 1 (SiLU), FP32 accumulation.  k_splits > 1 adds
+ * op(A) op(B) into C over that many K slices instead (no bias or act). */
+tit_status tit_hip_nn_gemm(int M, int N, int K, const float* A, int transA,
+                           const float* B, int transB, const float* bias,
+                           int act, int k_splits, float* C);
+
 /* ---- host-side scene set-up (no device needed) ---------------------------
  * 2D/3D dam break on a lattice as titwcsph builds it (wcsph.cpp:104-142):
  * fluid at dr*(i+1, j+1[, k+1]) for a column of column[0] x column[1]

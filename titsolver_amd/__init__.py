@@ -1,6 +1,6 @@
 """MI355X (gfx950) WCSPH step of TitSolver behind a C ABI (libtit_hip.so)."""
 from ._abi import (FLAVOUR_CFG, FLAVOUR_REF, KERNEL_WENDLAND_C2,  # noqa: F401
                    KERNEL_WENDLAND_C4, LIB_PATH, load)
-from .sph import (FluidEquations, HipContext, ParticleArray,  # noqa: F401
-                  SSPRKIntegrator, TitError, cfg_equations, dam_break,
-                  ref_equations)
+from .sph import (FluidEquations, HipContext, LearnedStep,  # noqa: F401
+                  ParticleArray, SSPRKIntegrator, TitError, cfg_equations,
+                  dam_break, nn_gemm, ref_equations)

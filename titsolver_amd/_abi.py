@@ -52,6 +52,13 @@ class Schedule(C.Structure):
                 ("occupancy", C.c_double)]
 
 
+class LearnedReport(C.Structure):
+    _fields_ = [("learned_steps", C.c_int), ("judged", C.c_int),
+                ("rejected", C.c_int), ("plain_steps", C.c_int),
+                ("max_rho_err", C.c_double), ("last_rho_err", C.c_double),
+                ("max_front_err", C.c_double)]
+
+
 class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64),
                 ("total_ms", C.c_double)]
@@ -59,6 +66,7 @@ class KernelStat(C.Structure):
 
 # name -> (restype, argtypes); mirrors include/tit_hip.h one to one.
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
 SIGNATURES = {
     "tit_hip_create": (C.c_int, [C.c_int, C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "tit_hip_upload": (C.c_int, [C.c_void_p, C.POINTER(HostView), C.POINTER(Tank)]),
@@ -82,6 +90,25 @@ SIGNATURES = {
                                  C.POINTER(C.c_uint64)]),
     "tit_hip_last_error": (C.c_char_p, [C.c_void_p]),
     "tit_hip_destroy": (None, [C.c_void_p]),
+    "tit_hip_learned_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "tit_hip_learned_params": (C.c_int, [C.c_void_p, _FP, C.c_size_t,
+                                         C.POINTER(C.c_size_t)]),
+    "tit_hip_learned_set_params": (C.c_int, [C.c_void_p, _FP, C.c_size_t]),
+    "tit_hip_learned_scales": (C.c_int, [C.c_void_p, _DP]),
+    "tit_hip_learned_set_scales": (C.c_int, [C.c_void_p, _DP]),
+    "tit_hip_learned_record": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "tit_hip_learned_clear_samples": (C.c_int, [C.c_void_p]),
+    "tit_hip_learned_sample": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_size_t),
+                                         C.POINTER(C.c_uint32), _FP, _FP, _FP,
+                                         C.POINTER(C.c_uint8)]),
+    "tit_hip_learned_loss_grad": (C.c_int, [C.c_void_p, C.c_int, _DP, _FP]),
+    "tit_hip_learned_train": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _DP]),
+    "tit_hip_learned_graph": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), _FP, _FP]),
+    "tit_hip_learned_forward": (C.c_int, [C.c_void_p, _FP]),
+    "tit_hip_learned_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double,
+                                      C.POINTER(LearnedReport)]),
+    "tit_hip_nn_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, _FP, C.c_int, _FP,
+                                  C.c_int, _FP, C.c_int, C.c_int, _FP]),
     "tit_dam_break_count": (C.c_int, [C.c_int, C.c_double, _DP, _DP, C.c_int,
                                       C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "tit_dam_break_fill": (C.c_int, [C.c_int, C.c_double, _DP, _DP, C.c_int,

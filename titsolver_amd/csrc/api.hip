@@ -16,7 +16,7 @@
 #include <vector>
 
 #include "../../include/tit_hip.h"
-#include "sph_kernels.hpp"
+#include "context.hpp"
 
 using namespace tit_amd;
 
@@ -79,96 +79,26 @@ Profiler::~Profiler() {
 
 }  // namespace tit_amd
 
-struct tit_hip_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  tit_hip_params prm{};
-  DevParams dp{};
-  std::string err;
-  Profiler prof;
 
-  // sizes
-  uint32_t n = 0;
-  uint32_t n_fixed = 0;
-  int n_tanks = 0;
-  uint32_t n_cells = 0;
-  int key_bits = 1;
-  bool uploaded = false;
-  uint64_t rebuilds = 0;
-
-  // device memory
-  std::vector<void*> allocs;
-  DevState A{}, T{}, B{}, C{};
-  uint32_t *key = nullptr, *key_alt = nullptr, *val = nullptr, *val_alt = nullptr;
-  uint32_t *id = nullptr, *id_alt = nullptr, *tag = nullptr, *tag_alt = nullptr;
-  uint32_t* cell_off = nullptr;
-  uint32_t* fixed_list = nullptr;
-  uint32_t* counter = nullptr;  // [0] fixed count, [1] flags
-  SortScratch sort;
-  DevTank* tanks = nullptr;
-  double* normal = nullptr;
-  double *dt = nullptr, *time = nullptr, *disp = nullptr;
-  unsigned long long* dt_min = nullptr;
-  uint32_t* vmax = nullptr;
-  double *stage_r = nullptr, *stage_v = nullptr, *stage_rho = nullptr,
-         *stage_p = nullptr;
-  double* rhs_drho = nullptr;
-  double* rhs_dv[3] = {nullptr, nullptr, nullptr};
-
-  // chunk schedule of the stage kernel
-  std::vector<DevTank> host_tanks;
-  uint4* cols = nullptr;
-  uint32_t n_cols = 0;
-  uint4* chunks = nullptr;
-  uint32_t* overflow = nullptr;
-  uint32_t* block_tot = nullptr;
-  uint32_t* chunk_counts = nullptr;
-  size_t chunk_cap = 0;
-  uint32_t chunk_targets = kChunkMax;
-  uint32_t lds_cap = 0;
-  uint32_t grid_lds = 0, grid_global = 0;
-  int n_cu = 0;
-  double occupancy = 1.0;  // mean particles per occupied cell at upload
-
-  // graph of two steps, valid while the rebuild parity matches
-  hipGraphExec_t graph = nullptr;
-  hipGraph_t graph_def = nullptr;
-  uint64_t graph_parity = 0;
-};
-
-namespace {
+namespace tit_amd {
 
 tit_status fail(tit_hip_ctx* c, tit_status s, const std::string& msg) {
   if (c) c->err = msg;
   return s;
 }
 
-#define TIT_HIP_TRY(ctx, call)                                              \
-  do {                                                                      \
-    hipError_t e_ = (call);                                                 \
-    if (e_ != hipSuccess) {                                                 \
-      return fail((ctx), e_ == hipErrorOutOfMemory ? TIT_E_OOM : TIT_E_HIP, \
-                  std::string(#call) + ": " + hipGetErrorString(e_));       \
-    }                                                                       \
-  } while (0)
 
-template <class T>
-hipError_t dalloc(tit_hip_ctx* c, T** p, size_t count) {
-  void* q = nullptr;
-  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
-  if (e == hipSuccess) c->allocs.push_back(q);
-  *p = static_cast<T*>(q);
-  return e;
-}
 
 void drop_graph(tit_hip_ctx* c) {
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
-  if (c->graph_def) {
-    (void)hipGraphDestroy(c->graph_def);
-    c->graph_def = nullptr;
+  for (int k = 0; k < 2; ++k) {
+    if (c->graph[k]) {
+      (void)hipGraphExecDestroy(c->graph[k]);
+      c->graph[k] = nullptr;
+    }
+    if (c->graph_def[k]) {
+      (void)hipGraphDestroy(c->graph_def[k]);
+      c->graph_def[k] = nullptr;
+    }
   }
 }
 
@@ -467,6 +397,41 @@ tit_status read_flags(tit_hip_ctx* c) {
 
 }  // namespace
 
+namespace tit_amd {
+
+// n_steps plain SSPRK steps: pairs of steps replay the two-step graph of the
+// current rebuild parity (captured on first use), an odd remainder runs
+// eagerly.  Synchronises and checks the device flags.
+tit_status run_plain(tit_hip_ctx* c, int n_steps) {
+  int done = 0;
+  if (!c->prof.enabled && n_steps >= 2) {
+    const int par = static_cast<int>(c->rebuilds & 1u);
+    if (!c->graph[par]) {
+      TIT_HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_step(c);
+      enqueue_step(c);
+      hipError_t e = hipStreamEndCapture(c->stream, &c->graph_def[par]);
+      c->rebuilds -= 2;  // captured, not executed
+      if (e != hipSuccess) {
+        c->graph_def[par] = nullptr;
+        return fail(c, TIT_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      }
+      TIT_HIP_TRY(c, hipGraphInstantiate(&c->graph[par], c->graph_def[par], nullptr, nullptr, 0));
+    }
+    for (; done + 2 <= n_steps; done += 2) {
+      TIT_HIP_TRY(c, hipGraphLaunch(c->graph[par], c->stream));
+      c->rebuilds += 2;
+    }
+  }
+  for (; done < n_steps; ++done) enqueue_step(c);
+  tit_status st = check_launch(c, "step");
+  if (st != TIT_OK) return st;
+  if (c->prof.enabled) c->prof.collect();
+  return read_flags(c);
+}
+
+}  // namespace tit_amd
+
 extern "C" {
 
 tit_status tit_hip_create(int device, const tit_hip_params* params,
@@ -505,6 +470,7 @@ void tit_hip_destroy(tit_hip_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  destroy_learned(c);
   free_all(c);
   c->prof.clear();
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -729,38 +695,12 @@ tit_status tit_hip_step(tit_hip_ctx* c, int n_steps, double* dt_last) {
   if (!c->uploaded) return fail(c, TIT_E_STATE, "step before upload");
   if (n_steps < 0) return fail(c, TIT_E_ARG, "negative step count");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
-  int done = 0;
-  if (!c->prof.enabled && n_steps >= 2) {
-    // Every rebuild swaps buffer pointers; a graph captured at the other
-    // parity would read the wrong buffers.
-    if (c->graph && c->graph_parity != (c->rebuilds & 1u)) drop_graph(c);
-    if (!c->graph) {
-      c->graph_parity = c->rebuilds & 1u;
-      TIT_HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_step(c);
-      enqueue_step(c);
-      hipError_t e = hipStreamEndCapture(c->stream, &c->graph_def);
-      c->rebuilds -= 2;  // captured, not executed
-      if (e != hipSuccess) {
-        c->graph_def = nullptr;
-        return fail(c, TIT_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-      }
-      TIT_HIP_TRY(c, hipGraphInstantiate(&c->graph, c->graph_def, nullptr, nullptr, 0));
-    }
-    for (; done + 2 <= n_steps; done += 2) {
-      TIT_HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
-      c->rebuilds += 2;
-    }
-  }
-  for (; done < n_steps; ++done) enqueue_step(c);
-  tit_status st = check_launch(c, "step");
+  tit_status st = run_plain(c, n_steps);
   if (st != TIT_OK) return st;
-  if (c->prof.enabled) c->prof.collect();
-  st = read_flags(c);
   if (dt_last) {
     TIT_HIP_TRY(c, hipMemcpy(dt_last, c->dt, sizeof(double), hipMemcpyDeviceToHost));
   }
-  return st;
+  return TIT_OK;
 }
 
 tit_status tit_hip_download(tit_hip_ctx* c, double* r, double* v, double* rho,

@@ -151,6 +151,27 @@ __device__ __forceinline__ double eos_density_from_potential(const DevParams& P,
 }
 
 // ---- compensated accumulation ----------------------------------------------
+// FluidEquations::compute_time_step for one fluid particle
+// (source/tit/sph/fluid_equations.hpp:199-222).
+template <int DIM>
+__device__ __forceinline__ double particle_time_step(const DevParams& P,
+                                                     double rho,
+                                                     const double* v,
+                                                     const double* dv) {
+  double v2 = v[0] * v[0];
+  double a2 = dv[0] * dv[0];
+#pragma unroll
+  for (int k = 1; k < DIM; ++k) {
+    v2 = v2 + v[k] * v[k];
+    a2 = a2 + dv[k] * dv[k];
+  }
+  double dt = P.cfl * P.h / (eos_sound_speed(P, rho) + sqrt(v2));
+  if (P.mu > 0.0) dt = fmin(dt, P.c_visc * (P.h * P.h) * rho / P.mu);
+  const double an = fmax(sqrt(a2), P.g);
+  dt = fmin(dt, P.c_force * sqrt(P.h / an));
+  return dt;
+}
+
 struct Kahan {
   float s, c;
   __device__ __forceinline__ void init() {

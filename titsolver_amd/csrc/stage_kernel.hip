@@ -202,22 +202,6 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
   return v;
 }
 
-template <int DIM>
-__device__ __forceinline__ double particle_dt(const DevParams& P, double rho,
-                                             const double* v, const double* dv) {
-  double v2 = v[0] * v[0];
-  double a2 = dv[0] * dv[0];
-#pragma unroll
-  for (int k = 1; k < DIM; ++k) {
-    v2 = v2 + v[k] * v[k];
-    a2 = a2 + dv[k] * dv[k];
-  }
-  double dt = P.cfl * P.h / (eos_sound_speed(P, rho) + sqrt(v2));
-  if (P.mu > 0.0) dt = fmin(dt, P.c_visc * (P.h * P.h) * rho / P.mu);
-  const double an = fmax(sqrt(a2), P.g);
-  return fmin(dt, P.c_force * sqrt(P.h / an));
-}
-
 struct ChunkInfo {
   uint32_t t, col_row, t0, t1;
   int cx, cy, nz;
@@ -321,7 +305,7 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
       }
       A.out.rho[i] = rhon;
       A.out.aux[i] = make_aux(P, rhon);
-      if constexpr (FINAL) dtmin = dbl_bits(particle_dt<DIM>(P, rhon, vn, dv));
+      if constexpr (FINAL) dtmin = dbl_bits(particle_time_step<DIM>(P, rhon, vn, dv));
     }
   }
   if constexpr (KIND != kRhsOnly) {

@@ -258,6 +258,124 @@ class HipContext:
         return n.value, rb.value
 
 
+def _fp(a):
+    if a is None:
+        return C.cast(None, C.POINTER(C.c_float))
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class LearnedStep:
+    """The learned 4 dt step of one HipContext (tit_hip_learned_* entry
+    points): network parameters, recorded training samples, training, and the
+    judged run with plain-step fallback."""
+
+    N_KNN = 8
+    NODE_FEAT = 16
+    EDGE_FEAT = 16
+    OUT = 16
+
+    def __init__(self, ctx: HipContext, seed: int = 0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        ctx._check(self.lib.tit_hip_learned_init(ctx.h, seed))
+
+    def params(self) -> np.ndarray:
+        n = C.c_size_t()
+        self.ctx._check(self.lib.tit_hip_learned_params(self.ctx.h, _fp(None), 0, C.byref(n)))
+        w = np.empty(n.value, dtype=np.float32)
+        self.ctx._check(self.lib.tit_hip_learned_params(self.ctx.h, _fp(w), w.size, C.byref(n)))
+        return w
+
+    def set_params(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self.ctx._check(self.lib.tit_hip_learned_set_params(self.ctx.h, _fp(w), w.size))
+
+    def scales(self) -> np.ndarray:
+        s = np.empty(6)
+        self.ctx._check(self.lib.tit_hip_learned_scales(self.ctx.h, _dp(s)))
+        return s
+
+    def set_scales(self, s):
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        assert s.size == 6
+        self.ctx._check(self.lib.tit_hip_learned_set_scales(self.ctx.h, _dp(s)))
+
+    def record(self, plain_steps: int = 4) -> int:
+        k = C.c_int()
+        self.ctx._check(self.lib.tit_hip_learned_record(self.ctx.h, plain_steps, C.byref(k)))
+        return k.value
+
+    def clear_samples(self):
+        self.ctx._check(self.lib.tit_hip_learned_clear_samples(self.ctx.h))
+
+    def sample(self, index: int) -> dict:
+        n = C.c_size_t()
+        null_u32 = C.cast(None, C.POINTER(C.c_uint32))
+        null_u8 = C.cast(None, C.POINTER(C.c_uint8))
+        self.ctx._check(self.lib.tit_hip_learned_sample(
+            self.ctx.h, index, C.byref(n), null_u32, _fp(None), _fp(None), _fp(None), null_u8))
+        n = n.value
+        nbr = np.empty((n, self.N_KNN), dtype=np.uint32)
+        nf = np.empty((n, self.NODE_FEAT), dtype=np.float32)
+        ef = np.empty((n, self.N_KNN, self.EDGE_FEAT), dtype=np.float32)
+        y = np.empty((n, 4), dtype=np.float32)
+        fl = np.empty(n, dtype=np.uint8)
+        self.ctx._check(self.lib.tit_hip_learned_sample(
+            self.ctx.h, index, C.byref(C.c_size_t()),
+            nbr.ctypes.data_as(C.POINTER(C.c_uint32)), _fp(nf), _fp(ef), _fp(y),
+            fl.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return {"nbr": nbr, "node_feat": nf, "edge_feat": ef, "target": y,
+                "fluid": fl.astype(bool)}
+
+    def loss_grad(self, index: int):
+        loss = C.c_double()
+        g = np.empty(self.params().size, dtype=np.float32)
+        self.ctx._check(self.lib.tit_hip_learned_loss_grad(self.ctx.h, index, C.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def train(self, iters: int, lr: float = 1e-3) -> np.ndarray:
+        losses = np.zeros(max(iters, 1))
+        self.ctx._check(self.lib.tit_hip_learned_train(self.ctx.h, iters, lr, _dp(losses)))
+        return losses[:iters]
+
+    def graph(self):
+        n = self.ctx.n
+        nbr = np.empty((n, self.N_KNN), dtype=np.uint32)
+        nf = np.empty((n, self.NODE_FEAT), dtype=np.float32)
+        ef = np.empty((n, self.N_KNN, self.EDGE_FEAT), dtype=np.float32)
+        self.ctx._check(self.lib.tit_hip_learned_graph(
+            self.ctx.h, nbr.ctypes.data_as(C.POINTER(C.c_uint32)), _fp(nf), _fp(ef)))
+        return nbr, nf, ef
+
+    def forward(self) -> np.ndarray:
+        out = np.empty((self.ctx.n, self.OUT), dtype=np.float32)
+        self.ctx._check(self.lib.tit_hip_learned_forward(self.ctx.h, _fp(out)))
+        return out
+
+    def run(self, n_steps: int, judge_every: int = 10, tol: float = 0.01) -> dict:
+        rep = _abi.LearnedReport()
+        self.ctx._check(self.lib.tit_hip_learned_run(self.ctx.h, n_steps, judge_every, tol,
+                                                     C.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in rep._fields_}
+
+
+def nn_gemm(A, B, transA=False, transB=False, bias=None, act=0, k_splits=1, C0=None):
+    """The network's BF16 MFMA GEMM on the current device (test hook)."""
+    lib = _abi.load()
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    M, K = (A.shape[1], A.shape[0]) if transA else A.shape
+    N = B.shape[0] if transB else B.shape[1]
+    out = np.zeros((M, N), dtype=np.float32) if C0 is None else np.array(C0, dtype=np.float32)
+    b = None if bias is None else np.ascontiguousarray(bias, dtype=np.float32)
+    st = lib.tit_hip_nn_gemm(M, N, K, _fp(A), int(transA), _fp(B), int(transB), _fp(b),
+                             act, k_splits, _fp(out))
+    if st != 0:
+        raise TitError(st, "tit_hip_nn_gemm failed")
+    return out
+
+
 class SSPRKIntegrator:
     """SSPRKIntegrator::step on device-resident particles."""
 
