@@ -226,6 +226,10 @@ tit_status tit_hip_learned_graph(tit_hip_ctx* ctx, uint32_t* nbr_ref,
                                  float* node_feat, float* edge_feat);
 /* Network output (n x 16, reference order) on the current state. */
 tit_status tit_hip_learned_forward(tit_hip_ctx* ctx, float* out);
+/* Inference path: 1 (default) runs each round as one fused kernel with the
+ * weights resident in LDS; 0 runs it as separate GEMM launches (the training
+ * path's forward).  Both give the same network to BF16 rounding. */
+tit_status tit_hip_learned_set_fused(tit_hip_ctx* ctx, int enable);
 
 typedef struct tit_hip_learned_report {
   int learned_steps;    /* learned steps taken, judged ones included */
@@ -235,6 +239,10 @@ typedef struct tit_hip_learned_report {
   double max_rho_err;   /* largest max_i |rho_L - rho_P| / rho_P seen */
   double last_rho_err;
   double max_front_err; /* largest |front_L - front_P| / |front_P| seen */
+  double max_dv_err;    /* largest |v_L - v_P| / |v_P - v_S| (L2 over fluid,
+                           S the judged step's start): error of the step's
+                           velocity change */
+  double last_dv_err;
 } tit_hip_learned_report;
 /* n_steps learned steps.  The last step of every group of judge_every is
  * judged: it is taken from state S and compared with 4 plain steps from S

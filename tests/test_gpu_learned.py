@@ -162,7 +162,8 @@ def test_training_reduces_loss_and_beats_untrained():
     w, sc = L.params(), L.scales()
     ctx.close()
 
-    # held-out tanks: one judged learned step, trained vs untrained network
+    # held-out tanks: one judged learned step; error of the velocity change
+    # against four plain steps, trained vs untrained network
     errs = {}
     for name in ("trained", "untrained"):
         hctx, _, _ = _batched(99, 2, 0.04)
@@ -172,10 +173,11 @@ def test_training_reduces_loss_and_beats_untrained():
         if name == "trained":
             H.set_params(w)
         rep = H.run(1, judge_every=1, tol=1e9)
-        errs[name] = rep["max_rho_err"]
+        errs[name] = (rep["max_dv_err"], rep["max_rho_err"])
         hctx.close()
-    print(f"held-out density error of one 4 dt step: {errs}")
-    assert errs["trained"] < errs["untrained"]
+    print(f"held-out (velocity-change error, density error) of one 4 dt step: {errs}")
+    assert errs["trained"][0] < 0.8 * errs["untrained"][0]
+    assert errs["trained"][1] < 0.01
 
 
 def test_run_with_zero_tolerance_is_the_plain_trajectory():
@@ -205,4 +207,30 @@ def test_run_accepting_advances_four_dt_per_step():
     assert t1[0] > t0[0] + 8 * dt0[0]
     g = ctx.download()
     assert np.all(np.isfinite(g["rho"])) and np.all(np.isfinite(g["r"]))
+    ctx.close()
+
+
+@pytest.mark.parametrize("n_tanks", [1, 3])
+def test_fused_forward_matches_unfused_and_oracle(n_tanks):
+    ctx, eq, _ = _batched(5, n_tanks, 0.03)
+    ctx.step(20)
+    L = ta.LearnedStep(ctx, seed=4)
+    w = L.params()
+    P = S.unpack(w)
+    # make both rounds matter: larger output weights
+    for p in P:
+        p["W5"] = p["W5"] * 8.0
+    L.set_params(S.pack(P).astype(np.float32))
+    w = L.params()
+    L.set_fused(True)
+    fused = L.forward()
+    L.set_fused(False)
+    unfused = L.forward()
+    nbr, nf, ef = L.graph()
+    u_o, _ = S.forward(w, nf, ef, nbr)
+    d_fu = np.linalg.norm(fused - unfused) / np.linalg.norm(unfused)
+    d_fo = np.linalg.norm(fused - u_o) / np.linalg.norm(u_o)
+    print(f"n={ctx.n}: fused vs unfused {d_fu:.2e}, fused vs FP64 oracle {d_fo:.2e}")
+    assert d_fu < 1e-2
+    assert d_fo < 3e-2
     ctx.close()

@@ -56,7 +56,8 @@ class LearnedReport(C.Structure):
     _fields_ = [("learned_steps", C.c_int), ("judged", C.c_int),
                 ("rejected", C.c_int), ("plain_steps", C.c_int),
                 ("max_rho_err", C.c_double), ("last_rho_err", C.c_double),
-                ("max_front_err", C.c_double)]
+                ("max_front_err", C.c_double), ("max_dv_err", C.c_double),
+                ("last_dv_err", C.c_double)]
 
 
 class KernelStat(C.Structure):
@@ -105,6 +106,7 @@ SIGNATURES = {
     "tit_hip_learned_train": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _DP]),
     "tit_hip_learned_graph": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), _FP, _FP]),
     "tit_hip_learned_forward": (C.c_int, [C.c_void_p, _FP]),
+    "tit_hip_learned_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
     "tit_hip_learned_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double,
                                       C.POINTER(LearnedReport)]),
     "tit_hip_nn_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, _FP, C.c_int, _FP,

@@ -342,15 +342,20 @@ __global__ void k_advance_time(int n_tanks, double* time, const double* dt, doub
 
 // ---- judge ------------------------------------------------------------------------
 
+// Keeps the state's density, x and velocity by particle id.
 template <int DIM>
 __global__ __launch_bounds__(kBlk) void k_keep_by_id(uint32_t n, DevState st,
                                                      const uint32_t* __restrict__ id,
                                                      double* __restrict__ rho_by_id,
-                                                     double* __restrict__ x_by_id) {
+                                                     double* __restrict__ x_by_id,
+                                                     double* __restrict__ v_by_id) {
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   if (i >= n) return;
-  rho_by_id[id[i]] = st.rho[i];
-  x_by_id[id[i]] = st.x[0][i];
+  const uint32_t j = id[i];
+  if (rho_by_id) rho_by_id[j] = st.rho[i];
+  if (x_by_id) x_by_id[j] = st.x[0][i];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) v_by_id[static_cast<size_t>(j) * DIM + k] = st.v[k][i];
 }
 
 __device__ __forceinline__ unsigned long long ordered_bits(double x) {
@@ -358,23 +363,59 @@ __device__ __forceinline__ unsigned long long ordered_bits(double x) {
   return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
 
-// out[0]: max relative density error (FP32 bits); out64[0], out64[1]: the
-// largest x over fluid of the plain and the learned states (ordered bits).
+// Judge of one learned step L against the plain reference P (the current
+// state), both from the start state S, over fluid particles:
+//   err_bits: max |rho_L - rho_P| / rho_P (FP32 bits)
+//   front[0], front[1]: largest x of P and of L (ordered bits)
+//   dv[0] += |v_L - v_P|^2, dv[1] += |v_P - v_S|^2
+template <int DIM>
 __global__ __launch_bounds__(kBlk) void k_judge(uint32_t n, DevState st,
                                                 const uint32_t* __restrict__ id,
                                                 const uint32_t* __restrict__ tag,
                                                 const double* __restrict__ rho_l,
                                                 const double* __restrict__ x_l,
+                                                const double* __restrict__ v_l,
+                                                const double* __restrict__ v_s,
                                                 uint32_t* __restrict__ err_bits,
-                                                unsigned long long* __restrict__ front) {
+                                                unsigned long long* __restrict__ front,
+                                                double* __restrict__ dv) {
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
-  if (i >= n || (tag[i] & 1u)) return;
-  const uint32_t j = id[i];
-  const double rp = st.rho[i];
-  const float e = static_cast<float>(fabs(rho_l[j] - rp) / rp);
-  atomicMax(err_bits, __float_as_uint(e));
-  atomicMax(front, ordered_bits(st.x[0][i]));
-  atomicMax(front + 1, ordered_bits(x_l[j]));
+  double num = 0.0, den = 0.0;
+  if (i < n && !(tag[i] & 1u)) {
+    const uint32_t j = id[i];
+    const double rp = st.rho[i];
+    const float e = static_cast<float>(fabs(rho_l[j] - rp) / rp);
+    atomicMax(err_bits, __float_as_uint(e));
+    atomicMax(front, ordered_bits(st.x[0][i]));
+    atomicMax(front + 1, ordered_bits(x_l[j]));
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const double vp = st.v[k][i];
+      const double a = v_l[static_cast<size_t>(j) * DIM + k] - vp;
+      const double b = vp - v_s[static_cast<size_t>(j) * DIM + k];
+      num += a * a;
+      den += b * b;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    num += __shfl_xor(num, o);
+    den += __shfl_xor(den, o);
+  }
+  __shared__ double red[2][kBlk / 64];
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = num;
+    red[1][threadIdx.x >> 6] = den;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < kBlk / 64; ++w) {
+      a += red[0][w];
+      b += red[1][w];
+    }
+    atomicAdd(dv, a);
+    atomicAdd(dv + 1, b);
+  }
 }
 
 // ---- recording --------------------------------------------------------------------
@@ -504,6 +545,8 @@ struct Snapshot {
 struct Learned {
   float* w32 = nullptr;
   bf16* w16 = nullptr;
+  bf16* frag = nullptr;      // MFMA fragments of w16 for the fused rounds
+  bool fused = true;         // inference through the fused round kernel
   float* grad = nullptr;
   float* adam_m = nullptr;
   float* adam_v = nullptr;
@@ -522,6 +565,9 @@ struct Learned {
   Snapshot accepted, before;
   double* rho_l = nullptr;
   double* x_l = nullptr;
+  double* v_l = nullptr;     // n x dim, by id
+  double* v_s = nullptr;     // n x dim, by id (judged step's start)
+  double* dv = nullptr;      // 2 sums
   uint32_t* err_bits = nullptr;
   unsigned long long* front = nullptr;
   // recording
@@ -877,6 +923,8 @@ tit_status need_learned(tit_hip_ctx* c) {
     L->v_ref = L->rho_ref = L->r_ref = L->p_ref = L->t0 = nullptr;
     TIT_HIP_TRY(c, lalloc(L->mem_n, &L->rho_l, c->n));
     TIT_HIP_TRY(c, lalloc(L->mem_n, &L->x_l, c->n));
+    TIT_HIP_TRY(c, lalloc(L->mem_n, &L->v_l, static_cast<size_t>(c->n) * c->prm.dim));
+    TIT_HIP_TRY(c, lalloc(L->mem_n, &L->v_s, static_cast<size_t>(c->n) * c->prm.dim));
     L->sized_n = c->n;
     L->sized_tanks = c->n_tanks;
   }
@@ -904,6 +952,7 @@ tit_status build_graph(tit_hip_ctx* c) {
   if (L->infer.cap < n) TIT_HIP_TRY(c, alloc_acts(L->infer, n, false));
   const DevParams& P = c->dp;
   const size_t E = static_cast<size_t>(n) * kKnn;
+  ProfScope ps(c->prof, s, "learned_graph");
   if (P.dim == 2) {
     k_knn<2><<<nblk(n), kBlk, 0, s>>>(P, c->tanks, n, c->A, c->key, c->tag, c->id,
                                       c->cell_off, L->nbr);
@@ -936,7 +985,17 @@ tit_status infer_current(tit_hip_ctx* c) {
   tit_status st = build_graph(c);
   if (st != TIT_OK) return st;
   Learned* L = c->learned;
-  for (int r = 0; r < kRounds; ++r) forward_round(c->stream, L, L->infer, r, c->n, L->ef, L->nbr);
+  Activations& A = L->infer;
+  ProfScope ps(c->prof, c->stream, "learned_network");
+  if (L->fused) {
+    for (int r = 0; r < kRounds; ++r) {
+      const bool last = r + 1 == kRounds;
+      launch_round_fused(c->stream, c->n_cu, c->n, r, L->frag, L->w32, L->nbr, L->ef, A.H[r],
+                         last ? nullptr : A.H[r + 1], last ? A.u[r] : nullptr);
+    }
+  } else {
+    for (int r = 0; r < kRounds; ++r) forward_round(c->stream, L, A, r, c->n, L->ef, L->nbr);
+  }
   return check_launch(c, "learned forward");
 }
 
@@ -948,6 +1007,7 @@ tit_status learned_step(tit_hip_ctx* c) {
   if (st != TIT_OK) return st;
   Learned* L = c->learned;
   constexpr double kFactor = 4.0;
+  ProfScope ps(c->prof, c->stream, "learned_update");
   if (c->dp.dim == 2) {
     k_learned_update<2><<<nblk(c->n), kBlk, 0, c->stream>>>(
         c->dp, L->sc, c->n, c->A, c->tag, c->dt, kFactor, L->infer.u[kRounds - 1],
@@ -1021,6 +1081,7 @@ tit_status upload_weights(tit_hip_ctx* c, const float* w) {
   Learned* L = c->learned;
   TIT_HIP_TRY(c, hipMemcpy(L->w32, w, kParams * sizeof(float), hipMemcpyHostToDevice));
   k_to_bf16<<<nblk(kParams), kBlk, 0, c->stream>>>(kParams, L->w32, L->w16);
+  pack_fragments(c->stream, L->w16, L->frag);
   TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
   return check_launch(c, "weights");
 }
@@ -1041,12 +1102,14 @@ tit_status tit_hip_learned_init(tit_hip_ctx* c, uint64_t seed) {
   c->learned = L;
   TIT_HIP_TRY(c, lalloc(L->mem, &L->w32, kParams));
   TIT_HIP_TRY(c, lalloc(L->mem, &L->w16, kParams));
+  TIT_HIP_TRY(c, lalloc(L->mem, &L->frag, fused_fragment_elems()));
   TIT_HIP_TRY(c, lalloc(L->mem, &L->grad, kParams));
   TIT_HIP_TRY(c, lalloc(L->mem, &L->adam_m, kParams));
   TIT_HIP_TRY(c, lalloc(L->mem, &L->adam_v, kParams));
   TIT_HIP_TRY(c, lalloc(L->mem, &L->loss, 1));
   TIT_HIP_TRY(c, lalloc(L->mem, &L->err_bits, 1));
   TIT_HIP_TRY(c, lalloc(L->mem, &L->front, 2));
+  TIT_HIP_TRY(c, lalloc(L->mem, &L->dv, 2));
   L->sc = default_scales(c);
   std::vector<float> w;
   init_weights(w, seed);
@@ -1079,6 +1142,13 @@ tit_status tit_hip_learned_set_params(tit_hip_ctx* c, const float* w, size_t n) 
   fill_f32(c->stream, kParams, c->learned->adam_m, 0.0f);
   fill_f32(c->stream, kParams, c->learned->adam_v, 0.0f);
   return upload_weights(c, w);
+}
+
+tit_status tit_hip_learned_set_fused(tit_hip_ctx* c, int enable) {
+  tit_status st = need_learned(c);
+  if (st != TIT_OK) return st;
+  c->learned->fused = enable != 0;
+  return TIT_OK;
 }
 
 tit_status tit_hip_learned_scales(tit_hip_ctx* c, double* out) {
@@ -1347,6 +1417,7 @@ tit_status tit_hip_learned_train(tit_hip_ctx* c, int iters, double lr, double* l
     adam_update(s, kParams, L->w32, L->w16, L->grad, L->adam_m, L->adam_v,
                 static_cast<float>(lr), b1, b2, eps, bc1, bc2);
   }
+  pack_fragments(s, L->w16, L->frag);
   hipError_t e = hipStreamSynchronize(s);
   if (e == hipSuccess && losses && iters > 0) {
     e = hipMemcpy(losses, d_losses, iters * sizeof(double), hipMemcpyDeviceToHost);
@@ -1378,24 +1449,37 @@ tit_status tit_hip_learned_run(tit_hip_ctx* c, int n_steps, int judge_every, dou
     }
     // judged step: learned and plain from the same state
     TIT_HIP_TRY(c, save(c, L->before));
+    if (c->dp.dim == 2) {
+      k_keep_by_id<2><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, nullptr, nullptr, L->v_s);
+    } else {
+      k_keep_by_id<3><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, nullptr, nullptr, L->v_s);
+    }
     st = learned_step(c);
     if (st != TIT_OK) return st;
     if (c->dp.dim == 2) {
-      k_keep_by_id<2><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, L->rho_l, L->x_l);
+      k_keep_by_id<2><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, L->rho_l, L->x_l, L->v_l);
     } else {
-      k_keep_by_id<3><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, L->rho_l, L->x_l);
+      k_keep_by_id<3><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, L->rho_l, L->x_l, L->v_l);
     }
     TIT_HIP_TRY(c, restore(c, L->before));
     st = run_plain(c, kPlainPerLearned);
     if (st != TIT_OK) return st;
     TIT_HIP_TRY(c, hipMemsetAsync(L->err_bits, 0, sizeof(uint32_t), s));
     TIT_HIP_TRY(c, hipMemsetAsync(L->front, 0, 2 * sizeof(unsigned long long), s));
-    k_judge<<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, c->tag, L->rho_l, L->x_l,
-                                        L->err_bits, L->front);
+    TIT_HIP_TRY(c, hipMemsetAsync(L->dv, 0, 2 * sizeof(double), s));
+    if (c->dp.dim == 2) {
+      k_judge<2><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, c->tag, L->rho_l, L->x_l,
+                                             L->v_l, L->v_s, L->err_bits, L->front, L->dv);
+    } else {
+      k_judge<3><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, c->tag, L->rho_l, L->x_l,
+                                             L->v_l, L->v_s, L->err_bits, L->front, L->dv);
+    }
     uint32_t eb = 0;
     unsigned long long fr[2] = {0, 0};
+    double dvs[2] = {0.0, 0.0};
     TIT_HIP_TRY(c, hipMemcpyAsync(&eb, L->err_bits, 4, hipMemcpyDeviceToHost, s));
     TIT_HIP_TRY(c, hipMemcpyAsync(fr, L->front, 16, hipMemcpyDeviceToHost, s));
+    TIT_HIP_TRY(c, hipMemcpyAsync(dvs, L->dv, 16, hipMemcpyDeviceToHost, s));
     TIT_HIP_TRY(c, hipStreamSynchronize(s));
     st = check_launch(c, "judge");
     if (st != TIT_OK) return st;
@@ -1413,6 +1497,9 @@ tit_status tit_hip_learned_run(tit_hip_ctx* c, int n_steps, int judge_every, dou
     R.judged += 1;
     R.max_rho_err = std::max(R.max_rho_err, static_cast<double>(err));
     R.max_front_err = std::max(R.max_front_err, ferr);
+    const double dverr = dvs[1] > 0.0 ? std::sqrt(dvs[0] / dvs[1]) : 0.0;
+    R.max_dv_err = std::max(R.max_dv_err, dverr);
+    R.last_dv_err = dverr;
     R.last_rho_err = err;
     R.plain_steps += kPlainPerLearned;
     if (!(static_cast<double>(err) <= tol)) {
@@ -1427,6 +1514,7 @@ tit_status tit_hip_learned_run(tit_hip_ctx* c, int n_steps, int judge_every, dou
     done += group;
   }
   TIT_HIP_TRY(c, hipStreamSynchronize(s));
+  if (c->prof.enabled) c->prof.collect();
   st = read_flags(c);
   if (rep) *rep = R;
   return st;

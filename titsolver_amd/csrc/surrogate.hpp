@@ -61,4 +61,14 @@ struct LearnedScales {
   float out_scale[4];
 };
 
+// Fused inference (surrogate_fused.hip): MFMA operand fragments of both
+// rounds' weights, packed from the BF16 parameter vector, and one launch per
+// round.  H (n x kNodeIn) is the round's node input; Hn (may be null)
+// receives [nf | u] for the next round and u (may be null) the outputs.
+size_t fused_fragment_elems();
+void pack_fragments(hipStream_t s, const bf16* w16, bf16* frag);
+void launch_round_fused(hipStream_t s, int n_cu, uint32_t n, int round,
+                        const bf16* frag, const float* w32, const uint32_t* nbr,
+                        const bf16* ef, const bf16* H, bf16* Hn, float* u);
+
 }  // namespace tit_amd

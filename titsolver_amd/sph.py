@@ -353,6 +353,9 @@ class LearnedStep:
         self.ctx._check(self.lib.tit_hip_learned_forward(self.ctx.h, _fp(out)))
         return out
 
+    def set_fused(self, on: bool):
+        self.ctx._check(self.lib.tit_hip_learned_set_fused(self.ctx.h, 1 if on else 0))
+
     def run(self, n_steps: int, judge_every: int = 10, tol: float = 0.01) -> dict:
         rep = _abi.LearnedReport()
         self.ctx._check(self.lib.tit_hip_learned_run(self.ctx.h, n_steps, judge_every, tol,
