@@ -60,7 +60,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--dr", type=float, default=0.001)
+    ap.add_argument("--dr", type=float, default=None,
+                    help="particle spacing (default: the workload's)")
+    ap.add_argument("--workload", choices=["dam2d", "dam3d", "batched"], default="dam2d",
+                    help="dam2d: 2D dam break (default, dr = 1 mm, 2.0 M fluid); "
+                         "dam3d: 3D dam break (dr = 1 cm, h = 1.3 dr, 2.0 M fluid); "
+                         "batched: 256 seeded 2D tanks (dr = 5 mm)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default stage schedule")
     ap.add_argument("--profile-steps", type=int, default=60,
                     help="eager steps with per-launch HIP events after the "
                          "timed region (kernel durations)")
@@ -74,6 +81,13 @@ def parse():
     ap.add_argument("--copy-probe-mb", type=int, default=512,
                     help="device-to-device copy of this many MiB (attainable "
                          "HBM bandwidth; reference kernel for counter scaling)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-step leg")
+    ap.add_argument("--learned-particles", type=int, default=1_000_000,
+                    help="fluid particles of the held-out evaluation batch")
+    ap.add_argument("--learned-steps", type=int, default=50,
+                    help="timed learned steps (4 dt each; judge every 10)")
+    ap.add_argument("--train-iters", type=int, default=1200)
     return ap.parse_args()
 
 
@@ -107,19 +121,23 @@ def main():
         sys.exit(2)
     import titsolver_amd as ta
 
-    dr = args.dr
-    print(f"bench: building 2D dam break dr={dr}", file=sys.stderr, flush=True)
-    pa = ta.dam_break(2, dr, TANK, COLUMN)
-    eq = ta.cfg_equations(2, dr, COLUMN[1])
-    nf = pa.n_fluid
+    arrays, eq, desc = build_workload(args)
+    pa = arrays[0]
+    nf = sum(a.n_fluid for a in arrays)
+    n_fixed = sum(a.n_fixed for a in arrays)
+    dim = eq.dim
     ctx = ta.HipContext(eq, order=3)
-    ctx.upload([pa])
-    if args.chunk or args.lds_cap >= 0:
-        ctx.set_schedule(args.chunk, None if args.lds_cap < 0 else args.lds_cap)
-    sched = ctx.get_schedule()
-    print(f"bench: {nf} fluid + {pa.n_fixed} fixed particles uploaded; "
+    ctx.upload(arrays)
+    print(f"bench: {desc}: {nf} fluid + {n_fixed} fixed particles uploaded; "
           f"warmup {args.warmup}", file=sys.stderr, flush=True)
     ctx.step(args.warmup)
+    tuned = None
+    if args.chunk or args.lds_cap >= 0:
+        ctx.set_schedule(args.chunk, None if args.lds_cap < 0 else args.lds_cap)
+    elif not args.no_autotune:
+        tuned = ctx.autotune(4)
+        print(f"bench: autotuned schedule {tuned}", file=sys.stderr, flush=True)
+    sched = ctx.get_schedule()
 
     # Timed region: K steps replayed from the captured two-step hipGraph;
     # step() returns after the context's stream has drained.
@@ -140,7 +158,7 @@ def main():
                for k, (n, t) in stats.items()}
     st_launches, st_ms = stats["stage"]
     avg_stage_s = st_ms / st_launches * 1e-3
-    bytes_launch = stage_bytes_per_particle(2) * nf
+    bytes_launch = stage_bytes_per_particle(dim) * nf
     achieved = bytes_launch / avg_stage_s / 1e9
     t_step, dt_next = ctx.tank_clock()
 
@@ -158,12 +176,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 state/integrator, f32 pair sums (Kahan)",
         "data": "synthetic (lattice dam break, hydrostatic start)",
-        "config": {"workload": f"2D dam break dr={dr} m, {nf} fluid + "
-                               f"{pa.n_fixed} fixed particles, Wendland C2 h=2dr, "
-                               "delta-SPH 0.1, alpha 0.02, SSPRK3 CFL 0.25",
-                   "fluid_particles": nf, "fixed_particles": pa.n_fixed,
+        "config": {"workload": f"{desc}, {nf} fluid + {n_fixed} fixed particles, "
+                               f"Wendland C2 h={eq.h / args.dr:g}dr, delta-SPH 0.1, "
+                               "alpha 0.02, SSPRK3 CFL 0.25",
+                   "fluid_particles": nf, "fixed_particles": n_fixed,
                    "sim_time_s": float(t_step[0]), "dt_next_s": float(dt_next[0]),
-                   "schedule": sched},
+                   "schedule": sched, "autotune": tuned},
         "roofline": {"bound": "hbm", "kernel": "k_stage (fused pair sums + SSPRK update)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -174,11 +192,130 @@ def main():
     }
     ctx.close()
 
+    if not args.no_learned and args.workload == "dam2d":
+        out["learned"] = learned_leg(args)
     if args.copy_probe_mb > 0:
         out["copy_probe"] = copy_probe(args.copy_probe_mb)
-    if not args.no_cpu and args.cpu_steps > 0:
+    if not args.no_cpu and args.cpu_steps > 0 and len(arrays) == 1:
         out["cpu_baseline"], out["parity"] = cpu_baseline(pa, eq, args.cpu_steps)
     print(json.dumps(out), flush=True)
+
+
+def build_workload(args):
+    import titsolver_amd as ta
+    if args.workload == "dam2d":
+        args.dr = args.dr or 0.001
+        arrays = [ta.dam_break(2, args.dr, TANK, COLUMN)]
+        eq = ta.cfg_equations(2, args.dr, COLUMN[1])
+        return arrays, eq, f"2D dam break dr={args.dr} m"
+    if args.workload == "dam3d":
+        # tank 3.22 x 2.4 (open top) x 1.0 m, water 1.0 x 2.0 x 1.0 m
+        args.dr = args.dr or 0.01
+        k = round(1.0 / args.dr)
+        tank3 = [TANK[0], TANK[1], (k + 1) * args.dr]
+        arrays = [ta.dam_break(3, args.dr, tank3, [COLUMN[0], COLUMN[1], k * args.dr])]
+        eq = ta.cfg_equations(3, args.dr, COLUMN[1], h_factor=1.3)
+        return arrays, eq, f"3D dam break dr={args.dr} m"
+    args.dr = args.dr or 0.005
+    arrays = seeded_tanks(0, args.dr, n_tanks=256)
+    eq = ta.cfg_equations(2, args.dr, 2.0)
+    return arrays, eq, f"256 batched 2D dam breaks (seeded columns) dr={args.dr} m"
+
+
+def seeded_tanks(seed: int, dr: float, n_tanks: int = 0, min_fluid: int = 0):
+    """Dam breaks in the 3.22 x 2.4 m tank with seeded columns
+    U[0.3, 1.2] x U[0.5, 2.0] m (the batched configuration); n_tanks of them,
+    or as many as needed to reach min_fluid fluid particles."""
+    import titsolver_amd as ta
+    rng = np.random.default_rng(seed)
+    arrays, nf = [], 0
+    while (n_tanks and len(arrays) < n_tanks) or (not n_tanks and nf < min_fluid):
+        col = [float(rng.uniform(0.3, 1.2)), float(rng.uniform(0.5, 2.0))]
+        arrays.append(ta.dam_break(2, dr, TANK, col))
+        nf += arrays[-1].n_fluid
+    return arrays
+
+
+def learned_leg(args):
+    """Learned 4 dt step: trained on this solver's own trajectories of seeded
+    batched tanks (dr = 5 mm), evaluated on held-out seeds with a plain-step
+    judge every 10 learned steps (fallback above 1% density error).  Both forms
+    are timed on the same held-out batch from the same state; a learned step
+    counts as 4 steps of dt."""
+    import titsolver_amd as ta
+    dr = 0.005
+    eq = ta.cfg_equations(2, dr, 2.0)
+    # training data: 4 seeded tanks, 12 samples over the first ~250 steps
+    tr = seeded_tanks(0, dr, n_tanks=4)
+    ctx = ta.HipContext(eq)
+    ctx.upload(tr)
+    net = ta.LearnedStep(ctx, seed=0)
+    ctx.step(20)
+    for _ in range(12):
+        net.record(4)
+        ctx.step(16)
+    print(f"bench: training on {sum(a.n_fluid for a in tr)} fluid particles x 12 samples",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    losses = net.train(args.train_iters, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    w, sc = net.params(), net.scales()
+    ctx.close()
+
+    ev = seeded_tanks(1, dr, min_fluid=args.learned_particles)
+    nf = sum(a.n_fluid for a in ev)
+    res = {"workload": f"{len(ev)} held-out seeded 2D tanks, dr={dr} m, {nf} fluid particles",
+           "fluid_particles": nf, "train_iters": args.train_iters,
+           "train_s": t_train, "train_loss_first": float(np.mean(losses[:12])),
+           "train_loss_last": float(np.mean(losses[-12:]))}
+    # plain form
+    pctx = ta.HipContext(eq)
+    pctx.upload(ev)
+    pctx.step(40)
+    t0 = time.perf_counter()
+    k_plain = 4 * args.learned_steps
+    pctx.step(k_plain)
+    t_plain = time.perf_counter() - t0
+    pctx.close()
+    # learned form, same start state
+    lctx = ta.HipContext(eq)
+    lctx.upload(ev)
+    lctx.step(40)
+    L = ta.LearnedStep(lctx, seed=1)
+    L.set_scales(sc)
+    L.set_params(w)
+    L.run(10, judge_every=10, tol=0.01)  # warm-up (allocations, graphs)
+    t0 = time.perf_counter()
+    rep = L.run(args.learned_steps, judge_every=10, tol=0.01)
+    t_learned = time.perf_counter() - t0
+    lctx.set_profiling(True)
+    L.run(10, judge_every=10, tol=0.01)
+    stats = lctx.kernel_stats()
+    lctx.close()
+    plain_v = nf * k_plain / t_plain
+    learned_v = nf * 4 * args.learned_steps / t_learned
+    # MFMA work of the two fused rounds: per node, 8 edges x (80x128 + 128x128)
+    # + (160x128 + 128x128 + 128x16) multiply-adds per round
+    n_all = sum(a.n for a in ev)
+    flop = 2 * 2 * n_all * (8 * (80 * 128 + 128 * 128) + 160 * 128 + 128 * 128 + 128 * 16)
+    net_n, net_ms = stats.get("learned_network", (0, 0.0))
+    t_net = net_ms * 1e-3 / max(net_n, 1)
+    res["mfma"] = {"flop_per_learned_step": flop, "network_us": t_net * 1e6,
+                   "achieved_TFLOPs": flop / t_net / 1e12 if t_net else None,
+                   "peak_TFLOPs": 2500.0,
+                   "frac": flop / t_net / 2.5e15 if t_net else None}
+    res.update({
+        "plain_value": plain_v, "plain_ms_per_step": 1e3 * t_plain / k_plain,
+        "value": learned_v, "unit": "fluid particle-steps/s (a learned step counts 4)",
+        "ms_per_learned_step": 1e3 * t_learned / args.learned_steps,
+        "speedup_vs_plain": learned_v / plain_v,
+        "judge": rep,
+        "kernels_us_per_launch": {k: 1e3 * t / max(n, 1) for k, (n, t) in stats.items()},
+    })
+    print(f"bench: learned {res['ms_per_learned_step']:.3f} ms/step (4 dt), plain "
+          f"{res['plain_ms_per_step']:.3f} ms/step, ratio {res['speedup_vs_plain']:.2f}",
+          file=sys.stderr, flush=True)
+    return res
 
 
 def cpu_baseline(pa, eq, steps):

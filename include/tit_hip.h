@@ -166,6 +166,14 @@ tit_status tit_hip_get_schedule(tit_hip_ctx* ctx, tit_hip_schedule* out);
 /* chunk_targets = 0 keeps the current chunk length. */
 tit_status tit_hip_set_schedule(tit_hip_ctx* ctx, const tit_hip_schedule* s);
 
+/* Autotuner of the stage schedule: times every candidate (chunk_targets in
+ * {32, 48, 64, 96, 128} x, in 2D, LDS capacity {max, max/2, 0}) over
+ * steps_per_candidate real steps of the current state (all schedules give the
+ * same bits, so the steps are ordinary progress), keeps the fastest and
+ * returns it with its time per step.  No reference counterpart. */
+tit_status tit_hip_autotune(tit_hip_ctx* ctx, int steps_per_candidate,
+                            tit_hip_schedule* best, double* best_ms_per_step);
+
 /* Number of particles (all tanks) and of neighbour-index rebuilds so far. */
 tit_status tit_hip_counts(tit_hip_ctx* ctx, size_t* n_particles,
                           uint64_t* n_rebuilds);

@@ -219,3 +219,27 @@ def test_schedules_are_bitwise_identical(dim):
         for f in ("r", "v", "rho"):
             assert np.array_equal(o[f], outs[0][f]), f
     assert base["chunk_targets"] >= 1 and base["occupancy"] > 1.0
+
+
+def test_autotune_keeps_the_trajectory():
+    """The autotuner's candidate runs are ordinary steps: after it, the state
+    equals a plain run of the same number of steps, bit for bit."""
+    dr = 0.02
+    pa = perturbed_dam_break(2, dr, seed=3)
+    eq = _eq(2, dr)
+    a = _ctx(eq, [pa])
+    best = a.autotune(2)
+    n_a, steps = a.counts()
+    assert best["chunk_targets"] in (32, 48, 64, 96, 128)
+    assert best["ms_per_step"] > 0
+    b = _ctx(eq, [pa])
+    b.step(steps)
+    ga, gb = a.download(), b.download()
+    for k in ("r", "v", "rho"):
+        assert np.array_equal(ga[k], gb[k]), k
+    # and it keeps stepping identically under the chosen schedule
+    a.step(3)
+    b.step(3)
+    assert np.array_equal(a.download()["rho"], b.download()["rho"])
+    a.close()
+    b.close()

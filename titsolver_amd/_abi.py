@@ -87,6 +87,7 @@ SIGNATURES = {
                                        C.POINTER(C.c_int)]),
     "tit_hip_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
     "tit_hip_set_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
+    "tit_hip_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Schedule), _DP]),
     "tit_hip_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t),
                                  C.POINTER(C.c_uint64)]),
     "tit_hip_last_error": (C.c_char_p, [C.c_void_p]),

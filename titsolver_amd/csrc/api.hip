@@ -871,6 +871,67 @@ tit_status tit_hip_set_schedule(tit_hip_ctx* c, const tit_hip_schedule* sch) {
   return build_schedule(c);
 }
 
+tit_status tit_hip_autotune(tit_hip_ctx* c, int steps_per_candidate,
+                            tit_hip_schedule* best, double* best_ms_per_step) {
+  if (!c) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "autotune before upload");
+  if (steps_per_candidate < 2) return fail(c, TIT_E_ARG, "need at least 2 steps per candidate");
+  if (c->prof.enabled) return fail(c, TIT_E_STATE, "autotune with profiling enabled");
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  const int timed = steps_per_candidate / 2 * 2;
+  const uint32_t lds_max = stage_lds_capacity(c->prm.dim);
+  std::vector<uint32_t> chunks = {32, 48, 64, 96, 128};
+  std::vector<uint32_t> caps = {lds_max};
+  if (lds_max > 0) {
+    caps.push_back(lds_max / 2);
+    caps.push_back(0);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  TIT_HIP_TRY(c, hipEventCreate(&e0));
+  TIT_HIP_TRY(c, hipEventCreate(&e1));
+  double best_ms = 1e300;
+  uint32_t best_chunk = c->chunk_targets, best_cap = c->lds_cap;
+  tit_status st = TIT_OK;
+  for (uint32_t ch : chunks) {
+    for (uint32_t cap : caps) {
+      c->chunk_targets = ch;
+      c->lds_cap = cap;
+      st = build_schedule(c);
+      if (st == TIT_OK) st = run_plain(c, 2);  // captures this parity's graph
+      if (st != TIT_OK) break;
+      hipError_t e = hipEventRecord(e0, c->stream);
+      if (e == hipSuccess) st = run_plain(c, timed);
+      if (e == hipSuccess && st == TIT_OK) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess && st == TIT_OK) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess && st == TIT_OK) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) st = fail(c, TIT_E_HIP, std::string("autotune: ") + hipGetErrorString(e));
+      if (st != TIT_OK) break;
+      const double per = ms / timed;
+      if (per < best_ms) {
+        best_ms = per;
+        best_chunk = ch;
+        best_cap = cap;
+      }
+    }
+    if (st != TIT_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != TIT_OK) return st;
+  c->chunk_targets = best_chunk;
+  c->lds_cap = best_cap;
+  st = build_schedule(c);
+  if (st != TIT_OK) return st;
+  if (best) {
+    best->chunk_targets = static_cast<int>(best_chunk);
+    best->lds_capacity = static_cast<int>(best_cap);
+    best->occupancy = c->occupancy;
+  }
+  if (best_ms_per_step) *best_ms_per_step = best_ms;
+  return TIT_OK;
+}
+
 tit_status tit_hip_counts(tit_hip_ctx* c, size_t* n_particles, uint64_t* n_rebuilds) {
   if (!c) return TIT_E_ARG;
   if (n_particles) *n_particles = c->n;

@@ -17,14 +17,6 @@ typedef __attribute__((ext_vector_type(16))) float f32x16;
 
 constexpr int kBM = 64, kBN = 64, kBK = 16;
 
-__device__ __forceinline__ float silu(float x) {
-  return x / (1.0f + __expf(-x));
-}
-
-__device__ __forceinline__ float silu_grad(float x) {
-  const float s = 1.0f / (1.0f + __expf(-x));
-  return s * (1.0f + x * (1.0f - s));
-}
 
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) bf16 sA[kBM][kBK];
@@ -92,7 +84,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
       if (g.G1) x += g.G1[static_cast<size_t>(g.r1[m]) * g.N + n];
       if (g.G2) x += g.G2[static_cast<size_t>(g.r2[m]) * g.N + n];
       if (g.pre) g.pre[static_cast<size_t>(m) * g.ldpre + n] = x;
-      const float y = g.act == kActSilu ? silu(x) : x;
+      const float y = g.act == kActSilu ? nn_silu(x) : x;
       if (g.out32) g.out32[static_cast<size_t>(m) * g.ldo32 + n] = y;
       if (g.out16) g.out16[static_cast<size_t>(m) * g.ldo16 + n] = static_cast<bf16>(y);
     }
@@ -105,7 +97,7 @@ __global__ void k_silu_backward(int M, int N, const float* dy, int ld_dy,
   const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= static_cast<size_t>(M) * N) return;
   const size_t m = t / N, n = t % N;
-  const float g = dy[(m / group) * ld_dy + n] * silu_grad(pre[t]);
+  const float g = dy[(m / group) * ld_dy + n] * nn_silu_grad(pre[t]);
   if (dx32) dx32[t] = g;
   if (dx16) dx16[t] = static_cast<bf16>(g);
 }

@@ -12,6 +12,18 @@ typedef __bf16 bf16;
 
 enum NnAct { kActNone = 0, kActSilu = 1 };
 
+// Logistic function 1 / (1 + e^-x) with the hardware exp2 and reciprocal
+// (about 1 ulp each); every SiLU of the network (training GEMM epilogues,
+// their derivative and the fused inference kernel) uses this one.
+__device__ __forceinline__ float nn_sigmoid(float x) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * x));
+}
+__device__ __forceinline__ float nn_silu(float x) { return x * nn_sigmoid(x); }
+__device__ __forceinline__ float nn_silu_grad(float x) {
+  const float s = nn_sigmoid(x);
+  return s * (1.0f + x * (1.0f - s));
+}
+
 // C[M x N] = act(op(A)[M x K] . op(B)[K x N] + bias + G1[r1[m]] + G2[r2[m]])
 // with BF16 operands and FP32 accumulation.  op(A) is A (row-major, lda) or,
 // with transA, the transpose of a row-major K x M matrix; likewise for B.

@@ -201,16 +201,19 @@ __global__ __launch_bounds__(kBlock) void k_cell_offsets(
 // ---- boundary extrapolation ---------------------------------------------------
 // FluidEquations::setup_boundary (source/tit/sph/fluid_equations.hpp:122-164):
 // Shepard average of H(rho_b) + g (r_be . n_e) n_e[1] over fluid neighbours,
-// solved for rho_e; v_e = 0.  FP64 throughout.  One wave per fixed particle:
-// the lanes split the stencil's candidates and the two sums are reduced with
-// wave shuffles.
+// solved for rho_e; v_e = 0.  FP64 throughout.  A group of kBoundaryLanes
+// lanes per fixed particle: the lanes split the stencil's candidates and the
+// two sums are reduced with shuffles inside the group.
+
+constexpr int kBoundaryLanes = 16;
 
 template <int DIM, int KERNEL>
 __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   constexpr int NCOL = DIM == 2 ? 2 * kStencil + 1 : (2 * kStencil + 1) * (2 * kStencil + 1);
-  const uint32_t k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (k >= A.n_fixed) return;  // wave-uniform
+  constexpr int G = kBoundaryLanes;
+  const uint32_t k = blockIdx.x * (kBlock / G) + (threadIdx.x / G);
+  const int lane = threadIdx.x % G;
+  if (k >= A.n_fixed) return;  // uniform in the group
   const DevParams& P = A.P;
   const uint32_t i = A.fixed_list[k];
   const DevTank T = A.tanks[A.tag[i] >> 1];
@@ -283,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   }
   double S = 0.0, H = 0.0;
   const double xi_1 = P.xi - 1.0, inv_xi_1 = 1.0 / xi_1, cs2 = P.cs0 * P.cs0;
-  for (uint32_t t = lane; t < rl[NCOL]; t += 64) {
+  for (uint32_t t = lane; t < rl[NCOL]; t += G) {
     int q = 0;
     while (t >= rl[q + 1]) ++q;
     const uint32_t j = rb[q] + (t - rl[q]);
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     H += V_b * (H_b + P.g * rn * ne[1]) * W;
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
+  for (int off = G / 2; off > 0; off >>= 1) {
     S += __shfl_xor(S, off);
     H += __shfl_xor(H, off);
   }
@@ -508,7 +511,7 @@ void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
 
 void launch_boundary(hipStream_t s, const BoundaryArgs& a) {
   if (a.n_fixed == 0) return;
-  const uint32_t g = grid_for(a.n_fixed, kBlock / 64);
+  const uint32_t g = grid_for(a.n_fixed, kBlock / kBoundaryLanes);
   if (a.P.dim == 2) {
     if (a.P.kernel == kKernelC4) {
       k_boundary<2, kKernelC4><<<g, kBlock, 0, s>>>(a);

@@ -48,7 +48,6 @@ constexpr int kThreads = 512;
 constexpr int kYld = 168;  // padded LDS row of y = [H | M]
 constexpr int kHld = 136;  // padded LDS row of hidden node activations
 
-__device__ __forceinline__ float silu(float x) { return x / (1.0f + __expf(-x)); }
 
 // ---- fragment packing -------------------------------------------------------------
 
@@ -212,7 +211,7 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
           for (int t = 0; t < 8; ++t) {
             const int q = 2 * jj + (t >> 2), s = t & 3;
             const int f = 32 * ob + 8 * q + 4 * h + s;
-            v[t] = static_cast<bf16>(silu(acc[c][8 * jj + t] + b1[f]));
+            v[t] = static_cast<bf16>(nn_silu(acc[c][8 * jj + t] + b1[f]));
           }
           H1[c][2 * ob + jj] = v;
         }
@@ -239,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
 #pragma unroll
           for (int q = 2 * jj; q < 2 * jj + 2; ++q) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) s += silu(acc[c][4 * q + e] + bb);
+            for (int e = 0; e < 4; ++e) s += nn_silu(acc[c][4 * q + e] + bb);
           }
           const int pl = 4 * blk + 2 * h + jj;  // tile-local particle
           sY[pl * kYld + kNodeIn + f] = static_cast<bf16>(s);
@@ -270,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const int f = 32 * ob + 8 * q + 4 * h + s;
-          sH3[prow * kHld + f] = static_cast<bf16>(silu(acc[4 * q + s] + b3[f]));
+          sH3[prow * kHld + f] = static_cast<bf16>(nn_silu(acc[4 * q + s] + b3[f]));
         }
       }
     }
@@ -288,7 +287,7 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const int f = 32 * ob + 8 * q + 4 * h + s;
-          sY[prow * kHld + f] = static_cast<bf16>(silu(acc[4 * q + s] + b4[f]));
+          sY[prow * kHld + f] = static_cast<bf16>(nn_silu(acc[4 * q + s] + b4[f]));
         }
       }
     }

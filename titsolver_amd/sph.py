@@ -251,6 +251,15 @@ class HipContext:
                             else lds_capacity, 0.0)
         self._check(self.lib.tit_hip_set_schedule(self.h, C.byref(sch)))
 
+    def autotune(self, steps_per_candidate: int = 4) -> dict:
+        """Times every stage schedule over real steps, keeps the fastest."""
+        sch = _abi.Schedule()
+        ms = C.c_double()
+        self._check(self.lib.tit_hip_autotune(self.h, steps_per_candidate, C.byref(sch),
+                                              C.byref(ms)))
+        return {"chunk_targets": sch.chunk_targets, "lds_capacity": sch.lds_capacity,
+                "occupancy": sch.occupancy, "ms_per_step": ms.value}
+
     def counts(self):
         n = C.c_size_t()
         rb = C.c_uint64()
