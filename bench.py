@@ -87,7 +87,7 @@ def parse():
                     help="fluid particles of the held-out evaluation batch")
     ap.add_argument("--learned-steps", type=int, default=50,
                     help="timed learned steps (4 dt each; judge every 10)")
-    ap.add_argument("--train-iters", type=int, default=1200)
+    ap.add_argument("--train-iters", type=int, default=2400)
     return ap.parse_args()
 
 
@@ -245,8 +245,8 @@ def learned_leg(args):
     import titsolver_amd as ta
     dr = 0.005
     eq = ta.cfg_equations(2, dr, 2.0)
-    # training data: 4 seeded tanks, 12 samples over the first ~250 steps
-    tr = seeded_tanks(0, dr, n_tanks=4)
+    # training data: 2 seeded tanks, 12 samples over the first ~250 steps
+    tr = seeded_tanks(0, dr, n_tanks=2)
     ctx = ta.HipContext(eq)
     ctx.upload(tr)
     net = ta.LearnedStep(ctx, seed=0)
@@ -257,7 +257,9 @@ def learned_leg(args):
     print(f"bench: training on {sum(a.n_fluid for a in tr)} fluid particles x 12 samples",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    losses = net.train(args.train_iters, lr=1e-3)
+    # Adam, learning rate 2e-3 halved every quarter of the iterations
+    losses = np.concatenate([net.train(args.train_iters // 4, lr=2e-3 / 2 ** k)
+                             for k in range(4)])
     t_train = time.perf_counter() - t0
     w, sc = net.params(), net.scales()
     ctx.close()

@@ -138,6 +138,12 @@ tit_status tit_hip_compute_rhs(tit_hip_ctx* ctx, double* drho_dt,
 tit_status tit_hip_neighbors(tit_hip_ctx* ctx, uint64_t* off, uint32_t* idx,
                              size_t cap, size_t* total);
 
+/* Number of neighbours (self excluded) whose pair terms the fused stage
+ * kernel summed for each particle in the last right-hand side of the current
+ * state, reference order; 0 for fixed particles (test hook: must equal the
+ * adjacency sizes of tit_hip_neighbors minus one for fluid particles). */
+tit_status tit_hip_pair_counts(tit_hip_ctx* ctx, uint32_t* count);
+
 /* Kernel timing with HIP events on the context's stream.  While enabled,
  * steps run eagerly (no graph replay) with an event pair around every launch.
  * Mirrors TIT_PROFILE_SECTION (source/tit/core/profiler.hpp:42-44). */
@@ -174,9 +180,12 @@ tit_status tit_hip_set_schedule(tit_hip_ctx* ctx, const tit_hip_schedule* s);
 tit_status tit_hip_autotune(tit_hip_ctx* ctx, int steps_per_candidate,
                             tit_hip_schedule* best, double* best_ms_per_step);
 
-/* Number of particles (all tanks) and of neighbour-index rebuilds so far. */
+/* Number of particles (all tanks), of cell-index rebuilds run so far (the
+ * index is rebuilt only when a tank's particles could otherwise leave the
+ * skin, or when forced by a hook, a schedule change or an upload) and of
+ * plain steps taken since upload. */
 tit_status tit_hip_counts(tit_hip_ctx* ctx, size_t* n_particles,
-                          uint64_t* n_rebuilds);
+                          uint64_t* n_rebuilds, uint64_t* n_steps);
 
 const char* tit_hip_last_error(const tit_hip_ctx* ctx);
 void tit_hip_destroy(tit_hip_ctx* ctx);

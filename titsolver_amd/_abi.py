@@ -82,6 +82,7 @@ SIGNATURES = {
     "tit_hip_neighbors": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint32), C.c_size_t,
                                     C.POINTER(C.c_size_t)]),
+    "tit_hip_pair_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "tit_hip_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "tit_hip_kernel_stats": (C.c_int, [C.c_void_p, C.POINTER(KernelStat), C.c_int,
                                        C.POINTER(C.c_int)]),
@@ -89,7 +90,7 @@ SIGNATURES = {
     "tit_hip_set_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
     "tit_hip_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Schedule), _DP]),
     "tit_hip_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t),
-                                 C.POINTER(C.c_uint64)]),
+                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "tit_hip_last_error": (C.c_char_p, [C.c_void_p]),
     "tit_hip_destroy": (None, [C.c_void_p]),
     "tit_hip_learned_init": (C.c_int, [C.c_void_p, C.c_uint64]),

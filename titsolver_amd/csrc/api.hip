@@ -1,13 +1,16 @@
 // C ABI of the MI355X WCSPH step: context, buffers, step schedule.
 //
-// Buffer rotation of one SSPRK3 step (state in A at the step start):
-//   rebuild  keys(A) -> stable radix sort -> permute A into T, swap A and T
+// Buffers of one SSPRK3 step (state in A at the step start):
+//   begin    pending dt -> dt; per tank: rebuild if the cell index could
+//            stop covering every neighbour by the end of the step
+//   rebuild  (marked tanks only) keys(A) -> stable radix sort -> permute A
+//            into T and copy back -> cell offsets -> chunk list
 //   stage 1  boundary(A); B = A + dt L(A)
 //   stage 2  boundary(B); C = 3/4 A + 1/4 (B + dt L(B))
 //   stage 3  boundary(C); A = 1/3 A + 2/3 (C + dt L(C))  (+ next dt, in place:
 //            each particle reads and writes only its own slot of A)
-// Every rebuild swaps the A/T, key and id/tag pointers, so two steps bring
-// every pointer back to where it started: the hipGraph holds two steps.
+// No buffer changes roles, and the rebuild decision is taken on the device,
+// so one captured hipGraph (two steps) serves every step.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -202,50 +205,48 @@ double cell_size(const tit_hip_ctx* c) {
 
 // ---- step pieces ----------------------------------------------------------------
 
-void swap_state(DevState& a, DevState& b) {
-  DevState t = a;
-  a = b;
-  b = t;
-}
 
-template <class T>
-void swap_ptr(T*& a, T*& b) {
-  T* t = a;
-  a = b;
-  b = t;
-}
 
 
 StageArgs stage_args(tit_hip_ctx* c);
 
-void enqueue_rebuild(tit_hip_ctx* c) {
+RebuildCtl rebuild_ctl(tit_hip_ctx* c) {
+  return RebuildCtl{c->rebuild_ctl, c->tank_need, c->rebuild_count};
+}
+
+// Cell rebuild: keys -> stable radix sort -> permute into T and copy back ->
+// cell offsets -> chunk list.  Buffers keep their roles; on the device the
+// whole sequence runs only for the tanks marked before it (by step_begin,
+// or every tank with kRebuildForce, or every tank only when a forced
+// rebuild is pending with kRebuildPending).
+void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode) {
   hipStream_t s = c->stream;
+  const RebuildCtl rc = rebuild_ctl(c);
+  if (mode == kRebuildForce) launch_force_rebuild(s, c->n_tanks, rc);
+  if (mode == kRebuildPending) launch_pending_rebuild(s, c->n_tanks, rc);
   {
     ProfScope ps(c->prof, s, "cell_keys");
-    launch_cell_keys(s, c->dp, c->tanks, c->n, c->A, c->tag, c->key,
-                     c->counter + 1);
+    launch_cell_keys(s, c->dp, c->tanks, c->n, c->A, c->tag, c->key, c->counter + 1, rc);
   }
-  radix_sort_pairs(s, c->prof, c->n, c->key_bits, &c->key, &c->key_alt,
-                   &c->val, &c->val_alt, c->sort);
+  radix_sort_pairs(s, c->prof, c->n, c->key_bits, c->key, c->key_alt, c->val, c->val_alt,
+                   c->sort, rc.ctl);
   {
     ProfScope ps(c->prof, s, "permute");
-    launch_permute(s, c->dp, c->n, c->val, c->A, c->T, c->id, c->id_alt,
-                   c->tag, c->tag_alt, c->fixed_list, c->counter);
+    launch_permute(s, c->dp, c->n, c->val, c->A, c->T, c->id, c->id_alt, c->tag, c->tag_alt,
+                   c->fixed_list, c->counter, rc.ctl);
+    launch_restore_order(s, c->prm.dim, c->n, c->T, c->A, c->id_alt, c->id, c->tag_alt,
+                         c->tag, rc.ctl);
   }
-  swap_state(c->A, c->T);
-  swap_ptr(c->id, c->id_alt);
-  swap_ptr(c->tag, c->tag_alt);
   {
     ProfScope ps(c->prof, s, "cell_offsets");
-    launch_cell_offsets(s, c->n_cells, c->n, c->key, c->cell_off);
+    launch_cell_offsets(s, c->n_cells, c->n, c->key, c->cell_off, rc.ctl);
   }
   {
     ProfScope ps(c->prof, s, "build_chunks");
     StageArgs a = stage_args(c);
-    launch_build_chunks(s, a, c->block_tot);
+    launch_build_chunks(s, a, c->block_tot, rc.ctl);
   }
-  launch_reset_disp(s, c->n_tanks, c->disp);
-  c->rebuilds += 1;
+  launch_reset_disp(s, c->n_tanks, c->disp, rc);
 }
 
 void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
@@ -313,6 +314,13 @@ tit_status build_schedule(tit_hip_ctx* c) {
   if (!c->chunk_counts) TIT_HIP_TRY(c, dalloc(c, &c->chunk_counts, 4));
   stage_grids(c->dp, c->n_cu, &c->grid_lds, &c->grid_global);
   drop_graph(c);
+  if (c->rebuild_ctl) {
+    // the chunk list follows the schedule: recut it from the current cell
+    // offsets (gated on ctl[2], so nothing runs before the first rebuild)
+    StageArgs a = stage_args(c);
+    launch_build_chunks(c->stream, a, c->block_tot, c->rebuild_ctl + 2);
+    TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
   return TIT_OK;
 }
 
@@ -337,6 +345,7 @@ StageArgs stage_args(tit_hip_ctx* c) {
   a.dt_min = c->dt_min;
   a.vmax = c->vmax;
   a.rhs_drho = c->rhs_drho;
+  a.rhs_count = c->rhs_count;
   for (int k = 0; k < 3; ++k) a.rhs_dv[k] = c->rhs_dv[k];
   return a;
 }
@@ -355,8 +364,9 @@ void enqueue_stage(tit_hip_ctx* c, const DevState& in, const DevState& old,
 // One SSPRK step, SSPRKIntegrator::step (time_integrator.hpp:159-184).
 void enqueue_step(tit_hip_ctx* c) {
   hipStream_t s = c->stream;
-  launch_step_begin(s, c->n_tanks, c->dt_min, c->dt, c->vmax);
-  enqueue_rebuild(c);
+  launch_step_begin(s, c->n_tanks, c->dt_min, c->dt, c->vmax, c->disp, c->dp.skin_half,
+                    rebuild_ctl(c));
+  enqueue_rebuild(c, kRebuildMarked);
   enqueue_boundary(c, c->A);
   enqueue_stage(c, c->A, c->A, c->B, kEuler, 1.0, false);
   enqueue_boundary(c, c->B);
@@ -405,13 +415,12 @@ namespace tit_amd {
 tit_status run_plain(tit_hip_ctx* c, int n_steps) {
   int done = 0;
   if (!c->prof.enabled && n_steps >= 2) {
-    const int par = static_cast<int>(c->rebuilds & 1u);
+    const int par = 0;  // buffers keep their roles: one graph serves every step
     if (!c->graph[par]) {
       TIT_HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       enqueue_step(c);
       enqueue_step(c);
       hipError_t e = hipStreamEndCapture(c->stream, &c->graph_def[par]);
-      c->rebuilds -= 2;  // captured, not executed
       if (e != hipSuccess) {
         c->graph_def[par] = nullptr;
         return fail(c, TIT_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -420,10 +429,10 @@ tit_status run_plain(tit_hip_ctx* c, int n_steps) {
     }
     for (; done + 2 <= n_steps; done += 2) {
       TIT_HIP_TRY(c, hipGraphLaunch(c->graph[par], c->stream));
-      c->rebuilds += 2;
     }
   }
   for (; done < n_steps; ++done) enqueue_step(c);
+  c->steps += static_cast<uint64_t>(n_steps);
   tit_status st = check_launch(c, "step");
   if (st != TIT_OK) return st;
   if (c->prof.enabled) c->prof.collect();
@@ -619,6 +628,9 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->cell_off, static_cast<size_t>(c->n_cells) + 1));
   TIT_HIP_TRY(c, dalloc(c, &c->fixed_list, std::max<uint32_t>(c->n_fixed, 1)));
   TIT_HIP_TRY(c, dalloc(c, &c->counter, 4));
+  TIT_HIP_TRY(c, dalloc(c, &c->rebuild_ctl, 4));
+  TIT_HIP_TRY(c, dalloc(c, &c->tank_need, n_tanks));
+  TIT_HIP_TRY(c, dalloc(c, &c->rebuild_count, 1));
   c->sort.max_tiles = radix_sort_tiles(n);
   TIT_HIP_TRY(c, dalloc(c, &c->sort.hist, 256ull * std::max<uint32_t>(c->sort.max_tiles, 1)));
   TIT_HIP_TRY(c, dalloc(c, &c->sort.tot, 256));
@@ -634,6 +646,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->stage_rho, n));
   TIT_HIP_TRY(c, dalloc(c, &c->stage_p, n));
   TIT_HIP_TRY(c, dalloc(c, &c->rhs_drho, n));
+  TIT_HIP_TRY(c, dalloc(c, &c->rhs_count, n));
   for (int k = 0; k < dim; ++k) TIT_HIP_TRY(c, dalloc(c, &c->rhs_dv[k], n));
 
   hipStream_t s = c->stream;
@@ -663,6 +676,11 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
                                 hipMemcpyHostToDevice, s));
   TIT_HIP_TRY(c, hipMemcpyAsync(c->counter, z32.data(), 4 * sizeof(uint32_t),
                                 hipMemcpyHostToDevice, s));
+  // the first step rebuilds every tank
+  const uint32_t ctl0[4] = {0u, 1u, 0u, 0u};
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->rebuild_ctl, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemsetAsync(c->tank_need, 0, n_tanks * sizeof(uint32_t), s));
+  TIT_HIP_TRY(c, hipMemsetAsync(c->rebuild_count, 0, sizeof(unsigned long long), s));
   // First step size from the uploaded state and dv/dt (zero by default).
   double* d_dv = nullptr;
   if (any_dv) {
@@ -686,7 +704,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   st = build_schedule(c);
   if (st != TIT_OK) return st;
   c->uploaded = true;
-  c->rebuilds = 0;
+  c->steps = 0;
   return TIT_OK;
 }
 
@@ -751,7 +769,7 @@ tit_status tit_hip_prepare(tit_hip_ctx* c) {
   if (!c) return TIT_E_ARG;
   if (!c->uploaded) return fail(c, TIT_E_STATE, "prepare before upload");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
-  enqueue_rebuild(c);
+  enqueue_rebuild(c, kRebuildPending);
   enqueue_boundary(c, c->A);
   tit_status st = check_launch(c, "prepare");
   if (st != TIT_OK) return st;
@@ -763,7 +781,7 @@ tit_status tit_hip_compute_rhs(tit_hip_ctx* c, double* drho_dt, double* dv_dt) {
   if (!c) return TIT_E_ARG;
   if (!c->uploaded) return fail(c, TIT_E_STATE, "compute_rhs before upload");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
-  enqueue_rebuild(c);
+  enqueue_rebuild(c, kRebuildPending);
   enqueue_stage(c, c->A, c->A, c->A, kRhsOnly, 1.0, false);
   // reuse the staging buffers: drho -> stage_rho, dv -> stage_v
   launch_export_rhs(c->stream, c->prm.dim, c->n, c->id, c->rhs_drho, c->rhs_dv,
@@ -778,12 +796,28 @@ tit_status tit_hip_compute_rhs(tit_hip_ctx* c, double* drho_dt, double* dv_dt) {
   return read_flags(c);
 }
 
+tit_status tit_hip_pair_counts(tit_hip_ctx* c, uint32_t* count) {
+  if (!c || !count) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "pair_counts before upload");
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  enqueue_rebuild(c, kRebuildPending);
+  enqueue_stage(c, c->A, c->A, c->A, kRhsOnly, 1.0, false);
+  uint32_t* out = reinterpret_cast<uint32_t*>(c->stage_rho);  // n doubles >= n u32
+  launch_export_u32(c->stream, c->n, c->id, c->rhs_count, out);
+  tit_status st = check_launch(c, "pair_counts");
+  if (st != TIT_OK) return st;
+  TIT_HIP_TRY(c, hipMemcpyAsync(count, out, c->n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                c->stream));
+  TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return read_flags(c);
+}
+
 tit_status tit_hip_neighbors(tit_hip_ctx* c, uint64_t* off, uint32_t* idx,
                              size_t cap, size_t* total) {
   if (!c || !off) return TIT_E_ARG;
   if (!c->uploaded) return fail(c, TIT_E_STATE, "neighbors before upload");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
-  enqueue_rebuild(c);
+  enqueue_rebuild(c, kRebuildPending);
   const size_t n = c->n;
   uint64_t* d_cnt = nullptr;
   TIT_HIP_TRY(c, hipMalloc(&d_cnt, (n + 1) * sizeof(uint64_t)));
@@ -932,10 +966,21 @@ tit_status tit_hip_autotune(tit_hip_ctx* c, int steps_per_candidate,
   return TIT_OK;
 }
 
-tit_status tit_hip_counts(tit_hip_ctx* c, size_t* n_particles, uint64_t* n_rebuilds) {
+tit_status tit_hip_counts(tit_hip_ctx* c, size_t* n_particles, uint64_t* n_rebuilds,
+                          uint64_t* n_steps) {
   if (!c) return TIT_E_ARG;
   if (n_particles) *n_particles = c->n;
-  if (n_rebuilds) *n_rebuilds = c->rebuilds;
+  if (n_steps) *n_steps = c->steps;
+  if (n_rebuilds) {
+    *n_rebuilds = 0;
+    if (c->uploaded) {
+      unsigned long long r = 0;
+      TIT_HIP_TRY(c, hipSetDevice(c->device));
+      TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+      TIT_HIP_TRY(c, hipMemcpy(&r, c->rebuild_count, sizeof(r), hipMemcpyDeviceToHost));
+      *n_rebuilds = r;
+    }
+  }
   return TIT_OK;
 }
 

@@ -34,7 +34,7 @@ struct tit_hip_ctx {
   uint32_t n_cells = 0;
   int key_bits = 1;
   bool uploaded = false;
-  uint64_t rebuilds = 0;
+  uint64_t steps = 0;  // plain steps taken since upload
 
   // device memory
   std::vector<void*> allocs;
@@ -44,6 +44,9 @@ struct tit_hip_ctx {
   uint32_t* cell_off = nullptr;
   uint32_t* fixed_list = nullptr;
   uint32_t* counter = nullptr;  // [0] fixed count, [1] flags
+  uint32_t* rebuild_ctl = nullptr;       // RebuildCtl::ctl
+  uint32_t* tank_need = nullptr;         // RebuildCtl::need
+  unsigned long long* rebuild_count = nullptr;
   SortScratch sort;
   DevTank* tanks = nullptr;
   double* normal = nullptr;
@@ -53,6 +56,7 @@ struct tit_hip_ctx {
   double *stage_r = nullptr, *stage_v = nullptr, *stage_rho = nullptr,
          *stage_p = nullptr;
   double* rhs_drho = nullptr;
+  uint32_t* rhs_count = nullptr;  // pair counts of the last right-hand side
   double* rhs_dv[3] = {nullptr, nullptr, nullptr};
 
   // chunk schedule of the stage kernel
@@ -103,8 +107,13 @@ hipError_t dalloc(tit_hip_ctx* c, T** p, size_t count) {
 
 void forget(tit_hip_ctx* c, void* q);
 void drop_graph(tit_hip_ctx* c);
-void swap_state(DevState& a, DevState& b);
-void enqueue_rebuild(tit_hip_ctx* c);
+RebuildCtl rebuild_ctl(tit_hip_ctx* c);
+enum RebuildMode {
+  kRebuildMarked,   // the tanks the last step_begin marked (nothing if none)
+  kRebuildPending,  // every tank if a forced rebuild is pending, else none
+  kRebuildForce     // every tank
+};
+void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode);
 void enqueue_boundary(tit_hip_ctx* c, const DevState& st);
 void enqueue_step(tit_hip_ctx* c);
 tit_status check_launch(tit_hip_ctx* c, const char* what);

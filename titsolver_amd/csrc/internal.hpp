@@ -67,12 +67,12 @@ struct SortScratch {
 };
 uint32_t radix_sort_tiles(uint32_t n);
 // Stable LSD sort of (keys, values) by the low `key_bits` bits of the keys.
-// The first pass takes values = index (vals_in is not read).  On return the
-// sorted pairs are in *keys / *vals (pointers may be swapped with the alt
-// buffers).
+// The first pass takes values = index (vals is not read).  On return the
+// sorted pairs are in keys / vals (the alt arrays are scratch).  Nothing runs
+// on the device while ctl[0] is 0.
 void radix_sort_pairs(hipStream_t stream, Profiler& prof, uint32_t n,
-                      int key_bits, uint32_t** keys, uint32_t** keys_alt,
-                      uint32_t** vals, uint32_t** vals_alt,
-                      const SortScratch& scratch);
+                      int key_bits, uint32_t* keys, uint32_t* keys_alt,
+                      uint32_t* vals, uint32_t* vals_alt,
+                      const SortScratch& scratch, const uint32_t* ctl);
 
 }  // namespace tit_amd

@@ -15,12 +15,55 @@ namespace {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 
-constexpr int kBM = 64, kBN = 64, kBK = 16;
+constexpr int kBM = 64, kBN = 64, kBK = 64;
+constexpr int kLdsRow = kBK + 8;  // padded LDS row (144 B): conflict-free 16 B reads
 
+// Eight consecutive BF16 of src starting at element `at`, `valid` of them
+// in range (the rest zero): one 16-byte load when all are valid and aligned.
+__device__ __forceinline__ uint4 load8(const bf16* src, size_t at, int valid) {
+  if (valid >= 8 && (reinterpret_cast<uintptr_t>(src + at) & 15u) == 0) {
+    return *reinterpret_cast<const uint4*>(src + at);
+  }
+  union {
+    uint4 u;
+    bf16 h[8];
+  } r;
+  r.u = make_uint4(0u, 0u, 0u, 0u);
+  for (int t = 0; t < 8 && t < valid; ++t) r.h[t] = src[at + t];
+  return r.u;
+}
+
+// Stages a 64 x 64 operand tile as tile[row][k] (row = m for A, n for B).
+// `major_k`: the source rows run along k (A row-major, or B transposed);
+// otherwise they run along m / n and the tile is transposed into LDS.
+__device__ __forceinline__ void stage_tile(bf16 (*tile)[kLdsRow], const bf16* src, int ld,
+                                           bool major_k, int r0, int rows, int k0, int kend,
+                                           int tid) {
+  for (int u = tid; u < kBM * kBK / 8; u += 256) {
+    if (major_k) {
+      const int row = u >> 3, kq = (u & 7) * 8;
+      const int r = r0 + row, k = k0 + kq;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (r < rows && k < kend) v = load8(src, static_cast<size_t>(r) * ld + k, kend - k);
+      *reinterpret_cast<uint4*>(&tile[row][kq]) = v;
+    } else {
+      const int krow = u >> 3, rq = (u & 7) * 8;
+      const int k = k0 + krow, r = r0 + rq;
+      union {
+        uint4 u;
+        bf16 h[8];
+      } v;
+      v.u = make_uint4(0u, 0u, 0u, 0u);
+      if (k < kend && r < rows) v.u = load8(src, static_cast<size_t>(k) * ld + r, rows - r);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) tile[rq + t][krow] = v.h[t];
+    }
+  }
+}
 
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) bf16 sA[kBM][kBK];
-  __shared__ __attribute__((aligned(16))) bf16 sB[kBN][kBK];
+  __shared__ __attribute__((aligned(16))) bf16 sA[kBM][kLdsRow];
+  __shared__ __attribute__((aligned(16))) bf16 sB[kBN][kLdsRow];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int m0 = blockIdx.x * kBM, n0 = blockIdx.y * kBN;
@@ -31,40 +74,16 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     kend = min(g.K, kbeg + chunk);
   }
   f32x16 acc = {};
-  const bf16 zero = static_cast<bf16>(0.0f);
   for (int k0 = kbeg; k0 < kend; k0 += kBK) {
-    {  // A tile: 64 rows x 16 k, 4 elements per thread
-      const int r = tid >> 2, kk = (tid & 3) * 4;
-      const int m = m0 + r;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int k = k0 + kk + q;
-        bf16 v = zero;
-        if (m < g.M && k < kend) {
-          v = g.transA ? g.A[static_cast<size_t>(k) * g.lda + m]
-                       : g.A[static_cast<size_t>(m) * g.lda + k];
-        }
-        sA[r][kk + q] = v;
-      }
-    }
-    {  // B tile: 16 k x 64 cols, stored column-major
-      const int c = tid >> 2, kk = (tid & 3) * 4;
-      const int n = n0 + c;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int k = k0 + kk + q;
-        bf16 v = zero;
-        if (n < g.N && k < kend) {
-          v = g.transB ? g.B[static_cast<size_t>(n) * g.ldb + k]
-                       : g.B[static_cast<size_t>(k) * g.ldb + n];
-        }
-        sB[c][kk + q] = v;
-      }
-    }
+    stage_tile(sA, g.A, g.lda, !g.transA, m0, g.M, k0, kend, tid);
+    stage_tile(sB, g.B, g.ldb, g.transB, n0, g.N, k0, kend, tid);
     __syncthreads();
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[wm * 32 + (lane & 31)][8 * (lane >> 5)]);
-    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[wn * 32 + (lane & 31)][8 * (lane >> 5)]);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 16) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[wm * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[wn * 32 + (lane & 31)][kk + 8 * (lane >> 5)]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
     __syncthreads();
   }
   const int n = n0 + wn * 32 + (lane & 31);

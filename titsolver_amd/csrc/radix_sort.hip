@@ -1,5 +1,9 @@
 // Stable LSD radix sort of (cell key, particle slot) pairs, 8-bit digits.
 //
+// The sorted pairs end in the input arrays (an odd pass count copies them
+// back), so no buffer changes roles and a captured step can hold the sort
+// whether or not it runs (every kernel returns at once while ctl[0] is 0).
+//
 // Each pass is three launches:
 //   hist    one workgroup per 4096-key tile counts its digits in LDS;
 //   scan    one workgroup per digit turns the digit-major tile counts into
@@ -40,7 +44,8 @@ __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
 
 __global__ __launch_bounds__(kThreads) void k_radix_hist(
     const uint32_t* __restrict__ keys, uint32_t n, int shift,
-    uint32_t* __restrict__ hist, uint32_t ntiles) {
+    uint32_t* __restrict__ hist, uint32_t ntiles, const uint32_t* __restrict__ ctl) {
+  if (!ctl[0]) return;
   __shared__ uint32_t h[256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   h[tid] = 0;
@@ -76,7 +81,9 @@ __device__ __forceinline__ uint32_t block_inclusive_scan256(uint32_t v,
 }
 
 __global__ __launch_bounds__(256) void k_radix_digit_scan(
-    uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t* __restrict__ tot) {
+    uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t* __restrict__ tot,
+    const uint32_t* __restrict__ ctl) {
+  if (!ctl[0]) return;
   __shared__ uint32_t sh[256];
   uint32_t* row = hist + static_cast<size_t>(blockIdx.x) * ntiles;
   const int tid = threadIdx.x;
@@ -98,7 +105,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
     uint32_t n, int shift, const uint32_t* __restrict__ hist,
-    const uint32_t* __restrict__ tot, uint32_t ntiles) {
+    const uint32_t* __restrict__ tot, uint32_t ntiles, const uint32_t* __restrict__ ctl) {
+  if (!ctl[0]) return;
   __shared__ uint32_t wcnt[kWaves][256];
   __shared__ uint32_t sh[256];
   __shared__ uint32_t tile_base[256];
@@ -151,6 +159,17 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(
   }
 }
 
+__global__ __launch_bounds__(kThreads) void k_copy_pairs(
+    uint32_t n, const uint32_t* __restrict__ ka, const uint32_t* __restrict__ va,
+    uint32_t* __restrict__ kb, uint32_t* __restrict__ vb, const uint32_t* __restrict__ ctl) {
+  if (!ctl[0]) return;
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) {
+    kb[i] = ka[i];
+    vb[i] = va[i];
+  }
+}
+
 }  // namespace
 
 uint32_t radix_sort_tiles(uint32_t n) {
@@ -158,37 +177,41 @@ uint32_t radix_sort_tiles(uint32_t n) {
 }
 
 void radix_sort_pairs(hipStream_t stream, Profiler& prof, uint32_t n,
-                      int key_bits, uint32_t** keys, uint32_t** keys_alt,
-                      uint32_t** vals, uint32_t** vals_alt,
-                      const SortScratch& scratch) {
+                      int key_bits, uint32_t* keys, uint32_t* keys_alt,
+                      uint32_t* vals, uint32_t* vals_alt,
+                      const SortScratch& scratch, const uint32_t* ctl) {
   if (n == 0) return;
   const uint32_t ntiles = radix_sort_tiles(n);
   int passes = (key_bits + 7) / 8;
   if (passes < 1) passes = 1;
+  uint32_t *kin = keys, *kout = keys_alt, *vin = vals, *vout = vals_alt;
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
     {
       ProfScope ps(prof, stream, "radix_hist");
-      k_radix_hist<<<ntiles, kThreads, 0, stream>>>(*keys, n, shift,
-                                                     scratch.hist, ntiles);
+      k_radix_hist<<<ntiles, kThreads, 0, stream>>>(kin, n, shift, scratch.hist, ntiles, ctl);
     }
     {
       ProfScope ps(prof, stream, "radix_digit_scan");
-      k_radix_digit_scan<<<256, 256, 0, stream>>>(scratch.hist, ntiles,
-                                                  scratch.tot);
+      k_radix_digit_scan<<<256, 256, 0, stream>>>(scratch.hist, ntiles, scratch.tot, ctl);
     }
     {
       ProfScope ps(prof, stream, "radix_scatter");
-      k_radix_scatter<<<ntiles, kThreads, 0, stream>>>(
-          *keys, p == 0 ? nullptr : *vals, *keys_alt, *vals_alt, n, shift,
-          scratch.hist, scratch.tot, ntiles);
+      k_radix_scatter<<<ntiles, kThreads, 0, stream>>>(kin, p == 0 ? nullptr : vin, kout, vout,
+                                                        n, shift, scratch.hist, scratch.tot,
+                                                        ntiles, ctl);
     }
-    uint32_t* t = *keys;
-    *keys = *keys_alt;
-    *keys_alt = t;
-    t = *vals;
-    *vals = *vals_alt;
-    *vals_alt = t;
+    uint32_t* t = kin;
+    kin = kout;
+    kout = t;
+    t = vin;
+    vin = vout;
+    vout = t;
+  }
+  if (passes & 1) {  // the result is in the alternate arrays: copy it back
+    ProfScope ps(prof, stream, "radix_copy");
+    k_copy_pairs<<<(n + kThreads - 1) / kThreads, kThreads, 0, stream>>>(n, kin, vin, keys,
+                                                                        vals, ctl);
   }
 }
 

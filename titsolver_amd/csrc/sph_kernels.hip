@@ -118,10 +118,15 @@ template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_cell_keys(
     DevParams P, const DevTank* __restrict__ tanks, uint32_t n, DevState st,
     const uint32_t* __restrict__ tag, uint32_t* __restrict__ key,
-    uint32_t* __restrict__ flags) {
+    uint32_t* __restrict__ flags, const uint32_t* __restrict__ ctl,
+    const uint32_t* __restrict__ need) {
+  if (!ctl[0]) return;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i == 0) flags[-1] = 0u;  // fixed-particle counter, filled by k_permute
   if (i >= n) return;
-  const DevTank& T = tanks[tag[i] >> 1];
+  const uint32_t t = tag[i] >> 1;
+  if (!need[t]) return;  // the tank keeps its keys, hence its order
+  const DevTank& T = tanks[t];
   uint32_t flat = 0;
   bool bad = false;
 #pragma unroll
@@ -143,7 +148,6 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(
   }
   key[i] = T.cell_off + flat;
   if (bad) atomicOr(flags, 2u);
-  if (i == 0) flags[-1] = 0u;  // fixed-particle counter, filled by k_permute
 }
 
 // Gathers the state into cell order and lists the fixed particles (the list
@@ -153,7 +157,9 @@ __global__ __launch_bounds__(kBlock) void k_permute(
     DevParams P, uint32_t n, const uint32_t* __restrict__ perm, DevState src, DevState dst,
     const uint32_t* __restrict__ id_src, uint32_t* __restrict__ id_dst,
     const uint32_t* __restrict__ tag_src, uint32_t* __restrict__ tag_dst,
-    uint32_t* __restrict__ fixed_list, uint32_t* __restrict__ fixed_count) {
+    uint32_t* __restrict__ fixed_list, uint32_t* __restrict__ fixed_count,
+    const uint32_t* __restrict__ ctl) {
+  if (!ctl[0]) return;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const bool in = i < n;
   const uint32_t tg = in ? tag_src[perm[i]] : 0u;
@@ -180,22 +186,37 @@ __global__ __launch_bounds__(kBlock) void k_permute(
   tag_dst[i] = tg;
 }
 
-// CSR offsets: off[c] = first sorted position with key >= c.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_restore_order(
+    uint32_t n, DevState src, DevState dst, const uint32_t* __restrict__ id_src,
+    uint32_t* __restrict__ id_dst, const uint32_t* __restrict__ tag_src,
+    uint32_t* __restrict__ tag_dst, const uint32_t* __restrict__ ctl) {
+  if (!ctl[0]) return;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    dst.x[k][i] = src.x[k][i];
+    dst.v[k][i] = src.v[k][i];
+  }
+  dst.rho[i] = src.rho[i];
+  dst.aux[i] = src.aux[i];
+  id_dst[i] = id_src[i];
+  tag_dst[i] = tag_src[i];
+}
+
+// CSR offsets: off[c] = first sorted position with key >= c.  Thread i
+// (0..n) fills the cells c in (key[i-1], key[i]] with i (key[-1] = -1,
+// key[n] = n_cells), so every cell is written exactly once.
 __global__ __launch_bounds__(kBlock) void k_cell_offsets(
     uint32_t n_cells, uint32_t n, const uint32_t* __restrict__ key,
-    uint32_t* __restrict__ off) {
-  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-  if (c > n_cells) return;
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (key[mid] < c) {
-      lo = mid + 1;
-    } else {
-      hi = mid;
-    }
-  }
-  off[c] = lo;
+    uint32_t* __restrict__ off, const uint32_t* __restrict__ ctl) {
+  if (!ctl[0]) return;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i > n) return;
+  const int64_t prev = i > 0 ? static_cast<int64_t>(key[i - 1]) : -1;
+  const int64_t cur = i < n ? static_cast<int64_t>(key[i]) : static_cast<int64_t>(n_cells);
+  for (int64_t c = prev + 1; c <= cur; ++c) off[c] = i;
 }
 
 // ---- boundary extrapolation ---------------------------------------------------
@@ -350,15 +371,38 @@ __global__ __launch_bounds__(kBlock) void k_time_step(
   wave_atomic_min_u64(dt_min, tg >> 1, dbl_bits(dt), fluid);
 }
 
-__global__ void k_step_begin(int n_tanks, unsigned long long* dt_min,
-                             double* dt, uint32_t* vmax) {
+__global__ void k_rebuild_clear(uint32_t* ctl) { ctl[0] = 0u; }
+
+__global__ void k_step_begin(int n_tanks, unsigned long long* dt_min, double* dt,
+                             uint32_t* vmax, const double* disp, double skin_half,
+                             uint32_t* ctl, uint32_t* need) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tanks) return;
   const unsigned long long b = dt_min[t];
   // A tank without fluid keeps dt = 0 (nothing moves).
-  dt[t] = b == ~0ull ? 0.0 : __longlong_as_double(static_cast<long long>(b));
+  const double h = b == ~0ull ? 0.0 : __longlong_as_double(static_cast<long long>(b));
+  dt[t] = h;
   dt_min[t] = ~0ull;
+  const double vprev = static_cast<double>(__uint_as_float(vmax[t]));
   vmax[t] = 0u;
+  const bool nd = ctl[1] != 0u || !(disp[t] + 2.0 * h * vprev <= 0.9 * skin_half);
+  need[t] = nd ? 1u : 0u;
+  if (nd) atomicOr(ctl, 1u);
+}
+
+__global__ void k_force_rebuild(int n_tanks, uint32_t* ctl, uint32_t* need) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) ctl[0] = 1u;
+  if (t < n_tanks) need[t] = 1u;
+}
+
+// Rebuild only if a forced one is pending (the current index is otherwise
+// valid: every particle is within half the skin of its cell).
+__global__ void k_pending_rebuild(int n_tanks, uint32_t* ctl, uint32_t* need) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t f = ctl[1];
+  if (t < n_tanks) need[t] = f;
+  if (t == 0) ctl[0] = f;
 }
 
 __global__ void k_step_end(int n_tanks, const double* dt, const uint32_t* vmax,
@@ -372,9 +416,16 @@ __global__ void k_step_end(int n_tanks, const double* dt, const uint32_t* vmax,
   if (!(d <= skin_half)) atomicOr(flags, 1u);
 }
 
-__global__ void k_reset_disp(int n_tanks, double* disp) {
+__global__ void k_reset_disp(int n_tanks, double* disp, uint32_t* ctl,
+                             const uint32_t* need, unsigned long long* count) {
+  if (!ctl[0]) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n_tanks) disp[t] = 0.0;
+  if (t == 0) {
+    ctl[1] = 0u;  // no forced rebuild pending
+    ctl[2] = 1u;  // the cell index is valid
+    *count += 1ull;
+  }
+  if (t < n_tanks && need[t]) disp[t] = 0.0;
 }
 
 // ---- export ---------------------------------------------------------------------
@@ -478,35 +529,51 @@ void launch_import(hipStream_t s, int dim, uint32_t n, const double* r,
 
 void launch_cell_keys(hipStream_t s, const DevParams& P, const DevTank* tanks,
                       uint32_t n, DevState st, const uint32_t* tag,
-                      uint32_t* key, uint32_t* flags) {
+                      uint32_t* key, uint32_t* flags, RebuildCtl rc) {
   if (n == 0) return;
   if (P.dim == 2) {
-    k_cell_keys<2><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, tag, key, flags);
+    k_cell_keys<2><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, tag, key, flags, rc.ctl,
+                                                  rc.need);
   } else {
-    k_cell_keys<3><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, tag, key, flags);
+    k_cell_keys<3><<<grid_for(n), kBlock, 0, s>>>(P, tanks, n, st, tag, key, flags, rc.ctl,
+                                                  rc.need);
   }
 }
 
 void launch_permute(hipStream_t s, const DevParams& P, uint32_t n, const uint32_t* perm,
                     DevState src, DevState dst, const uint32_t* id_src,
                     uint32_t* id_dst, const uint32_t* tag_src,
-                    uint32_t* tag_dst, uint32_t* fixed_list, uint32_t* fixed_count) {
+                    uint32_t* tag_dst, uint32_t* fixed_list, uint32_t* fixed_count,
+                    const uint32_t* ctl) {
   if (n == 0) return;
   if (P.dim == 2) {
     k_permute<2><<<grid_for(n), kBlock, 0, s>>>(P, n, perm, src, dst, id_src,
                                                 id_dst, tag_src, tag_dst, fixed_list,
-                                                fixed_count);
+                                                fixed_count, ctl);
   } else {
     k_permute<3><<<grid_for(n), kBlock, 0, s>>>(P, n, perm, src, dst, id_src,
                                                 id_dst, tag_src, tag_dst, fixed_list,
-                                                fixed_count);
+                                                fixed_count, ctl);
+  }
+}
+
+void launch_restore_order(hipStream_t s, int dim, uint32_t n, DevState src, DevState dst,
+                          const uint32_t* id_src, uint32_t* id_dst,
+                          const uint32_t* tag_src, uint32_t* tag_dst,
+                          const uint32_t* ctl) {
+  if (n == 0) return;
+  if (dim == 2) {
+    k_restore_order<2><<<grid_for(n), kBlock, 0, s>>>(n, src, dst, id_src, id_dst, tag_src,
+                                                      tag_dst, ctl);
+  } else {
+    k_restore_order<3><<<grid_for(n), kBlock, 0, s>>>(n, src, dst, id_src, id_dst, tag_src,
+                                                      tag_dst, ctl);
   }
 }
 
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
-                         const uint32_t* key, uint32_t* cell_off) {
-  k_cell_offsets<<<grid_for(n_cells + 1), kBlock, 0, s>>>(n_cells, n, key,
-                                                          cell_off);
+                         const uint32_t* key, uint32_t* cell_off, const uint32_t* ctl) {
+  k_cell_offsets<<<grid_for(n + 1), kBlock, 0, s>>>(n_cells, n, key, cell_off, ctl);
 }
 
 void launch_boundary(hipStream_t s, const BoundaryArgs& a) {
@@ -539,8 +606,23 @@ void launch_time_step(hipStream_t s, const DevParams& P, uint32_t n,
 }
 
 void launch_step_begin(hipStream_t s, int n_tanks, unsigned long long* dt_min,
-                       double* dt, uint32_t* vmax) {
-  k_step_begin<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, dt_min, dt, vmax);
+                       double* dt, uint32_t* vmax, const double* disp,
+                       double skin_half, RebuildCtl rc) {
+  k_rebuild_clear<<<1, 1, 0, s>>>(rc.ctl);
+  k_step_begin<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, dt_min, dt, vmax, disp,
+                                                     skin_half, rc.ctl, rc.need);
+}
+
+void launch_force_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc) {
+  k_force_rebuild<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, rc.ctl, rc.need);
+}
+
+__global__ void k_mark_pending(uint32_t* ctl) { ctl[1] = 1u; }
+
+void launch_mark_pending(hipStream_t s, RebuildCtl rc) { k_mark_pending<<<1, 1, 0, s>>>(rc.ctl); }
+
+void launch_pending_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc) {
+  k_pending_rebuild<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, rc.ctl, rc.need);
 }
 
 void launch_step_end(hipStream_t s, int n_tanks, const double* dt,
@@ -550,8 +632,9 @@ void launch_step_end(hipStream_t s, int n_tanks, const double* dt,
                                                    skin_half, flags);
 }
 
-void launch_reset_disp(hipStream_t s, int n_tanks, double* disp) {
-  k_reset_disp<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, disp);
+void launch_reset_disp(hipStream_t s, int n_tanks, double* disp, RebuildCtl rc) {
+  k_reset_disp<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, disp, rc.ctl, rc.need,
+                                                     rc.count);
 }
 
 void launch_export(hipStream_t s, const DevParams& P, uint32_t n, DevState st,
@@ -563,6 +646,17 @@ void launch_export(hipStream_t s, const DevParams& P, uint32_t n, DevState st,
   } else {
     k_export<3><<<grid_for(n), kBlock, 0, s>>>(P, n, st, id, r, v, rho, p);
   }
+}
+
+__global__ void k_export_u32(uint32_t n, const uint32_t* __restrict__ id,
+                             const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[id[i]] = src[i];
+}
+
+void launch_export_u32(hipStream_t s, uint32_t n, const uint32_t* id,
+                       const uint32_t* src, uint32_t* dst) {
+  if (n) k_export_u32<<<grid_for(n), kBlock, 0, s>>>(n, id, src, dst);
 }
 
 void launch_export_rhs(hipStream_t s, int dim, uint32_t n, const uint32_t* id,

@@ -25,6 +25,7 @@ struct StageArgs {
   // outputs of the right-hand-side-only form (sorted order)
   double* rhs_drho;
   double* rhs_dv[3];
+  uint32_t* rhs_count;       // accepted neighbours per target (may be null)
   // reductions
   unsigned long long* dt_min;  // per tank, FP64 bits
   uint32_t* vmax;              // per tank, FP32 bits
@@ -54,6 +55,18 @@ struct BoundaryArgs {
   uint32_t n_fixed;
 };
 
+// Device-side control of the cell rebuild.  ctl[0]: the rebuild of this step
+// runs (some tank needs it); ctl[1]: a rebuild of every tank is pending
+// (after upload); ctl[2]: the cell index is valid.  need[t]: tank t is
+// re-sorted (other tanks keep their keys, hence their order).  count: rebuilds
+// run so far.  Every rebuild kernel reads ctl[0] and returns at once when it
+// is clear, so a captured step holds the rebuild unconditionally.
+struct RebuildCtl {
+  uint32_t* ctl;
+  uint32_t* need;
+  unsigned long long* count;
+};
+
 // Kernel kinds of the stage launch.
 enum StageKind { kRhsOnly = 0, kEuler = 1, kCombine = 2 };
 
@@ -62,32 +75,54 @@ void launch_import(hipStream_t s, int dim, uint32_t n, const double* r,
                    uint32_t* id);
 void launch_cell_keys(hipStream_t s, const DevParams& P, const DevTank* tanks,
                       uint32_t n, DevState st, const uint32_t* tag,
-                      uint32_t* key, uint32_t* flags);
+                      uint32_t* key, uint32_t* flags, RebuildCtl rc);
 void launch_permute(hipStream_t s, const DevParams& P, uint32_t n, const uint32_t* perm,
                     DevState src, DevState dst, const uint32_t* id_src,
                     uint32_t* id_dst, const uint32_t* tag_src,
-                    uint32_t* tag_dst, uint32_t* fixed_list, uint32_t* fixed_count);
+                    uint32_t* tag_dst, uint32_t* fixed_list, uint32_t* fixed_count,
+                    const uint32_t* ctl);
+// Copies the permuted state (src) back into the working buffers (dst).
+void launch_restore_order(hipStream_t s, int dim, uint32_t n, DevState src, DevState dst,
+                          const uint32_t* id_src, uint32_t* id_dst,
+                          const uint32_t* tag_src, uint32_t* tag_dst,
+                          const uint32_t* ctl);
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
-                         const uint32_t* key, uint32_t* cell_off);
+                         const uint32_t* key, uint32_t* cell_off, const uint32_t* ctl);
 void launch_boundary(hipStream_t s, const BoundaryArgs& a);
 void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final);
 uint32_t stage_lds_capacity(int dim);
 void stage_grids(const DevParams& P, int n_cu, uint32_t* grid_lds,
                  uint32_t* grid_global);
 // Rebuilds the chunk list from the cell offsets (after a cell rebuild).
-void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot);
+void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot,
+                         const uint32_t* ctl);
 void launch_time_step(hipStream_t s, const DevParams& P, uint32_t n,
                       DevState st, const uint32_t* tag, const uint32_t* id,
                       const double* dv_ref, unsigned long long* dt_min);
+// Takes each tank's pending step size and decides its rebuild: needed when
+// forced or when disp + 2 dt vmax (vmax of the previous step) could pass 0.9
+// of half the skin by the end of the step.
 void launch_step_begin(hipStream_t s, int n_tanks, unsigned long long* dt_min,
-                       double* dt, uint32_t* vmax);
+                       double* dt, uint32_t* vmax, const double* disp,
+                       double skin_half, RebuildCtl rc);
+// Marks every tank for the next rebuild sequence.
+void launch_force_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc);
+// Makes a forced rebuild pending (after moves the displacement bound does not
+// cover, such as a learned step).
+void launch_mark_pending(hipStream_t s, RebuildCtl rc);
+// Marks every tank if a forced rebuild is pending, none otherwise.
+void launch_pending_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc);
 void launch_step_end(hipStream_t s, int n_tanks, const double* dt,
                      const uint32_t* vmax, double* time, double* disp,
                      double skin_half, uint32_t* flags);
-void launch_reset_disp(hipStream_t s, int n_tanks, double* disp);
+// Ends a rebuild sequence: zero displacement of the re-sorted tanks, count.
+void launch_reset_disp(hipStream_t s, int n_tanks, double* disp, RebuildCtl rc);
 void launch_export(hipStream_t s, const DevParams& P, uint32_t n, DevState st,
                    const uint32_t* id, double* r, double* v, double* rho,
                    double* p);
+// dst[id[i]] = src[i].
+void launch_export_u32(hipStream_t s, uint32_t n, const uint32_t* id,
+                       const uint32_t* src, uint32_t* dst);
 void launch_export_rhs(hipStream_t s, int dim, uint32_t n, const uint32_t* id,
                        const double* drho, double* const* dv, double* drho_ref,
                        double* dv_ref);

@@ -37,11 +37,13 @@ struct LdsPos {
   const uint32_t* lb;                    // LDS index of each column's row zlo
   int zlo, nz, ix, ncx;
 
-  __device__ __forceinline__ bool range(int k, int zc, uint32_t& b,
+  // Candidates of column k in rows [w0, w1] (inside the target's 5-row window).
+  __device__ __forceinline__ bool range(int k, int w0, int w1, uint32_t& b,
                                         uint32_t& e) const {
     const int ixk = ix - kStencil + k;
     if (ixk < 0 || ixk >= ncx) return false;
-    const int z0 = max(zc - kStencil, 0), z1 = min(zc + kStencil, nz - 1);
+    const int z0 = max(w0, 0), z1 = min(w1, nz - 1);
+    if (z0 > z1) return false;
     b = coff[k][z0 - zlo];
     e = coff[k][z1 + 1 - zlo];
     return true;
@@ -61,7 +63,7 @@ struct GlobalPos {
   const uint32_t* coff;
   int cx, cy;  // target column
 
-  __device__ __forceinline__ bool range(int k, int zc, uint32_t& b,
+  __device__ __forceinline__ bool range(int k, int w0, int w1, uint32_t& b,
                                         uint32_t& e) const {
     constexpr int L = DIM - 1;
     int ix, iy = 0;
@@ -74,7 +76,8 @@ struct GlobalPos {
     }
     if (ix < 0 || ix >= T.nc[0]) return false;
     const int nz = T.nc[L];
-    const int z0 = max(zc - kStencil, 0), z1 = min(zc + kStencil, nz - 1);
+    const int z0 = max(w0, 0), z1 = min(w1, nz - 1);
+    if (z0 > z1) return false;
     uint32_t row;
     if constexpr (DIM == 2) {
       row = T.cell_off + static_cast<uint32_t>(ix) * nz;
@@ -138,14 +141,88 @@ __device__ __forceinline__ uint32_t pop_lowest(uint64_t& mask) {
   return t;
 }
 
+// Per-target stencil trimming.  A particle of cell (ix, iy, z) lies within
+// skin/2 of that cell's box (it is sorted into it at the last rebuild and
+// has moved less than skin/2 since; only border cells hold particles outside
+// the grid, clamped in).  So for a target whose whole 5^d window is interior
+// (no border cell), a column whose grown box is farther than R from the
+// target in the column plane holds no neighbour, and only the rows within
+// sqrt(R^2 - gap^2) + skin/2 of the target can.  The windows are widened by
+// a relative 1e-6 margin; trimming removes only candidates the exact FP64
+// predicate rejects, so the accepted set and its order are unchanged.
+struct Trim {
+  bool on;
+  double x[3];
+  double lo[3], c[3], inv_c[3];
+  double R2, s;
+};
+
+template <int DIM>
+__device__ __forceinline__ Trim make_trim(const DevParams& P, const DevTank& T,
+                                          const double* xa, int cx, int cy, int zc) {
+  constexpr int L = DIM - 1;
+  Trim t;
+  bool inner = cx - kStencil > 0 && cx + kStencil < T.nc[0] - 1 && zc - kStencil > 0 &&
+               zc + kStencil < T.nc[L] - 1;
+  if constexpr (DIM == 3) inner = inner && cy - kStencil > 0 && cy + kStencil < T.nc[1] - 1;
+  t.on = inner;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    t.x[k] = xa[k];
+    t.lo[k] = T.lo[k];
+    t.inv_c[k] = T.inv_c[k];
+    t.c[k] = 1.0 / T.inv_c[k];
+  }
+  t.R2 = P.R2 * (1.0 + 1e-6);
+  t.s = P.skin_half * (1.0 + 1e-6);
+  return t;
+}
+
+// Row window [w0, w1] of stencil column k; false: no neighbour in column k.
+template <int DIM>
+__device__ __forceinline__ bool trim_window(const Trim& t, int cx, int cy, int zc, int k,
+                                            int& w0, int& w1) {
+  constexpr int L = DIM - 1;
+  w0 = zc - kStencil;
+  w1 = zc + kStencil;
+  if (!t.on) return true;
+  int ixk, iyk = 0;
+  if constexpr (DIM == 2) {
+    ixk = cx - kStencil + k;
+  } else {
+    ixk = cx - kStencil + k / kCols2;
+    iyk = cy - kStencil + k % kCols2;
+  }
+  auto gap = [&](int ax, int ic) {
+    const double a0 = t.lo[ax] + ic * t.c[ax] - t.s;
+    const double a1 = t.lo[ax] + (ic + 1) * t.c[ax] + t.s;
+    return fmax(0.0, fmax(a0 - t.x[ax], t.x[ax] - a1));
+  };
+  const double gx = gap(0, ixk);
+  double g2 = gx * gx;
+  if constexpr (DIM == 3) {
+    const double gy = gap(1, iyk);
+    g2 = g2 + gy * gy;
+  }
+  if (g2 > t.R2) return false;
+  const double half = sqrt(t.R2 - g2) + t.s;
+  const double z = t.x[L] - t.lo[L];
+  w0 = max(w0, static_cast<int>(floor((z - half) * t.inv_c[L] - 1e-6)));
+  w1 = min(w1, static_cast<int>(floor((z + half) * t.inv_c[L] + 1e-6)));
+  return w0 <= w1;
+}
+
 // Pair sums of one target over its stencil, column range by column range.
 template <int DIM, int KERNEL, int FLAVOUR, int NCOL, class Pos>
 __device__ __forceinline__ void gather(const DevParams& P, const Pos& ps,
                                        const DevState& S, const Target<DIM>& a,
-                                       uint32_t i, int zc, Sums<DIM>& acc) {
+                                       uint32_t i, int cx, int cy, int zc, const Trim& tr,
+                                       Sums<DIM>& acc, uint32_t& count) {
   for (int k = 0; k < NCOL; ++k) {
     uint32_t b, e;
-    if (!ps.range(k, zc, b, e)) continue;
+    int w0, w1;
+    if (!trim_window<DIM>(tr, cx, cy, zc, k, w0, w1)) continue;
+    if (!ps.range(k, w0, w1, b, e)) continue;
     for (uint32_t c0 = b; c0 < e; c0 += 64) {
       const uint32_t len = min(e - c0, 64u);
       uint64_t mask = 0;
@@ -168,6 +245,7 @@ __device__ __forceinline__ void gather(const DevParams& P, const Pos& ps,
         if (within<DIM>(P, x0, a.x)) mask |= 1ull << t;
       }
       if (i >= c0 && i < c0 + len) mask &= ~(1ull << (i - c0));  // not self
+      count += static_cast<uint32_t>(__builtin_popcountll(mask));
       if (!mask) continue;
       // Two register sets alternate: the next accepted candidate's loads are
       // in flight while the current one is summed.
@@ -202,6 +280,33 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
   return v;
 }
 
+// Per-wave running reductions of a persistent loop: the chunks a wave takes
+// in turn mostly belong to one tank, so the tank's slots are updated once per
+// change of tank (and at the end) instead of once per chunk.
+struct WaveReduce {
+  uint32_t t = ~0u;
+  float vmax = 0.0f;
+  unsigned long long dtmin = ~0ull;
+  __device__ __forceinline__ void flush(const StageArgs& A) {
+    if (t == ~0u) return;
+    if ((threadIdx.x & 63) == 0) {
+      if (vmax > 0.0f) atomicMax(A.vmax + t, __float_as_uint(vmax));
+      if (dtmin != ~0ull) atomicMin(A.dt_min + t, dtmin);
+    }
+    vmax = 0.0f;
+    dtmin = ~0ull;
+  }
+  __device__ __forceinline__ void add(const StageArgs& A, uint32_t tank, float vm,
+                                      unsigned long long dm) {
+    if (tank != t) {
+      flush(A);
+      t = tank;
+    }
+    vmax = fmaxf(vmax, vm);
+    dtmin = dm < dtmin ? dm : dtmin;
+  }
+};
+
 struct ChunkInfo {
   uint32_t t, col_row, t0, t1;
   int cx, cy, nz;
@@ -229,7 +334,7 @@ __device__ __forceinline__ ChunkInfo chunk_info(const StageArgs& A, uint32_t q) 
 // the next step's CFL candidate; per-wave reductions at the end.
 template <int DIM, int KERNEL, int FLAVOUR, int KIND, bool FINAL, class Pos>
 __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInfo& c,
-                                              const Pos& ps) {
+                                              const Pos& ps, WaveReduce& red) {
   constexpr int NCOL = DIM == 2 ? kCols2 : kCols2 * kCols2;
   const DevParams& P = A.P;
   const uint32_t i = c.t0 + threadIdx.x;
@@ -243,6 +348,7 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
       A.rhs_drho[i] = 0.0;
 #pragma unroll
       for (int k = 0; k < DIM; ++k) A.rhs_dv[k][i] = 0.0;
+      if (A.rhs_count) A.rhs_count[i] = 0u;
     } else {
 #pragma unroll
       for (int k = 0; k < DIM; ++k) {
@@ -266,7 +372,12 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
     const int zc = static_cast<int>(A.key[i] - c.col_row);
     Sums<DIM> acc;
     acc.init();
-    gather<DIM, KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, acc);
+    const Trim tr = make_trim<DIM>(P, c.T, xa, c.cx, c.cy, zc);
+    uint32_t count = 0;
+    gather<DIM, KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, c.cx, c.cy, zc, tr, acc, count);
+    if constexpr (KIND == kRhsOnly) {
+      if (A.rhs_count) A.rhs_count[i] = count;
+    }
     const double drho = static_cast<double>(acc.rho.value());
     double dv[DIM];
 #pragma unroll
@@ -309,13 +420,11 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
     }
   }
   if constexpr (KIND != kRhsOnly) {
-    // all targets of a chunk belong to one tank: one atomic per wave
+    // all targets of a chunk belong to one tank
     const float vm = wave_max_f32(vmax);
-    if ((threadIdx.x & 63) == 0 && vm > 0.0f) atomicMax(A.vmax + c.t, __float_as_uint(vm));
-    if constexpr (FINAL) {
-      const unsigned long long dm = wave_min_u64(dtmin);
-      if ((threadIdx.x & 63) == 0 && dm != ~0ull) atomicMin(A.dt_min + c.t, dm);
-    }
+    unsigned long long dm = ~0ull;
+    if constexpr (FINAL) dm = wave_min_u64(dtmin);
+    red.add(A, c.t, vm, dm);
   }
 }
 
@@ -341,6 +450,7 @@ __global__ __launch_bounds__(NT) void k_stage_lds(StageArgs A) {
   __shared__ int s_fits;
   const int tid = threadIdx.x;
   const Band band = xcd_band(A.counts[0]);
+  WaveReduce red;
   for (uint32_t q = band.lo; q < band.hi; q += band.step) {
     const ChunkInfo c = chunk_info<2>(A, q);
     const int zf = static_cast<int>(A.key[c.t0] - c.col_row);
@@ -384,8 +494,9 @@ __global__ __launch_bounds__(NT) void k_stage_lds(StageArgs A) {
     }
     __syncthreads();
     const LdsPos ps{s_xy, s_coff, s_gb, s_lb, zlo, c.nz, c.cx, c.T.nc[0]};
-    chunk_targets<2, KERNEL, FLAVOUR, KIND, FINAL>(A, c, ps);
+    chunk_targets<2, KERNEL, FLAVOUR, KIND, FINAL>(A, c, ps, red);
   }
+  red.flush(A);
 }
 
 // Chunks with positions from global memory: 3D, and 2D overflow chunks.
@@ -393,11 +504,13 @@ template <int DIM, int KERNEL, int FLAVOUR, int KIND, bool FINAL, int NT>
 __global__ __launch_bounds__(NT) void k_stage_global(StageArgs A, const uint32_t* list,
                                                      const uint32_t* count) {
   const Band band = xcd_band(*count);
+  WaveReduce red;
   for (uint32_t q = band.lo; q < band.hi; q += band.step) {
     const ChunkInfo c = chunk_info<DIM>(A, list ? list[q] : q);
     const GlobalPos<DIM> ps{c.T, A.in, A.cell_off, c.cx, c.cy};
-    chunk_targets<DIM, KERNEL, FLAVOUR, KIND, FINAL>(A, c, ps);
+    chunk_targets<DIM, KERNEL, FLAVOUR, KIND, FINAL>(A, c, ps, red);
   }
+  red.flush(A);
 }
 
 // ---- chunk list (after every cell rebuild) ----------------------------------
@@ -443,27 +556,54 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum,
   return r;
 }
 
-// One workgroup recuts every column into balanced chunks, in column order.
-__global__ __launch_bounds__(kScanBlock) void k_build_chunks(StageArgs A) {
+// Chunk list in column order, in three passes: chunks per column and their
+// per-workgroup totals; an exclusive scan of the totals (one workgroup, also
+// writes the chunk count); each column's chunks at its scanned offset.
+__global__ __launch_bounds__(kScanBlock) void k_chunk_count(StageArgs A, uint32_t* block_tot,
+                                                           const uint32_t* ctl) {
+  if (!ctl[0]) return;
+  __shared__ uint32_t wsum[kScanBlock / 64 + 1];
+  const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t beg = 0, n = 0, nch = 0;
+  if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
+  uint32_t tot;
+  (void)block_excl_scan(nch, wsum, &tot);
+  if (threadIdx.x == 0) block_tot[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_chunk_base(StageArgs A, uint32_t* block_tot,
+                                                          uint32_t n_blocks, const uint32_t* ctl) {
+  if (!ctl[0]) return;
   __shared__ uint32_t wsum[kScanBlock / 64 + 1];
   uint32_t base_off = 0;
-  for (uint32_t base = 0; base < A.n_cols; base += kScanBlock) {
-    const uint32_t c = base + threadIdx.x;
-    uint32_t beg = 0, n = 0, nch = 0;
-    if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
+  for (uint32_t base = 0; base < n_blocks; base += kScanBlock) {
+    const uint32_t b = base + threadIdx.x;
+    const uint32_t v = b < n_blocks ? block_tot[b] : 0u;
     uint32_t tot;
-    const uint32_t off = base_off + block_excl_scan(nch, wsum, &tot);
-    for (uint32_t k = 0; k < nch; ++k) {
-      const uint32_t p0 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * k) / nch);
-      const uint32_t p1 =
-          beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * (k + 1)) / nch);
-      A.chunks[off + k] = make_uint4(c, p0, p1, 0u);
-    }
+    const uint32_t off = base_off + block_excl_scan(v, wsum, &tot);
+    if (b < n_blocks) block_tot[b] = off;
     base_off += tot;
   }
   if (threadIdx.x == 0) {
     A.counts[0] = base_off;  // chunks
     A.counts[1] = 0;         // overflow chunks (appended by the stage kernel)
+  }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_chunk_emit(StageArgs A,
+                                                          const uint32_t* block_tot,
+                                                          const uint32_t* ctl) {
+  if (!ctl[0]) return;
+  __shared__ uint32_t wsum[kScanBlock / 64 + 1];
+  const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
+  uint32_t beg = 0, n = 0, nch = 0;
+  if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
+  uint32_t tot;
+  const uint32_t off = block_tot[blockIdx.x] + block_excl_scan(nch, wsum, &tot);
+  for (uint32_t k = 0; k < nch; ++k) {
+    const uint32_t p0 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * k) / nch);
+    const uint32_t p1 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * (k + 1)) / nch);
+    A.chunks[off + k] = make_uint4(c, p0, p1, 0u);
   }
 }
 
@@ -551,8 +691,13 @@ void stage_grids(const DevParams& P, int n_cu, uint32_t* grid_lds,
   *grid_global = 8u * static_cast<uint32_t>((bg * n_cu + 7) / 8);
 }
 
-void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t*) {
-  k_build_chunks<<<1, kScanBlock, 0, s>>>(a);
+void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot,
+                         const uint32_t* ctl) {
+  const uint32_t nb = (a.n_cols + kScanBlock - 1) / kScanBlock;
+  if (nb == 0) return;
+  k_chunk_count<<<nb, kScanBlock, 0, s>>>(a, block_tot, ctl);
+  k_chunk_base<<<1, kScanBlock, 0, s>>>(a, block_tot, nb, ctl);
+  k_chunk_emit<<<nb, kScanBlock, 0, s>>>(a, block_tot, ctl);
 }
 
 void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final) {

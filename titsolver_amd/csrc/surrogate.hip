@@ -318,20 +318,40 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
       cand = dbl_bits(particle_time_step<DIM>(P, rho, v1, a));
     }
   }
-  // one atomic per wave when the wave's fluid lanes share a tank
+  // wave minimum when the wave's fluid lanes share a tank, then one atomic
+  // per workgroup and tank
+  __shared__ unsigned long long s_min[kBlk / 64];
+  __shared__ uint32_t s_t[kBlk / 64];
+  const int w = threadIdx.x >> 6;
   const uint64_t act = __ballot(fluid);
-  if (act == 0) return;
-  const int leader = __ffsll(static_cast<long long>(act)) - 1;
+  const int leader = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
   const uint32_t t0 = __shfl(t, leader);
-  if (__ballot(fluid && t != t0) == 0) {
-    unsigned long long m = cand;
-    for (int o = 32; o > 0; o >>= 1) {
-      const unsigned long long q = __shfl_xor(m, o);
-      m = q < m ? q : m;
+  const bool uniform = act != 0 && __ballot(fluid && t != t0) == 0;
+  if (act != 0 && !uniform) {
+    if (fluid) atomicMin(dt_min + t, cand);
+  }
+  unsigned long long m = uniform ? cand : ~0ull;
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long q = __shfl_xor(m, o);
+    m = q < m ? q : m;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_min[w] = m;
+    s_t[w] = uniform ? t0 : ~0u;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < kBlk / 64; ++a) {
+      if (s_t[a] == ~0u) continue;
+      unsigned long long mm = s_min[a];
+      for (int b = a + 1; b < kBlk / 64; ++b) {
+        if (s_t[b] == s_t[a]) {
+          mm = s_min[b] < mm ? s_min[b] : mm;
+          s_t[b] = ~0u;
+        }
+      }
+      atomicMin(dt_min + s_t[a], mm);
     }
-    if ((threadIdx.x & 63) == leader) atomicMin(dt_min + t0, m);
-  } else if (fluid) {
-    atomicMin(dt_min + t, cand);
   }
 }
 
@@ -532,13 +552,28 @@ struct Sample {
   std::vector<void*> mem;
 };
 
+// Everything a plain step reads besides the parameters: the state in cell
+// order with its cell index (keys, offsets, chunk list, fixed list), the
+// clocks and the rebuild bookkeeping.  Restoring it makes the following
+// plain steps repeat bit for bit what they would have done.
 struct Snapshot {
   DevState s{};
   uint32_t* id = nullptr;
   uint32_t* tag = nullptr;
+  uint32_t* key = nullptr;
+  uint32_t* cell_off = nullptr;
+  uint4* chunks = nullptr;
+  uint32_t* chunk_counts = nullptr;
+  uint32_t* fixed_list = nullptr;
+  uint32_t* counter = nullptr;
+  uint32_t* ctl = nullptr;
+  uint32_t* need = nullptr;
+  uint32_t* vmax = nullptr;
+  double* disp = nullptr;
   double* time = nullptr;
   double* dt = nullptr;
   unsigned long long* dt_min = nullptr;  // pending step size of each tank
+  size_t chunk_cap = 0;
   bool alloc = false;
 };
 
@@ -980,7 +1015,7 @@ tit_status build_graph(tit_hip_ctx* c) {
 // Rebuild, boundary values, graph, features and the network's output on the
 // current state (in L->infer.u[kRounds - 1]).
 tit_status infer_current(tit_hip_ctx* c) {
-  enqueue_rebuild(c);
+  enqueue_rebuild(c, kRebuildForce);
   enqueue_boundary(c, c->A);
   tit_status st = build_graph(c);
   if (st != TIT_OK) return st;
@@ -1002,7 +1037,8 @@ tit_status infer_current(tit_hip_ctx* c) {
 // One learned step of 4 dt, dt being the tank's pending step size (taken as a
 // plain step takes it); leaves the next pending step size.
 tit_status learned_step(tit_hip_ctx* c) {
-  launch_step_begin(c->stream, c->n_tanks, c->dt_min, c->dt, c->vmax);
+  launch_step_begin(c->stream, c->n_tanks, c->dt_min, c->dt, c->vmax, c->disp,
+                    c->dp.skin_half, rebuild_ctl(c));
   tit_status st = infer_current(c);
   if (st != TIT_OK) return st;
   Learned* L = c->learned;
@@ -1019,63 +1055,88 @@ tit_status learned_step(tit_hip_ctx* c) {
   }
   k_advance_time<<<(c->n_tanks + 63) / 64, 64, 0, c->stream>>>(c->n_tanks, c->time, c->dt,
                                                                kFactor);
+  // the particles moved beyond what the displacement bound tracks: the next
+  // step (plain or learned) rebuilds the cell index
+  launch_mark_pending(c->stream, rebuild_ctl(c));
   return check_launch(c, "learned update");
 }
 
 hipError_t alloc_snapshot(tit_hip_ctx* c, Snapshot& S) {
-  if (S.alloc) return hipSuccess;
+  if (S.alloc && S.chunk_cap == c->chunk_cap) return hipSuccess;
   Learned* L = c->learned;
   auto& m = L->mem_n;
-  for (int k = 0; k < 3; ++k) S.s.x[k] = S.s.v[k] = nullptr;
-  for (int k = 0; k < c->prm.dim; ++k) {
-    TIT_TRY_E(lalloc(m, &S.s.x[k], c->n));
-    TIT_TRY_E(lalloc(m, &S.s.v[k], c->n));
+  if (!S.alloc) {
+    for (int k = 0; k < 3; ++k) S.s.x[k] = S.s.v[k] = nullptr;
+    for (int k = 0; k < c->prm.dim; ++k) {
+      TIT_TRY_E(lalloc(m, &S.s.x[k], c->n));
+      TIT_TRY_E(lalloc(m, &S.s.v[k], c->n));
+    }
+    TIT_TRY_E(lalloc(m, &S.s.rho, c->n));
+    TIT_TRY_E(lalloc(m, &S.s.aux, c->n));
+    TIT_TRY_E(lalloc(m, &S.id, c->n));
+    TIT_TRY_E(lalloc(m, &S.tag, c->n));
+    TIT_TRY_E(lalloc(m, &S.key, c->n));
+    TIT_TRY_E(lalloc(m, &S.cell_off, static_cast<size_t>(c->n_cells) + 1));
+    TIT_TRY_E(lalloc(m, &S.chunk_counts, 4));
+    TIT_TRY_E(lalloc(m, &S.fixed_list, std::max<uint32_t>(c->n_fixed, 1)));
+    TIT_TRY_E(lalloc(m, &S.counter, 4));
+    TIT_TRY_E(lalloc(m, &S.ctl, 4));
+    TIT_TRY_E(lalloc(m, &S.need, c->n_tanks));
+    TIT_TRY_E(lalloc(m, &S.vmax, c->n_tanks));
+    TIT_TRY_E(lalloc(m, &S.disp, c->n_tanks));
+    TIT_TRY_E(lalloc(m, &S.time, c->n_tanks));
+    TIT_TRY_E(lalloc(m, &S.dt, c->n_tanks));
+    TIT_TRY_E(lalloc(m, &S.dt_min, c->n_tanks));
   }
-  TIT_TRY_E(lalloc(m, &S.s.rho, c->n));
-  TIT_TRY_E(lalloc(m, &S.s.aux, c->n));
-  TIT_TRY_E(lalloc(m, &S.id, c->n));
-  TIT_TRY_E(lalloc(m, &S.tag, c->n));
-  TIT_TRY_E(lalloc(m, &S.time, c->n_tanks));
-  TIT_TRY_E(lalloc(m, &S.dt, c->n_tanks));
-  TIT_TRY_E(lalloc(m, &S.dt_min, c->n_tanks));
+  // the chunk buffer follows the context's schedule
+  TIT_TRY_E(lalloc(m, &S.chunks, c->chunk_cap));
+  S.chunk_cap = c->chunk_cap;
   S.alloc = true;
   return hipSuccess;
 }
 
-hipError_t copy_state(tit_hip_ctx* c, const DevState& from, const uint32_t* id_from,
-                      const uint32_t* tag_from, const double* time_from,
-                      const double* dt_from, const unsigned long long* dtm_from,
-                      DevState& to, uint32_t* id_to, uint32_t* tag_to, double* time_to,
-                      double* dt_to, unsigned long long* dtm_to) {
+template <class T>
+hipError_t dcopy(T* to, const T* from, size_t count, hipStream_t s) {
+  return hipMemcpyAsync(to, from, count * sizeof(T), hipMemcpyDeviceToDevice, s);
+}
+
+// to_snapshot: context -> S, else S -> context.
+hipError_t copy_state(tit_hip_ctx* c, Snapshot& S, bool to_snapshot) {
   hipStream_t s = c->stream;
-  const size_t n = c->n;
+  const size_t n = c->n, nt = c->n_tanks;
+  auto cp = [&](auto* ctx_p, auto* snap_p, size_t count) {
+    return to_snapshot ? dcopy(snap_p, ctx_p, count, s) : dcopy(ctx_p, snap_p, count, s);
+  };
   for (int k = 0; k < c->prm.dim; ++k) {
-    TIT_TRY_E(hipMemcpyAsync(to.x[k], from.x[k], n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    TIT_TRY_E(hipMemcpyAsync(to.v[k], from.v[k], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    TIT_TRY_E(cp(c->A.x[k], S.s.x[k], n));
+    TIT_TRY_E(cp(c->A.v[k], S.s.v[k], n));
   }
-  TIT_TRY_E(hipMemcpyAsync(to.rho, from.rho, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-  TIT_TRY_E(hipMemcpyAsync(to.aux, from.aux, n * sizeof(float2), hipMemcpyDeviceToDevice, s));
-  TIT_TRY_E(hipMemcpyAsync(id_to, id_from, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-  TIT_TRY_E(hipMemcpyAsync(tag_to, tag_from, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-  TIT_TRY_E(hipMemcpyAsync(time_to, time_from, c->n_tanks * sizeof(double),
-                           hipMemcpyDeviceToDevice, s));
-  TIT_TRY_E(hipMemcpyAsync(dt_to, dt_from, c->n_tanks * sizeof(double),
-                           hipMemcpyDeviceToDevice, s));
-  TIT_TRY_E(hipMemcpyAsync(dtm_to, dtm_from, c->n_tanks * sizeof(unsigned long long),
-                           hipMemcpyDeviceToDevice, s));
+  TIT_TRY_E(cp(c->A.rho, S.s.rho, n));
+  TIT_TRY_E(cp(c->A.aux, S.s.aux, n));
+  TIT_TRY_E(cp(c->id, S.id, n));
+  TIT_TRY_E(cp(c->tag, S.tag, n));
+  TIT_TRY_E(cp(c->key, S.key, n));
+  TIT_TRY_E(cp(c->cell_off, S.cell_off, static_cast<size_t>(c->n_cells) + 1));
+  TIT_TRY_E(cp(c->chunks, S.chunks, c->chunk_cap));
+  TIT_TRY_E(cp(c->chunk_counts, S.chunk_counts, 4));
+  TIT_TRY_E(cp(c->fixed_list, S.fixed_list, std::max<uint32_t>(c->n_fixed, 1)));
+  TIT_TRY_E(cp(c->counter, S.counter, 4));
+  TIT_TRY_E(cp(c->rebuild_ctl, S.ctl, 4));
+  TIT_TRY_E(cp(c->tank_need, S.need, nt));
+  TIT_TRY_E(cp(c->vmax, S.vmax, nt));
+  TIT_TRY_E(cp(c->disp, S.disp, nt));
+  TIT_TRY_E(cp(c->time, S.time, nt));
+  TIT_TRY_E(cp(c->dt, S.dt, nt));
+  TIT_TRY_E(cp(c->dt_min, S.dt_min, nt));
   return hipSuccess;
 }
 
 hipError_t save(tit_hip_ctx* c, Snapshot& S) {
   TIT_TRY_E(alloc_snapshot(c, S));
-  return copy_state(c, c->A, c->id, c->tag, c->time, c->dt, c->dt_min, S.s, S.id, S.tag,
-                    S.time, S.dt, S.dt_min);
+  return copy_state(c, S, true);
 }
 
-hipError_t restore(tit_hip_ctx* c, const Snapshot& S) {
-  return copy_state(c, S.s, S.id, S.tag, S.time, S.dt, S.dt_min, c->A, c->id, c->tag,
-                    c->time, c->dt, c->dt_min);
-}
+hipError_t restore(tit_hip_ctx* c, Snapshot& S) { return copy_state(c, S, false); }
 
 tit_status upload_weights(tit_hip_ctx* c, const float* w) {
   Learned* L = c->learned;
@@ -1188,7 +1249,7 @@ tit_status tit_hip_learned_graph(tit_hip_ctx* c, uint32_t* nbr_ref, float* node_
   float *d_nf = nullptr, *d_ef = nullptr;
   std::vector<void*> tmp;
   auto cleanup = [&] { free_mem(tmp); };
-  enqueue_rebuild(c);
+  enqueue_rebuild(c, kRebuildForce);
   enqueue_boundary(c, c->A);
   st = build_graph(c);
   if (st != TIT_OK) return st;
@@ -1248,7 +1309,7 @@ tit_status tit_hip_learned_record(tit_hip_ctx* c, int plain_steps, int* sample_i
   const uint32_t n = c->n;
   const int dim = c->prm.dim;
   hipStream_t s = c->stream;
-  enqueue_rebuild(c);
+  enqueue_rebuild(c, kRebuildForce);
   enqueue_boundary(c, c->A);
   st = build_graph(c);
   if (st != TIT_OK) return st;

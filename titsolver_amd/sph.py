@@ -228,6 +228,12 @@ class HipContext:
             idx.ctypes.data_as(C.POINTER(C.c_uint32)), idx.size, C.byref(total)))
         return off, idx[: total.value]
 
+    def pair_counts(self) -> np.ndarray:
+        """Neighbours summed per particle by the stage kernel (test hook)."""
+        out = np.empty(self.n, dtype=np.uint32)
+        self._check(self.lib.tit_hip_pair_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
     def set_profiling(self, on: bool):
         self._check(self.lib.tit_hip_set_profiling(self.h, 1 if on else 0))
 
@@ -261,10 +267,12 @@ class HipContext:
                 "occupancy": sch.occupancy, "ms_per_step": ms.value}
 
     def counts(self):
+        """(particles, cell-index rebuilds run, plain steps taken)."""
         n = C.c_size_t()
         rb = C.c_uint64()
-        self._check(self.lib.tit_hip_counts(self.h, C.byref(n), C.byref(rb)))
-        return n.value, rb.value
+        st = C.c_uint64()
+        self._check(self.lib.tit_hip_counts(self.h, C.byref(n), C.byref(rb), C.byref(st)))
+        return n.value, rb.value, st.value
 
 
 def _fp(a):
