@@ -88,6 +88,9 @@ def parse():
     ap.add_argument("--learned-steps", type=int, default=50,
                     help="timed learned steps (4 dt each; judge every 10)")
     ap.add_argument("--train-iters", type=int, default=2400)
+    ap.add_argument("--eager", action="store_true",
+                    help="no hipGraph replay: every launch eager with HIP events "
+                         "(for profiler runs)")
     return ap.parse_args()
 
 
@@ -121,20 +124,21 @@ def main():
         sys.exit(2)
     import titsolver_amd as ta
 
+    global EAGER
+    EAGER = args.eager
     arrays, eq, desc = build_workload(args)
     pa = arrays[0]
     nf = sum(a.n_fluid for a in arrays)
     n_fixed = sum(a.n_fixed for a in arrays)
     dim = eq.dim
-    ctx = ta.HipContext(eq, order=3)
-    ctx.upload(arrays)
+    ctx = make_ctx(eq, arrays)
     print(f"bench: {desc}: {nf} fluid + {n_fixed} fixed particles uploaded; "
           f"warmup {args.warmup}", file=sys.stderr, flush=True)
     ctx.step(args.warmup)
     tuned = None
     if args.chunk or args.lds_cap >= 0:
         ctx.set_schedule(args.chunk, None if args.lds_cap < 0 else args.lds_cap)
-    elif not args.no_autotune:
+    elif not args.no_autotune and not EAGER:
         tuned = ctx.autotune(4)
         print(f"bench: autotuned schedule {tuned}", file=sys.stderr, flush=True)
     sched = ctx.get_schedule()
@@ -201,6 +205,18 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+EAGER = False
+
+
+def make_ctx(eq, arrays):
+    import titsolver_amd as ta
+    ctx = ta.HipContext(eq, order=3)
+    ctx.upload(arrays)
+    if EAGER:
+        ctx.set_profiling(True)
+    return ctx
+
+
 def build_workload(args):
     import titsolver_amd as ta
     if args.workload == "dam2d":
@@ -247,8 +263,7 @@ def learned_leg(args):
     eq = ta.cfg_equations(2, dr, 2.0)
     # training data: 2 seeded tanks, 12 samples over the first ~250 steps
     tr = seeded_tanks(0, dr, n_tanks=2)
-    ctx = ta.HipContext(eq)
-    ctx.upload(tr)
+    ctx = make_ctx(eq, tr)
     net = ta.LearnedStep(ctx, seed=0)
     ctx.step(20)
     for _ in range(12):
@@ -271,8 +286,7 @@ def learned_leg(args):
            "train_s": t_train, "train_loss_first": float(np.mean(losses[:12])),
            "train_loss_last": float(np.mean(losses[-12:]))}
     # plain form
-    pctx = ta.HipContext(eq)
-    pctx.upload(ev)
+    pctx = make_ctx(eq, ev)
     pctx.step(40)
     t0 = time.perf_counter()
     k_plain = 4 * args.learned_steps
@@ -280,8 +294,7 @@ def learned_leg(args):
     t_plain = time.perf_counter() - t0
     pctx.close()
     # learned form, same start state
-    lctx = ta.HipContext(eq)
-    lctx.upload(ev)
+    lctx = make_ctx(eq, ev)
     lctx.step(40)
     L = ta.LearnedStep(lctx, seed=1)
     L.set_scales(sc)

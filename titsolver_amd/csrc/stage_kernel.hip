@@ -213,15 +213,54 @@ __device__ __forceinline__ bool trim_window(const Trim& t, int cx, int cy, int z
 }
 
 // Pair sums of one target over its stencil, column range by column range.
+// The row windows of the NCOL stencil columns, packed 6 bits per column
+// (3-bit offsets of the first and last row from zc - 2; 7 = empty column), so
+// no FP64 trimming state stays live through the pair loop.
+template <int DIM, int NCOL>
+struct Windows {
+  uint32_t w[(NCOL * 6 + 31) / 32];
+  __device__ __forceinline__ void set(int k, uint32_t v) {
+    const int bit = 6 * k;
+    w[bit >> 5] |= v << (bit & 31);
+    if ((bit & 31) > 26) w[(bit >> 5) + 1] |= v >> (32 - (bit & 31));
+  }
+  __device__ __forceinline__ uint32_t get(int k) const {
+    const int bit = 6 * k;
+    uint32_t v = w[bit >> 5] >> (bit & 31);
+    if ((bit & 31) > 26) v |= w[(bit >> 5) + 1] << (32 - (bit & 31));
+    return v & 63u;
+  }
+};
+
+template <int DIM, int NCOL>
+__device__ __forceinline__ Windows<DIM, NCOL> stencil_windows(const Trim& tr, int cx, int cy,
+                                                              int zc) {
+  Windows<DIM, NCOL> W;
+#pragma unroll
+  for (int q = 0; q < (NCOL * 6 + 31) / 32; ++q) W.w[q] = 0u;
+  for (int k = 0; k < NCOL; ++k) {
+    int w0, w1;
+    uint32_t v = 7u | (7u << 3);
+    if (trim_window<DIM>(tr, cx, cy, zc, k, w0, w1)) {
+      v = static_cast<uint32_t>(w0 - (zc - kStencil)) |
+          (static_cast<uint32_t>(w1 - (zc - kStencil)) << 3);
+    }
+    W.set(k, v);
+  }
+  return W;
+}
+
 template <int DIM, int KERNEL, int FLAVOUR, int NCOL, class Pos>
 __device__ __forceinline__ void gather(const DevParams& P, const Pos& ps,
                                        const DevState& S, const Target<DIM>& a,
-                                       uint32_t i, int cx, int cy, int zc, const Trim& tr,
+                                       uint32_t i, int zc, const Windows<DIM, NCOL>& W,
                                        Sums<DIM>& acc, uint32_t& count) {
   for (int k = 0; k < NCOL; ++k) {
     uint32_t b, e;
-    int w0, w1;
-    if (!trim_window<DIM>(tr, cx, cy, zc, k, w0, w1)) continue;
+    const uint32_t v = W.get(k);
+    if ((v & 7u) == 7u) continue;
+    const int w0 = zc - kStencil + static_cast<int>(v & 7u);
+    const int w1 = zc - kStencil + static_cast<int>(v >> 3);
     if (!ps.range(k, w0, w1, b, e)) continue;
     for (uint32_t c0 = b; c0 < e; c0 += 64) {
       const uint32_t len = min(e - c0, 64u);
@@ -372,9 +411,10 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
     const int zc = static_cast<int>(A.key[i] - c.col_row);
     Sums<DIM> acc;
     acc.init();
-    const Trim tr = make_trim<DIM>(P, c.T, xa, c.cx, c.cy, zc);
+    const Windows<DIM, NCOL> W =
+        stencil_windows<DIM, NCOL>(make_trim<DIM>(P, c.T, xa, c.cx, c.cy, zc), c.cx, c.cy, zc);
     uint32_t count = 0;
-    gather<DIM, KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, c.cx, c.cy, zc, tr, acc, count);
+    gather<DIM, KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, W, acc, count);
     if constexpr (KIND == kRhsOnly) {
       if (A.rhs_count) A.rhs_count[i] = count;
     }
