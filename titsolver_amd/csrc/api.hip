@@ -234,8 +234,8 @@ void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode) {
     ProfScope ps(c->prof, s, "permute");
     launch_permute(s, c->dp, c->n, c->val, c->A, c->T, c->id, c->id_alt, c->tag, c->tag_alt,
                    c->fixed_list, c->counter, rc.ctl);
-    launch_restore_order(s, c->prm.dim, c->n, c->T, c->A, c->id_alt, c->id, c->tag_alt,
-                         c->tag, rc.ctl);
+    launch_restore_order(s, c->prm.dim, c->n, c->val, c->T, c->A, c->id_alt, c->id,
+                         c->tag_alt, c->tag, rc.ctl);
   }
   {
     ProfScope ps(c->prof, s, "cell_offsets");

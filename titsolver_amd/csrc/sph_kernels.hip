@@ -174,6 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(
   }
   if (!in) return;
   const uint32_t s = perm[i];
+  if (s == i) return;  // stays in its slot (its tank was not re-sorted, or it kept its place)
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
     dst.x[k][i] = src.x[k][s];
@@ -188,12 +189,13 @@ __global__ __launch_bounds__(kBlock) void k_permute(
 
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_restore_order(
-    uint32_t n, DevState src, DevState dst, const uint32_t* __restrict__ id_src,
-    uint32_t* __restrict__ id_dst, const uint32_t* __restrict__ tag_src,
-    uint32_t* __restrict__ tag_dst, const uint32_t* __restrict__ ctl) {
+    uint32_t n, const uint32_t* __restrict__ perm, DevState src, DevState dst,
+    const uint32_t* __restrict__ id_src, uint32_t* __restrict__ id_dst,
+    const uint32_t* __restrict__ tag_src, uint32_t* __restrict__ tag_dst,
+    const uint32_t* __restrict__ ctl) {
   if (!ctl[0]) return;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || perm[i] == i) return;  // unmoved: dst already holds it
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
     dst.x[k][i] = src.x[k][i];
@@ -375,7 +377,7 @@ __global__ void k_rebuild_clear(uint32_t* ctl) { ctl[0] = 0u; }
 
 __global__ void k_step_begin(int n_tanks, unsigned long long* dt_min, double* dt,
                              uint32_t* vmax, const double* disp, double skin_half,
-                             uint32_t* ctl, uint32_t* need) {
+                             double span, uint32_t* ctl, uint32_t* need) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tanks) return;
   const unsigned long long b = dt_min[t];
@@ -385,7 +387,7 @@ __global__ void k_step_begin(int n_tanks, unsigned long long* dt_min, double* dt
   dt_min[t] = ~0ull;
   const double vprev = static_cast<double>(__uint_as_float(vmax[t]));
   vmax[t] = 0u;
-  const bool nd = ctl[1] != 0u || !(disp[t] + 2.0 * h * vprev <= 0.9 * skin_half);
+  const bool nd = ctl[1] != 0u || !(disp[t] + 2.0 * span * h * vprev <= 0.9 * skin_half);
   need[t] = nd ? 1u : 0u;
   if (nd) atomicOr(ctl, 1u);
 }
@@ -407,11 +409,12 @@ __global__ void k_pending_rebuild(int n_tanks, uint32_t* ctl, uint32_t* need) {
 
 __global__ void k_step_end(int n_tanks, const double* dt, const uint32_t* vmax,
                            double* time, double* disp, double skin_half,
-                           uint32_t* flags) {
+                           uint32_t* flags, double span) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tanks) return;
-  time[t] += dt[t];
-  const double d = disp[t] + dt[t] * static_cast<double>(__uint_as_float(vmax[t]));
+  const double T = span * dt[t];
+  time[t] += T;
+  const double d = disp[t] + T * static_cast<double>(__uint_as_float(vmax[t]));
   disp[t] = d;
   if (!(d <= skin_half)) atomicOr(flags, 1u);
 }
@@ -557,17 +560,17 @@ void launch_permute(hipStream_t s, const DevParams& P, uint32_t n, const uint32_
   }
 }
 
-void launch_restore_order(hipStream_t s, int dim, uint32_t n, DevState src, DevState dst,
-                          const uint32_t* id_src, uint32_t* id_dst,
-                          const uint32_t* tag_src, uint32_t* tag_dst,
+void launch_restore_order(hipStream_t s, int dim, uint32_t n, const uint32_t* perm,
+                          DevState src, DevState dst, const uint32_t* id_src,
+                          uint32_t* id_dst, const uint32_t* tag_src, uint32_t* tag_dst,
                           const uint32_t* ctl) {
   if (n == 0) return;
   if (dim == 2) {
-    k_restore_order<2><<<grid_for(n), kBlock, 0, s>>>(n, src, dst, id_src, id_dst, tag_src,
-                                                      tag_dst, ctl);
+    k_restore_order<2><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst, id_src, id_dst,
+                                                      tag_src, tag_dst, ctl);
   } else {
-    k_restore_order<3><<<grid_for(n), kBlock, 0, s>>>(n, src, dst, id_src, id_dst, tag_src,
-                                                      tag_dst, ctl);
+    k_restore_order<3><<<grid_for(n), kBlock, 0, s>>>(n, perm, src, dst, id_src, id_dst,
+                                                      tag_src, tag_dst, ctl);
   }
 }
 
@@ -607,10 +610,10 @@ void launch_time_step(hipStream_t s, const DevParams& P, uint32_t n,
 
 void launch_step_begin(hipStream_t s, int n_tanks, unsigned long long* dt_min,
                        double* dt, uint32_t* vmax, const double* disp,
-                       double skin_half, RebuildCtl rc) {
+                       double skin_half, RebuildCtl rc, double span) {
   k_rebuild_clear<<<1, 1, 0, s>>>(rc.ctl);
   k_step_begin<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, dt_min, dt, vmax, disp,
-                                                     skin_half, rc.ctl, rc.need);
+                                                     skin_half, span, rc.ctl, rc.need);
 }
 
 void launch_force_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc) {
@@ -627,9 +630,9 @@ void launch_pending_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc) {
 
 void launch_step_end(hipStream_t s, int n_tanks, const double* dt,
                      const uint32_t* vmax, double* time, double* disp,
-                     double skin_half, uint32_t* flags) {
+                     double skin_half, uint32_t* flags, double span) {
   k_step_end<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, dt, vmax, time, disp,
-                                                   skin_half, flags);
+                                                   skin_half, flags, span);
 }
 
 void launch_reset_disp(hipStream_t s, int n_tanks, double* disp, RebuildCtl rc) {

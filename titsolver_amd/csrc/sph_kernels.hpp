@@ -81,10 +81,11 @@ void launch_permute(hipStream_t s, const DevParams& P, uint32_t n, const uint32_
                     uint32_t* id_dst, const uint32_t* tag_src,
                     uint32_t* tag_dst, uint32_t* fixed_list, uint32_t* fixed_count,
                     const uint32_t* ctl);
-// Copies the permuted state (src) back into the working buffers (dst).
-void launch_restore_order(hipStream_t s, int dim, uint32_t n, DevState src, DevState dst,
-                          const uint32_t* id_src, uint32_t* id_dst,
-                          const uint32_t* tag_src, uint32_t* tag_dst,
+// Copies the permuted state (src) back into the working buffers (dst) for
+// the slots whose particle moved (perm[i] != i; the others were not written).
+void launch_restore_order(hipStream_t s, int dim, uint32_t n, const uint32_t* perm,
+                          DevState src, DevState dst, const uint32_t* id_src,
+                          uint32_t* id_dst, const uint32_t* tag_src, uint32_t* tag_dst,
                           const uint32_t* ctl);
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
                          const uint32_t* key, uint32_t* cell_off, const uint32_t* ctl);
@@ -99,12 +100,12 @@ void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot,
 void launch_time_step(hipStream_t s, const DevParams& P, uint32_t n,
                       DevState st, const uint32_t* tag, const uint32_t* id,
                       const double* dv_ref, unsigned long long* dt_min);
-// Takes each tank's pending step size and decides its rebuild: needed when
-// forced or when disp + 2 dt vmax (vmax of the previous step) could pass 0.9
-// of half the skin by the end of the step.
+// Takes each tank's pending step size dt and decides its rebuild for a step
+// of span x dt: needed when forced or when disp + 2 span dt vmax (vmax of the
+// previous step) could pass 0.9 of half the skin by the end of the step.
 void launch_step_begin(hipStream_t s, int n_tanks, unsigned long long* dt_min,
                        double* dt, uint32_t* vmax, const double* disp,
-                       double skin_half, RebuildCtl rc);
+                       double skin_half, RebuildCtl rc, double span = 1.0);
 // Marks every tank for the next rebuild sequence.
 void launch_force_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc);
 // Makes a forced rebuild pending (after moves the displacement bound does not
@@ -112,9 +113,10 @@ void launch_force_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc);
 void launch_mark_pending(hipStream_t s, RebuildCtl rc);
 // Marks every tank if a forced rebuild is pending, none otherwise.
 void launch_pending_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc);
+// time += span dt; disp += span dt vmax, flagging a skin overrun.
 void launch_step_end(hipStream_t s, int n_tanks, const double* dt,
                      const uint32_t* vmax, double* time, double* disp,
-                     double skin_half, uint32_t* flags);
+                     double skin_half, uint32_t* flags, double span = 1.0);
 // Ends a rebuild sequence: zero displacement of the re-sorted tanks, count.
 void launch_reset_disp(hipStream_t s, int n_tanks, double* disp, RebuildCtl rc);
 void launch_export(hipStream_t s, const DevParams& P, uint32_t n, DevState st,

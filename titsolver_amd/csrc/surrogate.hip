@@ -291,11 +291,13 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
                                                          const double* __restrict__ dt,
                                                          double factor,
                                                          const float* __restrict__ u,
-                                                         unsigned long long* __restrict__ dt_min) {
+                                                         unsigned long long* __restrict__ dt_min,
+                                                         uint32_t* __restrict__ vmax) {
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   bool fluid = false;
   uint32_t t = 0;
   unsigned long long cand = ~0ull;
+  float vm = 0.0f;  // largest speed over the step (displacement bound)
   if (i < n) {
     const uint32_t tg = tag[i];
     t = tg >> 1;
@@ -304,6 +306,7 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
       const double T = factor * dt[t];
       const float* ui = u + static_cast<size_t>(i) * kOut;
       double a[DIM], v1[DIM];
+      double s0 = 0.0, s1 = 0.0;
 #pragma unroll
       for (int k = 0; k < DIM; ++k) {
         a[k] = static_cast<double>(ui[k]) * sc.out_scale[k];
@@ -311,7 +314,11 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
         v1[k] = v0 + T * a[k];
         st.x[k][i] = st.x[k][i] + T * (0.5 * (v0 + v1[k]));
         st.v[k][i] = v1[k];
+        s0 += v0 * v0;
+        s1 += v1[k] * v1[k];
       }
+      // the position moves by T (v0 + v1) / 2: |.| <= T max(|v0|, |v1|)
+      vm = static_cast<float>(sqrt(fmax(s0, s1))) * (1.0f + 1e-6f);
       const double rho = st.rho[i] + T * (static_cast<double>(ui[3]) * sc.out_scale[3]);
       st.rho[i] = rho;
       st.aux[i] = make_aux(P, rho);
@@ -321,22 +328,27 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
   // wave minimum when the wave's fluid lanes share a tank, then one atomic
   // per workgroup and tank
   __shared__ unsigned long long s_min[kBlk / 64];
+  __shared__ float s_max[kBlk / 64];
   __shared__ uint32_t s_t[kBlk / 64];
   const int w = threadIdx.x >> 6;
   const uint64_t act = __ballot(fluid);
   const int leader = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
   const uint32_t t0 = __shfl(t, leader);
   const bool uniform = act != 0 && __ballot(fluid && t != t0) == 0;
-  if (act != 0 && !uniform) {
-    if (fluid) atomicMin(dt_min + t, cand);
+  if (act != 0 && !uniform && fluid) {
+    atomicMin(dt_min + t, cand);
+    atomicMax(vmax + t, __float_as_uint(vm));
   }
   unsigned long long m = uniform ? cand : ~0ull;
+  float mx = uniform ? vm : 0.0f;
   for (int o = 32; o > 0; o >>= 1) {
     const unsigned long long q = __shfl_xor(m, o);
     m = q < m ? q : m;
+    mx = fmaxf(mx, __shfl_xor(mx, o));
   }
   if ((threadIdx.x & 63) == 0) {
     s_min[w] = m;
+    s_max[w] = mx;
     s_t[w] = uniform ? t0 : ~0u;
   }
   __syncthreads();
@@ -344,21 +356,20 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
     for (int a = 0; a < kBlk / 64; ++a) {
       if (s_t[a] == ~0u) continue;
       unsigned long long mm = s_min[a];
+      float xx = s_max[a];
       for (int b = a + 1; b < kBlk / 64; ++b) {
         if (s_t[b] == s_t[a]) {
           mm = s_min[b] < mm ? s_min[b] : mm;
+          xx = fmaxf(xx, s_max[b]);
           s_t[b] = ~0u;
         }
       }
       atomicMin(dt_min + s_t[a], mm);
+      atomicMax(vmax + s_t[a], __float_as_uint(xx));
     }
   }
 }
 
-__global__ void k_advance_time(int n_tanks, double* time, const double* dt, double factor) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n_tanks) time[t] = time[t] + factor * dt[t];
-}
 
 // ---- judge ------------------------------------------------------------------------
 
@@ -1014,8 +1025,8 @@ tit_status build_graph(tit_hip_ctx* c) {
 
 // Rebuild, boundary values, graph, features and the network's output on the
 // current state (in L->infer.u[kRounds - 1]).
-tit_status infer_current(tit_hip_ctx* c) {
-  enqueue_rebuild(c, kRebuildForce);
+tit_status infer_current(tit_hip_ctx* c, RebuildMode mode) {
+  enqueue_rebuild(c, mode);
   enqueue_boundary(c, c->A);
   tit_status st = build_graph(c);
   if (st != TIT_OK) return st;
@@ -1037,27 +1048,26 @@ tit_status infer_current(tit_hip_ctx* c) {
 // One learned step of 4 dt, dt being the tank's pending step size (taken as a
 // plain step takes it); leaves the next pending step size.
 tit_status learned_step(tit_hip_ctx* c) {
+  constexpr double kFactor = 4.0;
   launch_step_begin(c->stream, c->n_tanks, c->dt_min, c->dt, c->vmax, c->disp,
-                    c->dp.skin_half, rebuild_ctl(c));
-  tit_status st = infer_current(c);
+                    c->dp.skin_half, rebuild_ctl(c), kFactor);
+  tit_status st = infer_current(c, kRebuildMarked);
   if (st != TIT_OK) return st;
   Learned* L = c->learned;
-  constexpr double kFactor = 4.0;
   ProfScope ps(c->prof, c->stream, "learned_update");
   if (c->dp.dim == 2) {
     k_learned_update<2><<<nblk(c->n), kBlk, 0, c->stream>>>(
         c->dp, L->sc, c->n, c->A, c->tag, c->dt, kFactor, L->infer.u[kRounds - 1],
-        c->dt_min);
+        c->dt_min, c->vmax);
   } else {
     k_learned_update<3><<<nblk(c->n), kBlk, 0, c->stream>>>(
         c->dp, L->sc, c->n, c->A, c->tag, c->dt, kFactor, L->infer.u[kRounds - 1],
-        c->dt_min);
+        c->dt_min, c->vmax);
   }
-  k_advance_time<<<(c->n_tanks + 63) / 64, 64, 0, c->stream>>>(c->n_tanks, c->time, c->dt,
-                                                               kFactor);
-  // the particles moved beyond what the displacement bound tracks: the next
-  // step (plain or learned) rebuilds the cell index
-  launch_mark_pending(c->stream, rebuild_ctl(c));
+  // clock and displacement bound of the step (the next step rebuilds the
+  // cell index if the bound requires)
+  launch_step_end(c->stream, c->n_tanks, c->dt, c->vmax, c->time, c->disp, c->dp.skin_half,
+                  c->counter + 1, kFactor);
   return check_launch(c, "learned update");
 }
 
@@ -1249,7 +1259,7 @@ tit_status tit_hip_learned_graph(tit_hip_ctx* c, uint32_t* nbr_ref, float* node_
   float *d_nf = nullptr, *d_ef = nullptr;
   std::vector<void*> tmp;
   auto cleanup = [&] { free_mem(tmp); };
-  enqueue_rebuild(c, kRebuildForce);
+  enqueue_rebuild(c, kRebuildPending);
   enqueue_boundary(c, c->A);
   st = build_graph(c);
   if (st != TIT_OK) return st;
@@ -1282,7 +1292,7 @@ tit_status tit_hip_learned_forward(tit_hip_ctx* c, float* out) {
   if (st != TIT_OK) return st;
   if (!out) return fail(c, TIT_E_ARG, "null output");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
-  st = infer_current(c);
+  st = infer_current(c, kRebuildPending);
   if (st != TIT_OK) return st;
   float* d_out = nullptr;
   std::vector<void*> tmp;
@@ -1309,7 +1319,7 @@ tit_status tit_hip_learned_record(tit_hip_ctx* c, int plain_steps, int* sample_i
   const uint32_t n = c->n;
   const int dim = c->prm.dim;
   hipStream_t s = c->stream;
-  enqueue_rebuild(c, kRebuildForce);
+  enqueue_rebuild(c, kRebuildPending);
   enqueue_boundary(c, c->A);
   st = build_graph(c);
   if (st != TIT_OK) return st;
