@@ -121,6 +121,7 @@ hipError_t alloc_state(tit_hip_ctx* c, DevState& s, uint32_t n, int dim) {
   }
   if (e == hipSuccess) e = dalloc(c, &s.rho, n);
   if (e == hipSuccess) e = dalloc(c, &s.aux, n);
+  if (e == hipSuccess) e = dalloc(c, &s.pv, n);
   return e;
 }
 
@@ -193,6 +194,7 @@ tit_status fill_params(tit_hip_ctx* c, const tit_hip_params& p) {
   d.two_mu = static_cast<float>(2.0 * p.mu);
   d.inv_rho0_f = static_cast<float>(1.0 / p.rho0);
   d.cs0_f = static_cast<float>(p.cs0);
+  d.rho0_f = static_cast<float>(p.rho0);
   return TIT_OK;
 }
 
@@ -662,7 +664,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
                                 hipMemcpyHostToDevice, s));
   TIT_HIP_TRY(c, hipMemcpyAsync(c->tanks, ht.data(), ht.size() * sizeof(DevTank),
                                 hipMemcpyHostToDevice, s));
-  launch_import(s, dim, n, c->stage_r, c->stage_v, c->stage_rho, c->A, c->id);
+  launch_import(s, c->dp, n, c->stage_r, c->stage_v, c->stage_rho, c->A, c->id);
   std::vector<double> zeros(n_tanks, 0.0);
   std::vector<unsigned long long> inf(n_tanks, ~0ull);
   std::vector<uint32_t> z32(4, 0u);

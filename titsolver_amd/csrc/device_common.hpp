@@ -41,6 +41,7 @@ struct DevParams {
   float two_mu;     // 2 mu            (physical viscosity)
   float inv_rho0_f; // 1 / rho0
   float cs0_f;      // cs0
+  float rho0_f;     // rho0
 };
 
 // Per-tank cell grid.  Cell size is at least (support radius + skin) / 2, so

@@ -57,6 +57,8 @@ struct DevState {
   double* v[3];
   double* rho;
   float2* aux;  // per particle (p / rho^2, cs) of rho, FP32
+  float4* pv;   // per particle (v (FP32, z = 0 in 2D), rho - rho0): what the
+                // pair terms read of a neighbour besides its position
 };
 
 // ---- radix sort (radix_sort.hip) -------------------------------------------

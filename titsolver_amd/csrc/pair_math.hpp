@@ -13,7 +13,7 @@
 //     REF  Pi_ab = 2 mu (v_ab . r_ab) / (rho_a rho_b |r_ab|^2)   (:299-300)
 #pragma once
 
-#include "device_common.hpp"
+#include "internal.hpp"
 
 namespace tit_amd {
 
@@ -37,7 +37,7 @@ struct Target {
   double x[DIM];
   float v[DIM];
   double rho;
-  float rho_f, Pa, csa;
+  float drho_f, rho_f, Pa, csa;  // rho - rho0, rho (FP32)
 };
 
 template <int DIM>
@@ -65,6 +65,22 @@ __device__ __forceinline__ float2 make_aux(const DevParams& P, double rho) {
                      sound_speed_f32(P, static_cast<float>(rho)));
 }
 
+// (v, rho - rho0) in FP32 for the neighbours' side of the pair terms.
+template <int DIM>
+__device__ __forceinline__ float4 make_pv(const DevParams& P, const double* v, double rho) {
+  return make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
+                     DIM == 3 ? static_cast<float>(v[DIM - 1]) : 0.0f,
+                     static_cast<float>(rho - P.rho0));
+}
+
+// Writes both per-particle pair records of a particle.
+template <int DIM>
+__device__ __forceinline__ void write_pair_data(const DevParams& P, const DevState& st,
+                                                uint32_t i, const double* v, double rho) {
+  st.aux[i] = make_aux(P, rho);
+  st.pv[i] = make_pv<DIM>(P, v, rho);
+}
+
 template <int DIM>
 __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
                                             const double* v, double rho,
@@ -75,18 +91,20 @@ __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
     a.v[k] = static_cast<float>(v[k]);
   }
   a.rho = rho;
-  a.rho_f = static_cast<float>(rho);
+  a.drho_f = static_cast<float>(rho - P.rho0);
+  a.rho_f = P.rho0_f + a.drho_f;
   a.Pa = aux.x;
   a.csa = aux.y;
 }
 
-// d = r_b - r_a in FP64 (as in the neighbour predicate); Pb = p_b/rho_b^2,
-// csb = cs(rho_b) from the neighbour's aux.
+// d = r_b - r_a in FP64 (as in the neighbour predicate); pvb = (v_b,
+// rho_b - rho0), Pb = p_b/rho_b^2, csb = cs(rho_b) from the neighbour's
+// pair records.
 template <int DIM, int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>& a,
-                                          const double* d, const float* vb,
-                                          double rho_b, float Pb, float csb,
-                                          Sums<DIM>& s) {
+                                          const double* d, const float4& pvb, float Pb,
+                                          float csb, Sums<DIM>& s) {
+  const float vb[3] = {pvb.x, pvb.y, pvb.z};
   float x[DIM];
 #pragma unroll
   for (int k = 0; k < DIM; ++k) x[k] = static_cast<float>(-d[k]);
@@ -101,8 +119,8 @@ __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>&
   float vr = 0.0f;
 #pragma unroll
   for (int k = 0; k < DIM; ++k) vr = __builtin_fmaf(a.v[k] - vb[k], x[k], vr);
-  const float drab = static_cast<float>(a.rho - rho_b);
-  const float rbf = static_cast<float>(rho_b);
+  const float drab = a.drho_f - pvb.w;
+  const float rbf = P.rho0_f + pvb.w;
   const float Vb = P.m_f * __builtin_amdgcn_rcpf(rbf);
   float cont, Pi;
   if constexpr (FLAVOUR == kFlavourCfg) {

@@ -101,16 +101,19 @@ __device__ __forceinline__ void wave_atomic_min_u64(unsigned long long* base,
 
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_import(
-    uint32_t n, const double* __restrict__ r, const double* __restrict__ v,
+    DevParams P, uint32_t n, const double* __restrict__ r, const double* __restrict__ v,
     const double* __restrict__ rho, DevState st, uint32_t* __restrict__ id) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
+  double vi[DIM];
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
     st.x[k][i] = r[static_cast<size_t>(i) * DIM + k];
-    st.v[k][i] = v ? v[static_cast<size_t>(i) * DIM + k] : 0.0;
+    vi[k] = v ? v[static_cast<size_t>(i) * DIM + k] : 0.0;
+    st.v[k][i] = vi[k];
   }
   st.rho[i] = rho[i];
+  write_pair_data<DIM>(P, st, i, vi, rho[i]);
   id[i] = i;
 }
 
@@ -182,7 +185,10 @@ __global__ __launch_bounds__(kBlock) void k_permute(
   }
   const double rho = src.rho[s];
   dst.rho[i] = rho;
-  dst.aux[i] = make_aux(P, rho);
+  double v[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) v[k] = dst.v[k][i];
+  write_pair_data<DIM>(P, dst, i, v, rho);
   id_dst[i] = id_src[s];
   tag_dst[i] = tg;
 }
@@ -203,6 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_restore_order(
   }
   dst.rho[i] = src.rho[i];
   dst.aux[i] = src.aux[i];
+  dst.pv[i] = src.pv[i];
   id_dst[i] = id_src[i];
   tag_dst[i] = tag_src[i];
 }
@@ -343,7 +350,8 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     const double Hs = fabs(S) <= tiny ? 0.0 : H / S;
     const double rho_e = eos_density_from_potential(P, Hs);
     A.s.rho[i] = rho_e;
-    A.s.aux[i] = make_aux(P, rho_e);
+    const double zero[DIM] = {};
+    write_pair_data<DIM>(P, A.s, i, zero, rho_e);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) A.s.v[d][i] = 0.0;
   }
@@ -519,14 +527,14 @@ __global__ __launch_bounds__(kBlock) void k_neighbor_fill(
 
 }  // namespace
 
-void launch_import(hipStream_t s, int dim, uint32_t n, const double* r,
+void launch_import(hipStream_t s, const DevParams& P, uint32_t n, const double* r,
                    const double* v, const double* rho, DevState st,
                    uint32_t* id) {
   if (n == 0) return;
-  if (dim == 2) {
-    k_import<2><<<grid_for(n), kBlock, 0, s>>>(n, r, v, rho, st, id);
+  if (P.dim == 2) {
+    k_import<2><<<grid_for(n), kBlock, 0, s>>>(P, n, r, v, rho, st, id);
   } else {
-    k_import<3><<<grid_for(n), kBlock, 0, s>>>(n, r, v, rho, st, id);
+    k_import<3><<<grid_for(n), kBlock, 0, s>>>(P, n, r, v, rho, st, id);
   }
 }
 

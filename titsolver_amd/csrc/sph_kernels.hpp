@@ -70,7 +70,8 @@ struct RebuildCtl {
 // Kernel kinds of the stage launch.
 enum StageKind { kRhsOnly = 0, kEuler = 1, kCombine = 2 };
 
-void launch_import(hipStream_t s, int dim, uint32_t n, const double* r,
+// Reference-order arrays into the state (also its pair records).
+void launch_import(hipStream_t s, const DevParams& P, uint32_t n, const double* r,
                    const double* v, const double* rho, DevState st,
                    uint32_t* id);
 void launch_cell_keys(hipStream_t s, const DevParams& P, const DevTank* tanks,

@@ -98,8 +98,7 @@ struct GlobalPos {
 template <int DIM>
 struct PairIn {
   double x[DIM];
-  float v[DIM];
-  double rho;
+  float4 pv;
   float2 aux;
 };
 
@@ -107,9 +106,7 @@ template <int DIM, class Pos>
 __device__ __forceinline__ void load_pair(const Pos& ps, const DevState& S, int k,
                                           uint32_t j, PairIn<DIM>& p) {
   ps.pos(k, j, p.x);
-#pragma unroll
-  for (int q = 0; q < DIM; ++q) p.v[q] = static_cast<float>(S.v[q][j]);
-  p.rho = S.rho[j];
+  p.pv = S.pv[j];
   p.aux = S.aux[j];
 }
 
@@ -119,7 +116,7 @@ __device__ __forceinline__ void sum_pair(const DevParams& P, const Target<DIM>& 
   double d[DIM];
 #pragma unroll
   for (int q = 0; q < DIM; ++q) d[q] = p.x[q] - a.x[q];
-  pair_term<DIM, KERNEL, FLAVOUR>(P, a, d, p.v, p.rho, p.aux.x, p.aux.y, acc);
+  pair_term<DIM, KERNEL, FLAVOUR>(P, a, d, p.pv, p.aux.x, p.aux.y, acc);
 }
 
 template <int DIM>
@@ -396,6 +393,7 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
       }
       A.out.rho[i] = A.in.rho[i];
       A.out.aux[i] = A.in.aux[i];
+      A.out.pv[i] = A.in.pv[i];
     }
   }
   if (fluid) {
@@ -455,7 +453,7 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
         A.out.v[k][i] = vn[k];
       }
       A.out.rho[i] = rhon;
-      A.out.aux[i] = make_aux(P, rhon);
+      write_pair_data<DIM>(P, A.out, i, vn, rhon);
       if constexpr (FINAL) dtmin = dbl_bits(particle_time_step<DIM>(P, rhon, vn, dv));
     }
   }

@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
       vm = static_cast<float>(sqrt(fmax(s0, s1))) * (1.0f + 1e-6f);
       const double rho = st.rho[i] + T * (static_cast<double>(ui[3]) * sc.out_scale[3]);
       st.rho[i] = rho;
-      st.aux[i] = make_aux(P, rho);
+      write_pair_data<DIM>(P, st, i, v1, rho);
       cand = dbl_bits(particle_time_step<DIM>(P, rho, v1, a));
     }
   }
@@ -1083,6 +1083,7 @@ hipError_t alloc_snapshot(tit_hip_ctx* c, Snapshot& S) {
     }
     TIT_TRY_E(lalloc(m, &S.s.rho, c->n));
     TIT_TRY_E(lalloc(m, &S.s.aux, c->n));
+    TIT_TRY_E(lalloc(m, &S.s.pv, c->n));
     TIT_TRY_E(lalloc(m, &S.id, c->n));
     TIT_TRY_E(lalloc(m, &S.tag, c->n));
     TIT_TRY_E(lalloc(m, &S.key, c->n));
@@ -1123,6 +1124,7 @@ hipError_t copy_state(tit_hip_ctx* c, Snapshot& S, bool to_snapshot) {
   }
   TIT_TRY_E(cp(c->A.rho, S.s.rho, n));
   TIT_TRY_E(cp(c->A.aux, S.s.aux, n));
+  TIT_TRY_E(cp(c->A.pv, S.s.pv, n));
   TIT_TRY_E(cp(c->id, S.id, n));
   TIT_TRY_E(cp(c->tag, S.tag, n));
   TIT_TRY_E(cp(c->key, S.key, n));
