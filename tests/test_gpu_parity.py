@@ -130,7 +130,9 @@ def test_dam_break_steps_vs_oracle(order):
     for s in range(50):
         dt_g = ctx.step(1)
         dt_o = oracle.ssprk_step(prm, st, order)
-        assert abs(dt_g - dt_o) <= 1e-9 * dt_o, (s, dt_g, dt_o)
+        # the force criterion sqrt(h / |a|) takes the acceleration from FP32
+        # pair terms (each rounded to ~6e-8), halved by the square root
+        assert abs(dt_g - dt_o) <= 1e-7 * dt_o, (s, dt_g, dt_o)
     out = ctx.download()
     rho_err = np.max(np.abs(out["rho"][:nf] - st.rho[:nf]) / st.rho[:nf])
     p_o = oracle.pressure(prm, st.rho[:nf])

@@ -195,6 +195,16 @@ tit_status fill_params(tit_hip_ctx* c, const tit_hip_params& p) {
   d.inv_rho0_f = static_cast<float>(1.0 / p.rho0);
   d.cs0_f = static_cast<float>(p.cs0);
   d.rho0_f = static_cast<float>(p.rho0);
+  const double gsc = c_w * omega / std::pow(p.h, p.dim + 2);
+  d.half_inv_h = static_cast<float>(0.5 / p.h);
+  d.mg = static_cast<float>(p.m * gsc);
+  if (p.flavour == TIT_FLAVOUR_CFG) {
+    d.kd = static_cast<float>(2.0 * p.delta * p.h * p.cs0 * gsc);
+    d.kv = static_cast<float>(0.5 * p.alpha * p.h * p.m * gsc);
+  } else {
+    d.kd = static_cast<float>(gsc);
+    d.kv = static_cast<float>(2.0 * p.mu * p.m * gsc);
+  }
   return TIT_OK;
 }
 

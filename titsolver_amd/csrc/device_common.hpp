@@ -42,6 +42,11 @@ struct DevParams {
   float inv_rho0_f; // 1 / rho0
   float cs0_f;      // cs0
   float rho0_f;     // rho0
+  // factors folded into the pair terms
+  float half_inv_h; // 1 / (2h)
+  float mg;         // m gsc
+  float kd;         // CFG: 2 delta h cs0 gsc; REF: gsc
+  float kv;         // CFG: alpha h m gsc / 2; REF: 2 mu m gsc
 };
 
 // Per-tank cell grid.  Cell size is at least (support radius + skin) / 2, so
@@ -151,7 +156,6 @@ __device__ __forceinline__ double eos_density_from_potential(const DevParams& P,
   return P.rho0 * pow(1.0 + xi_1 * H / (P.cs0 * P.cs0), 1.0 / xi_1);
 }
 
-// ---- compensated accumulation ----------------------------------------------
 // FluidEquations::compute_time_step for one fluid particle
 // (source/tit/sph/fluid_equations.hpp:199-222).
 template <int DIM>
@@ -173,6 +177,7 @@ __device__ __forceinline__ double particle_time_step(const DevParams& P,
   return dt;
 }
 
+// ---- compensated accumulation ----------------------------------------------
 struct Kahan {
   float s, c;
   __device__ __forceinline__ void init() {

@@ -11,6 +11,8 @@
 //     CFG  Pi_ab = alpha h cbar (v_ab . r_ab) / (rhobar (|r_ab|^2 + 0.01 h^2))
 //          for approaching pairs (Monaghan 1992), else 0
 //     REF  Pi_ab = 2 mu (v_ab . r_ab) / (rho_a rho_b |r_ab|^2)   (:299-300)
+// The uniform factors m gsc, kd, kv are folded in once (FP64, then rounded),
+// so a pair costs the arithmetic that varies per pair only.
 #pragma once
 
 #include "internal.hpp"
@@ -32,22 +34,58 @@ __device__ __forceinline__ float sound_speed_f32(const DevParams& P, float rho) 
   return P.cs0_f * __builtin_powf(ratio, 0.5f * (static_cast<float>(P.xi) - 1.0f));
 }
 
+// Per-target values of the pair terms, with the uniform factors folded in
+// (DevParams::mg, kd, kv).
 template <int DIM>
 struct Target {
   double x[DIM];
   float v[DIM];
-  double rho;
-  float drho_f, rho_f, Pa, csa;  // rho - rho0, rho (FP32)
+  float kd;  // kd (rho_a - rho0)
+  float rb;  // CFG: (rho_a + rho0) / 2; REF: rho_a
+  float cv;  // CFG: kv cs_a; REF: cs_a
+  float pa;  // mg p_a / rho_a^2
 };
 
+// Compensated per-particle sums.  Pair terms enter in twos: the FP32 sum of
+// two consecutive terms is added with Kahan compensation, so every path that
+// visits the same candidates in the same order produces the same bits.
 template <int DIM>
 struct Sums {
   Kahan rho;
   Kahan v[DIM];
+  float p_rho, p_v[DIM];  // a pending single term
+  bool pend;
   __device__ __forceinline__ void init() {
     rho.init();
 #pragma unroll
     for (int k = 0; k < DIM; ++k) v[k].init();
+    pend = false;
+  }
+  // two terms at once (no term may be pending)
+  __device__ __forceinline__ void add2(float c0, const float* f0, float c1, const float* f1) {
+    rho.add(c0 + c1);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) v[k].add(f0[k] + f1[k]);
+  }
+  __device__ __forceinline__ void add(float c, const float* f) {
+    if (pend) {
+      rho.add(p_rho + c);
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) v[k].add(p_v[k] + f[k]);
+    } else {
+      p_rho = c;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) p_v[k] = f[k];
+    }
+    pend = !pend;
+  }
+  // a last single term enters as its sum with zero, as in add2
+  __device__ __forceinline__ void finish() {
+    if (pend) {
+      const float z[DIM] = {};
+      add2(p_rho, p_v, 0.0f, z);
+      pend = false;
+    }
   }
 };
 
@@ -65,11 +103,13 @@ __device__ __forceinline__ float2 make_aux(const DevParams& P, double rho) {
                      sound_speed_f32(P, static_cast<float>(rho)));
 }
 
-// (v, rho - rho0) in FP32 for the neighbours' side of the pair terms.
+// (v, rho - rho0) in FP32 for the neighbours' side of the pair terms; in 2D
+// the third slot holds the particle volume m / rho instead of v_z.
 template <int DIM>
 __device__ __forceinline__ float4 make_pv(const DevParams& P, const double* v, double rho) {
   return make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
-                     DIM == 3 ? static_cast<float>(v[DIM - 1]) : 0.0f,
+                     DIM == 3 ? static_cast<float>(v[DIM - 1])
+                              : static_cast<float>(P.m / rho),
                      static_cast<float>(rho - P.rho0));
 }
 
@@ -81,7 +121,7 @@ __device__ __forceinline__ void write_pair_data(const DevParams& P, const DevSta
   st.pv[i] = make_pv<DIM>(P, v, rho);
 }
 
-template <int DIM>
+template <int DIM, int FLAVOUR>
 __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
                                             const double* v, double rho,
                                             float2 aux, Target<DIM>& a) {
@@ -90,22 +130,42 @@ __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
     a.x[k] = x[k];
     a.v[k] = static_cast<float>(v[k]);
   }
-  a.rho = rho;
-  a.drho_f = static_cast<float>(rho - P.rho0);
-  a.rho_f = P.rho0_f + a.drho_f;
-  a.Pa = aux.x;
-  a.csa = aux.y;
+  const float drho = static_cast<float>(rho - P.rho0);
+  a.kd = P.kd * drho;
+  if constexpr (FLAVOUR == kFlavourCfg) {
+    a.rb = 0.5f * (P.rho0_f + drho) + 0.5f * P.rho0_f;
+    a.cv = P.kv * aux.y;
+  } else {
+    a.rb = P.rho0_f + drho;
+    a.cv = aux.y;
+  }
+  a.pa = P.mg * aux.x;
 }
 
-// d = r_b - r_a in FP64 (as in the neighbour predicate); pvb = (v_b,
-// rho_b - rho0), Pb = p_b/rho_b^2, csb = cs(rho_b) from the neighbour's
-// pair records.
+// The kernel-gradient polynomial P(q) of grad W_ab = gsc P(q) r_ab as a
+// function of r = |r_ab| (device_common.hpp: kernel_grad_poly).
+template <int KERNEL>
+__device__ __forceinline__ float grad_poly_r(const DevParams& P, float r) {
+  if constexpr (KERNEL == kKernelC2) {
+    const float t = fmaxf(__builtin_fmaf(r, -P.half_inv_h, 1.0f), 0.0f);
+    return t * t * t;
+  } else {
+    const float q = r * P.inv_h;
+    const float t = fmaxf(__builtin_fmaf(q, -0.5f, 1.0f), 0.0f);
+    const float t2 = t * t;
+    return __builtin_fmaf(2.5f, q, 1.0f) * (t2 * t2 * t);
+  }
+}
+
+// One pair's continuity and momentum contributions.  d = r_b - r_a in FP64
+// (as in the neighbour predicate); pvb = (v_b, rho_b - rho0) with m / rho_b
+// in the third slot in 2D; Pb = p_b / rho_b^2, csb = cs(rho_b).
 template <int DIM, int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>& a,
                                           const double* d, const float4& pvb, float Pb,
-                                          float csb, Sums<DIM>& s) {
-  const float vb[3] = {pvb.x, pvb.y, pvb.z};
-  float x[DIM];
+                                          float csb, float& cont, float* mom) {
+  const float vb[3] = {pvb.x, pvb.y, DIM == 3 ? pvb.z : 0.0f};
+  float x[DIM];  // r_a - r_b
 #pragma unroll
   for (int k = 0; k < DIM; ++k) x[k] = static_cast<float>(-d[k]);
   float r2 = x[0] * x[0];
@@ -114,30 +174,27 @@ __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>&
   // hardware square root and reciprocal (about 1 ulp; the pair terms are
   // FP32 and compared with the FP64 oracle at a stated tolerance)
   const float r = __builtin_amdgcn_sqrtf(r2);
-  const float q = r * P.inv_h;
-  const float G = P.gsc * kernel_grad_poly<KERNEL>(q);
+  const float G = grad_poly_r<KERNEL>(P, r);
   float vr = 0.0f;
 #pragma unroll
   for (int k = 0; k < DIM; ++k) vr = __builtin_fmaf(a.v[k] - vb[k], x[k], vr);
-  const float drab = a.drho_f - pvb.w;
-  const float rbf = P.rho0_f + pvb.w;
-  const float Vb = P.m_f * __builtin_amdgcn_rcpf(rbf);
-  float cont, Pi;
+  const float Vb = DIM == 2 ? pvb.z : P.m_f * __builtin_amdgcn_rcpf(P.rho0_f + pvb.w);
+  const float drk = __builtin_fmaf(pvb.w, -P.kd, a.kd);  // kd (rho_a - rho_b)
+  float c, Pi;
   if constexpr (FLAVOUR == kFlavourCfg) {
-    cont = G * __builtin_fmaf(P.m_f, vr, P.dhc2 * Vb * drab);
-    const float cbar = 0.5f * (a.csa + csb);
-    const float rbar = 0.5f * (a.rho_f + rbf);
-    const float visc = P.ahc * cbar * vr * __builtin_amdgcn_rcpf(rbar * (r2 + P.eta2));
-    Pi = vr < 0.0f ? visc : 0.0f;
+    c = __builtin_fmaf(P.mg, vr, drk * Vb);
+    const float rbar = __builtin_fmaf(pvb.w, 0.5f, a.rb);
+    const float num = __builtin_fmaf(csb, P.kv, a.cv) * fminf(vr, 0.0f);
+    Pi = num * __builtin_amdgcn_rcpf(rbar * (r2 + P.eta2));
   } else {
-    const float csab = fmaxf(a.csa, csb);
-    cont = G * __builtin_fmaf(P.m_f, vr, csab * drab * Vb * r);
-    Pi = P.two_mu * vr * __builtin_amdgcn_rcpf(a.rho_f * rbf * r2);
+    c = __builtin_fmaf(P.mg, vr, fmaxf(a.cv, csb) * drk * Vb * r);
+    const float rb = P.rho0_f + pvb.w;
+    Pi = (P.kv * vr) * __builtin_amdgcn_rcpf(a.rb * rb * r2);
   }
-  const float coef = P.m_f * (Pi - (a.Pa + Pb)) * G;
-  s.rho.add(cont);
+  cont = c * G;
+  const float coef = __builtin_fmaf(Pb, -P.mg, Pi - a.pa) * G;
 #pragma unroll
-  for (int k = 0; k < DIM; ++k) s.v[k].add(coef * x[k]);
+  for (int k = 0; k < DIM; ++k) mom[k] = coef * x[k];
 }
 
 }  // namespace tit_amd

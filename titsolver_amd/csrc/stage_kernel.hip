@@ -11,13 +11,15 @@
 // 2D: the FP64 positions of the chunk's candidates (the 5 neighbouring
 // columns, the chunk's rows +- 2) are staged in LDS.  Each lane walks its own
 // 5 x 5 cell stencil one column range at a time: the exact FP64 predicate
-// (four LDS loads per batch) builds a 64-bit acceptance mask, and the FP32
-// pair terms run over the accepted candidates only, reading their velocity,
-// density and (p/rho^2, cs) from L2 one candidate ahead of the arithmetic.
+// (four LDS loads per batch) builds a 64-bit acceptance mask per batch, kept
+// in LDS; then the FP32 pair terms run over the accepted candidates of all
+// batches in one loop, reading their pair records (velocity, m/rho,
+// rho - rho0 and p/rho^2, cs) from L2 one candidate ahead of the arithmetic.
 // Chunks whose candidates exceed the LDS capacity, and 3D, take positions
 // from global memory too.  Every path visits candidates in the same order and
 // rounds identically, so all schedules produce the same bits.
 #include <algorithm>
+#include <type_traits>
 
 #include "pair_math.hpp"
 #include "sph_kernels.hpp"
@@ -27,7 +29,10 @@ namespace tit_amd {
 namespace {
 
 constexpr int kCols2 = 2 * kStencil + 1;
-constexpr int kRowsMax = 128;  // LDS cell-offset rows of one chunk
+constexpr int kRowsMax = 96;   // LDS cell-offset rows of one chunk
+constexpr int kWords = 5;      // acceptance masks held per lane (LDS path)
+constexpr uint32_t kWordSpan = 1u << 22;  // candidate global-index span of an LDS chunk
+static_assert(kStageLdsCap <= 1024, "LDS index packs into 10 bits");
 
 // Positions from LDS (2D).
 struct LdsPos {
@@ -36,6 +41,9 @@ struct LdsPos {
   const uint32_t* gb;                    // global index of each column's row zlo
   const uint32_t* lb;                    // LDS index of each column's row zlo
   int zlo, nz, ix, ncx;
+  uint32_t gbase;   // global index of the chunk's first candidate
+  uint64_t* wmask;  // this wave's [kWords][64] acceptance masks
+  uint32_t* wjl;    // their bit 0: global index - gbase (bits 10..31), LDS index
 
   // Candidates of column k in rows [w0, w1] (inside the target's 5-row window).
   __device__ __forceinline__ bool range(int k, int w0, int w1, uint32_t& b,
@@ -111,12 +119,20 @@ __device__ __forceinline__ void load_pair(const Pos& ps, const DevState& S, int 
 }
 
 template <int DIM, int KERNEL, int FLAVOUR>
-__device__ __forceinline__ void sum_pair(const DevParams& P, const Target<DIM>& a,
-                                         const PairIn<DIM>& p, Sums<DIM>& acc) {
+__device__ __forceinline__ void term_of(const DevParams& P, const Target<DIM>& a,
+                                        const PairIn<DIM>& p, float& c, float* f) {
   double d[DIM];
 #pragma unroll
   for (int q = 0; q < DIM; ++q) d[q] = p.x[q] - a.x[q];
-  pair_term<DIM, KERNEL, FLAVOUR>(P, a, d, p.pv, p.aux.x, p.aux.y, acc);
+  pair_term<DIM, KERNEL, FLAVOUR>(P, a, d, p.pv, p.aux.x, p.aux.y, c, f);
+}
+
+template <int DIM, int KERNEL, int FLAVOUR>
+__device__ __forceinline__ void sum_pair(const DevParams& P, const Target<DIM>& a,
+                                         const PairIn<DIM>& p, Sums<DIM>& acc) {
+  float c, f[DIM];
+  term_of<DIM, KERNEL, FLAVOUR>(P, a, p, c, f);
+  acc.add(c, f);
 }
 
 template <int DIM>
@@ -301,6 +317,133 @@ __device__ __forceinline__ void gather(const DevParams& P, const Pos& ps,
   }
 }
 
+// 2D with LDS positions.  The acceptance masks of up to kWords candidate
+// batches (normally one per stencil column) are kept per lane in LDS and
+// their accepted candidates are summed in one loop, so a wave's pair loop
+// runs for the largest accepted count among its lanes, not for the sum over
+// columns of each column's largest count.  Candidates are visited in the
+// same order as in gather(), so the sums are identical.
+template <int KERNEL, int FLAVOUR>
+__device__ __forceinline__ void sum_words(const DevParams& P, const LdsPos& ps,
+                                          const DevState& S, const Target<2>& a, int nw,
+                                          Sums<2>& acc) {
+  const int lane = threadIdx.x & 63;
+  int w = 0;
+  uint64_t m = 0, m_next = 0;
+  uint32_t l0 = 0, j0 = 0, jl_next = 0;
+  if (nw > 0) {  // the next word is read one word ahead
+    m_next = ps.wmask[lane];
+    jl_next = ps.wjl[lane];
+  }
+  auto next = [&](PairIn<2>& p) -> bool {
+    while (m == 0) {
+      if (w >= nw) return false;
+      m = m_next;
+      j0 = ps.gbase + (jl_next >> 10);
+      l0 = jl_next & 1023u;
+      ++w;
+      if (w < nw) {
+        m_next = ps.wmask[w * 64 + lane];
+        jl_next = ps.wjl[w * 64 + lane];
+      }
+    }
+    const uint32_t t = pop_lowest(m);
+    const double2 x = ps.xy[l0 + t];
+    p.x[0] = x.x;
+    p.x[1] = x.y;
+    p.pv = S.pv[j0 + t];
+    p.aux = S.aux[j0 + t];
+    return true;
+  };
+  PairIn<2> A, B;
+  if (acc.pend) {  // an odd term of an earlier batch of words waits for its partner
+    if (!next(A)) return;
+    sum_pair<2, KERNEL, FLAVOUR>(P, a, A, acc);
+  }
+  // two register sets alternate, as in gather(); their terms enter the
+  // compensated sums together
+  if (!next(A)) return;
+  while (true) {
+    const bool hb = next(B);
+    float ca, fa[2];
+    term_of<2, KERNEL, FLAVOUR>(P, a, A, ca, fa);
+    if (!hb) {
+      acc.add(ca, fa);
+      break;
+    }
+    const bool ha = next(A);
+    float cb, fb[2];
+    term_of<2, KERNEL, FLAVOUR>(P, a, B, cb, fb);
+    acc.add2(ca, fa, cb, fb);
+    if (!ha) break;
+  }
+}
+
+template <int KERNEL, int FLAVOUR, int NCOL>
+__device__ __forceinline__ void gather_words(const DevParams& P, const LdsPos& ps,
+                                             const DevState& S, const Target<2>& a,
+                                             uint32_t i, int zc, const Windows<2, NCOL>& W,
+                                             Sums<2>& acc, uint32_t& count) {
+  const int lane = threadIdx.x & 63;
+  int k = 0, kc = 0;
+  uint32_t c0 = 0, e = 0;
+  bool more = true;
+  while (more) {
+    int nw = 0;
+    while (nw < kWords) {
+      if (c0 >= e) {  // next non-empty column range
+        bool found = false;
+        for (; k < NCOL && !found; ++k) {
+          const uint32_t v = W.get(k);
+          if ((v & 7u) == 7u) continue;
+          const int w0 = zc - kStencil + static_cast<int>(v & 7u);
+          const int w1 = zc - kStencil + static_cast<int>(v >> 3);
+          uint32_t b, eb;
+          if (ps.range(k, w0, w1, b, eb) && b < eb) {
+            found = true;
+            kc = k;
+            c0 = b;
+            e = eb;
+          }
+        }
+        if (!found) {
+          more = false;
+          break;
+        }
+      }
+      const uint32_t len = min(e - c0, 64u);
+      const uint32_t lc = ps.lb[kc] + (c0 - ps.gb[kc]);
+      uint64_t mask = 0;
+      uint32_t t = 0;
+      for (; t + 4 <= len; t += 4) {
+        const double2 p0 = ps.xy[lc + t], p1 = ps.xy[lc + t + 1];
+        const double2 p2 = ps.xy[lc + t + 2], p3 = ps.xy[lc + t + 3];
+        const double x0[2] = {p0.x, p0.y}, x1[2] = {p1.x, p1.y};
+        const double x2[2] = {p2.x, p2.y}, x3[2] = {p3.x, p3.y};
+        const uint64_t mm = (within<2>(P, x0, a.x) ? 1ull : 0ull) |
+                            (within<2>(P, x1, a.x) ? 2ull : 0ull) |
+                            (within<2>(P, x2, a.x) ? 4ull : 0ull) |
+                            (within<2>(P, x3, a.x) ? 8ull : 0ull);
+        mask |= mm << t;
+      }
+      for (; t < len; ++t) {
+        const double2 p0 = ps.xy[lc + t];
+        const double x0[2] = {p0.x, p0.y};
+        if (within<2>(P, x0, a.x)) mask |= 1ull << t;
+      }
+      if (i >= c0 && i < c0 + len) mask &= ~(1ull << (i - c0));  // not self
+      count += static_cast<uint32_t>(__builtin_popcountll(mask));
+      if (mask) {
+        ps.wmask[nw * 64 + lane] = mask;
+        ps.wjl[nw * 64 + lane] = ((c0 - ps.gbase) << 10) | lc;
+        ++nw;
+      }
+      c0 += len;
+    }
+    sum_words<KERNEL, FLAVOUR>(P, ps, S, a, nw, acc);
+  }
+}
+
 __device__ __forceinline__ float wave_max_f32(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
@@ -405,14 +548,19 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
     }
     const double rho_a = A.in.rho[i];
     Target<DIM> a;
-    make_target<DIM>(P, xa, va, rho_a, A.in.aux[i], a);
+    make_target<DIM, FLAVOUR>(P, xa, va, rho_a, A.in.aux[i], a);
     const int zc = static_cast<int>(A.key[i] - c.col_row);
     Sums<DIM> acc;
     acc.init();
     const Windows<DIM, NCOL> W =
         stencil_windows<DIM, NCOL>(make_trim<DIM>(P, c.T, xa, c.cx, c.cy, zc), c.cx, c.cy, zc);
     uint32_t count = 0;
-    gather<DIM, KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, W, acc, count);
+    if constexpr (std::is_same_v<Pos, LdsPos>) {
+      gather_words<KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, W, acc, count);
+    } else {
+      gather<DIM, KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, W, acc, count);
+    }
+    acc.finish();
     if constexpr (KIND == kRhsOnly) {
       if (A.rhs_count) A.rhs_count[i] = count;
     }
@@ -486,6 +634,9 @@ __global__ __launch_bounds__(NT) void k_stage_lds(StageArgs A) {
   __shared__ uint32_t s_coff[kCols2][kRowsMax + 1];
   __shared__ uint32_t s_gb[kCols2], s_lb[kCols2 + 1];
   __shared__ int s_fits;
+  __shared__ uint64_t s_wmask[NT / 64][kWords * 64];
+  __shared__ uint32_t s_wjl[NT / 64][kWords * 64];
+  __shared__ uint32_t s_gbase;
   const int tid = threadIdx.x;
   const Band band = xcd_band(A.counts[0]);
   WaveReduce red;
@@ -509,16 +660,22 @@ __global__ __launch_bounds__(NT) void k_stage_lds(StageArgs A) {
     }
     __syncthreads();
     if (tid == 0) {
-      uint32_t lb = 0;
+      uint32_t lb = 0, g0 = ~0u, g1 = 0;
       if (rows_fit) {
         for (int k = 0; k < kCols2; ++k) {
           s_gb[k] = s_coff[k][0];
           s_lb[k] = lb;
           lb += s_coff[k][nrows] - s_coff[k][0];
+          const int ixk = c.cx - kStencil + k;
+          if (ixk >= 0 && ixk < c.T.nc[0]) {
+            g0 = min(g0, s_coff[k][0]);
+            g1 = max(g1, s_coff[k][nrows]);
+          }
         }
         s_lb[kCols2] = lb;
+        s_gbase = g0;
       }
-      s_fits = rows_fit && lb <= A.lds_cap;
+      s_fits = rows_fit && lb <= A.lds_cap && g1 - g0 < kWordSpan;
       if (!s_fits) A.overflow[atomicAdd(A.counts + 1, 1u)] = q;
     }
     __syncthreads();
@@ -531,7 +688,8 @@ __global__ __launch_bounds__(NT) void k_stage_lds(StageArgs A) {
       s_xy[l] = make_double2(A.in.x[0][j], A.in.x[1][j]);
     }
     __syncthreads();
-    const LdsPos ps{s_xy, s_coff, s_gb, s_lb, zlo, c.nz, c.cx, c.T.nc[0]};
+    const LdsPos ps{s_xy,   s_coff,   s_gb,      s_lb,           zlo,           c.nz,
+                    c.cx,   c.T.nc[0], s_gbase,  s_wmask[tid >> 6], s_wjl[tid >> 6]};
     chunk_targets<2, KERNEL, FLAVOUR, KIND, FINAL>(A, c, ps, red);
   }
   red.flush(A);
