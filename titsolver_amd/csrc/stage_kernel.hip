@@ -325,27 +325,22 @@ __device__ __forceinline__ void gather(const DevParams& P, const Pos& ps,
 // same order as in gather(), so the sums are identical.
 template <int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void sum_words(const DevParams& P, const LdsPos& ps,
-                                          const DevState& S, const Target<2>& a, int nw,
-                                          Sums<2>& acc) {
+                                          const DevState& S, const Target<2>& a,
+                                          uint32_t n_acc, Sums<2>& acc) {
   const int lane = threadIdx.x & 63;
   int w = 0;
-  uint64_t m = 0, m_next = 0;
-  uint32_t l0 = 0, j0 = 0, jl_next = 0;
-  if (nw > 0) {  // the next word is read one word ahead
-    m_next = ps.wmask[lane];
-    jl_next = ps.wjl[lane];
-  }
-  auto next = [&](PairIn<2>& p) -> bool {
-    while (m == 0) {
-      if (w >= nw) return false;
+  uint64_t m = 0, m_next = ps.wmask[lane];  // the next word is read one word ahead
+  uint32_t l0 = 0, j0 = 0, jl_next = ps.wjl[lane];
+  // The next of the n_acc accepted candidates (stored words are never
+  // empty; slots past the last stored word are read but never used).
+  auto take = [&](PairIn<2>& p) {
+    if (m == 0) {
       m = m_next;
       j0 = ps.gbase + (jl_next >> 10);
       l0 = jl_next & 1023u;
-      ++w;
-      if (w < nw) {
-        m_next = ps.wmask[w * 64 + lane];
-        jl_next = ps.wjl[w * 64 + lane];
-      }
+      w = min(w + 1, kWords - 1);
+      m_next = ps.wmask[w * 64 + lane];
+      jl_next = ps.wjl[w * 64 + lane];
     }
     const uint32_t t = pop_lowest(m);
     const double2 x = ps.xy[l0 + t];
@@ -353,25 +348,36 @@ __device__ __forceinline__ void sum_words(const DevParams& P, const LdsPos& ps,
     p.x[1] = x.y;
     p.pv = S.pv[j0 + t];
     p.aux = S.aux[j0 + t];
-    return true;
   };
+  uint32_t rem = n_acc;
   PairIn<2> A, B;
+  if (rem == 0) return;
   if (acc.pend) {  // an odd term of an earlier batch of words waits for its partner
-    if (!next(A)) return;
+    take(A);
     sum_pair<2, KERNEL, FLAVOUR>(P, a, A, acc);
+    if (--rem == 0) return;
   }
   // two register sets alternate, as in gather(); their terms enter the
   // compensated sums together
-  if (!next(A)) return;
+  take(A);
+  --rem;
   while (true) {
-    const bool hb = next(B);
+    const bool hb = rem > 0;
+    if (hb) {
+      take(B);
+      --rem;
+    }
     float ca, fa[2];
     term_of<2, KERNEL, FLAVOUR>(P, a, A, ca, fa);
     if (!hb) {
       acc.add(ca, fa);
       break;
     }
-    const bool ha = next(A);
+    const bool ha = rem > 0;
+    if (ha) {
+      take(A);
+      --rem;
+    }
     float cb, fb[2];
     term_of<2, KERNEL, FLAVOUR>(P, a, B, cb, fb);
     acc.add2(ca, fa, cb, fb);
@@ -390,6 +396,7 @@ __device__ __forceinline__ void gather_words(const DevParams& P, const LdsPos& p
   bool more = true;
   while (more) {
     int nw = 0;
+    uint32_t n_acc = 0;
     while (nw < kWords) {
       if (c0 >= e) {  // next non-empty column range
         bool found = false;
@@ -432,7 +439,9 @@ __device__ __forceinline__ void gather_words(const DevParams& P, const LdsPos& p
         if (within<2>(P, x0, a.x)) mask |= 1ull << t;
       }
       if (i >= c0 && i < c0 + len) mask &= ~(1ull << (i - c0));  // not self
-      count += static_cast<uint32_t>(__builtin_popcountll(mask));
+      const uint32_t na = static_cast<uint32_t>(__builtin_popcountll(mask));
+      count += na;
+      n_acc += na;
       if (mask) {
         ps.wmask[nw * 64 + lane] = mask;
         ps.wjl[nw * 64 + lane] = ((c0 - ps.gbase) << 10) | lc;
@@ -440,7 +449,7 @@ __device__ __forceinline__ void gather_words(const DevParams& P, const LdsPos& p
       }
       c0 += len;
     }
-    sum_words<KERNEL, FLAVOUR>(P, ps, S, a, nw, acc);
+    sum_words<KERNEL, FLAVOUR>(P, ps, S, a, n_acc, acc);
   }
 }
 
