@@ -192,7 +192,8 @@ void tit_hip_destroy(tit_hip_ctx* ctx);
 
 /* ---- learned step ----------------------------------------------------------
  * No reference counterpart: a message-passing network (edge MLP 80 -> 128
- * -> 128, node MLP 160 -> 128 -> 128 -> 16, SiLU, two rounds over the 8
+ * -> 128, node MLP 160 -> 128 -> 128 -> 16, hard-swish activation
+ * x min(max(x / 6 + 1/2, 0), 1), two rounds over the 8
  * nearest neighbours within h) predicts each fluid particle's acceleration
  * and density rate; one learned step advances 4 dt:
  *   v' = v + 4dt a,  r' = r + 4dt (v + v') / 2,  rho' = rho + 4dt drho/dt.
@@ -272,7 +273,7 @@ tit_status tit_hip_learned_run(tit_hip_ctx* ctx, int n_steps, int judge_every,
 /* The network's BF16 MFMA GEMM on the current device (test hook, no
  * context): A is M x K (K x M with transA), B is K x N (N x K with transB),
  * row-major host FP32, rounded to BF16; C (M x N) = act(op(A) op(B) + bias)
- * with act 0 (none) or 1 (SiLU), FP32 accumulation.  k_splits > 1 adds
+ * with act 0 (none) or 1 (hard-swish), FP32 accumulation.  k_splits > 1 adds
  * op(A) op(B) into C over that many K slices instead (no bias or act). */
 tit_status tit_hip_nn_gemm(int M, int N, int K, const float* A, int transA,
                            const float* B, int transB, const float* bias,

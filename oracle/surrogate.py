@@ -19,9 +19,10 @@ Definition (same parameter layout as the device code):
   (rho_b - rho_a) / (rho0 rho_s), valid flag, zeros (all zero for a missing
   edge);
 * per round r with H_r = [nf | u_{r-1}] (u_{-1} = 0):
-  x_e = [ef_e | H_a | H_b], m_e = silu(silu(x_e W1 + b1) W2 + b2),
+  x_e = [ef_e | H_a | H_b], m_e = act(act(x_e W1 + b1) W2 + b2),
   M_a = sum of the 8 m_e of a, y_a = [H_a | M_a],
-  u_a = silu(silu(y_a W3 + b3) W4 + b4) W5 + b5;
+  u_a = act(act(y_a W3 + b3) W4 + b4) W5 + b5, with the hard-swish
+  activation act(x) = x min(max(x / 6 + 1/2, 0), 1);
 * loss: mean over fluid particles and used channels (acceleration 0..dim-1
   and density rate 3) of (u_c - target_c / scale_c)^2.
 
@@ -29,7 +30,7 @@ Pins (tests/test_oracle_surrogate.py): the backward pass against central
 finite differences of the loss (FP64, relative error <= 1e-6); the graph
 against a plain O(N^2) neighbour search with ties broken by index; features
 and outputs against closed forms on hand-made configurations (zero weights
-give the bias, a single edge gives the SiLU composition, a lattice gives
+give the bias, a single path gives the activation's composition, a lattice gives
 8 neighbours at dr and sqrt(2) dr).
 """
 from __future__ import annotations
@@ -82,7 +83,8 @@ def pack(rounds: list[dict]) -> np.ndarray:
 
 # Planted mistakes for the tests that show the finite-difference check can
 # fail: 1 drops the neighbour (H_b) path of the latent gradient, 2 uses the
-# logistic function in place of SiLU's derivative, 3 drops the bias gradients.
+# gate min(max(x / 6 + 1/2, 0), 1) in place of the activation's derivative
+# (the product rule's second term dropped), 3 drops the bias gradients.
 _mutation = 0
 
 
@@ -91,15 +93,18 @@ def set_mutation(m: int):
     _mutation = int(m)
 
 
-def silu(x):
-    return x / (1.0 + np.exp(-x))
+def act(x):
+    """Hard-swish x min(max(x / 6 + 1/2, 0), 1)."""
+    return x * np.clip(x / 6.0 + 0.5, 0.0, 1.0)
 
 
-def silu_grad(x):
-    s = 1.0 / (1.0 + np.exp(-x))
+def act_grad(x):
+    """Derivative of act: 0 below -3, x / 3 + 1/2 on [-3, 3], 1 above 3."""
+    gate = np.clip(x / 6.0 + 0.5, 0.0, 1.0)
     if _mutation == 2:
-        return s
-    return s * (1.0 + x * (1.0 - s))
+        return gate
+    inner = (x > -3.0) & (x < 3.0)
+    return gate + np.where(inner, x / 6.0, 0.0)
 
 
 # ---- graph and features --------------------------------------------------------
@@ -188,15 +193,15 @@ def forward(w, nf, ef, nbr):
         p = P[r]
         X = np.concatenate([ef, H[a], H[b]], axis=1)
         z1 = X @ p["W1"] + p["b1"]
-        h1 = silu(z1)
+        h1 = act(z1)
         z2 = h1 @ p["W2"] + p["b2"]
-        m = silu(z2)
+        m = act(z2)
         M = m.reshape(n, N_KNN, HIDDEN).sum(axis=1)
         Y = np.concatenate([H, M], axis=1)
         z3 = Y @ p["W3"] + p["b3"]
-        h3 = silu(z3)
+        h3 = act(z3)
         z4 = h3 @ p["W4"] + p["b4"]
-        h4 = silu(z4)
+        h4 = act(z4)
         u = h4 @ p["W5"] + p["b5"]
         cache.append(dict(X=X, z1=z1, h1=h1, z2=z2, Y=Y, z3=z3, h3=h3, z4=z4, h4=h4))
         H = np.concatenate([nf, u], axis=1)
@@ -228,18 +233,18 @@ def backward(cache, du):
         g = {}
         g["W5"] = c["h4"].T @ du
         g["b5"] = du.sum(axis=0)
-        dz4 = (du @ p["W5"].T) * silu_grad(c["z4"])
+        dz4 = (du @ p["W5"].T) * act_grad(c["z4"])
         g["W4"] = c["h3"].T @ dz4
         g["b4"] = dz4.sum(axis=0)
-        dz3 = (dz4 @ p["W4"].T) * silu_grad(c["z3"])
+        dz3 = (dz4 @ p["W4"].T) * act_grad(c["z3"])
         g["W3"] = c["Y"].T @ dz3
         g["b3"] = dz3.sum(axis=0)
         dY = dz3 @ p["W3"].T
         dM = dY[:, NODE_IN:]
-        dz2 = np.repeat(dM, N_KNN, axis=0) * silu_grad(c["z2"])
+        dz2 = np.repeat(dM, N_KNN, axis=0) * act_grad(c["z2"])
         g["W2"] = c["h1"].T @ dz2
         g["b2"] = dz2.sum(axis=0)
-        dz1 = (dz2 @ p["W2"].T) * silu_grad(c["z1"])
+        dz1 = (dz2 @ p["W2"].T) * act_grad(c["z1"])
         g["W1"] = c["X"].T @ dz1
         g["b1"] = dz1.sum(axis=0)
         if _mutation == 3:

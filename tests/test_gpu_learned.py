@@ -21,8 +21,9 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
-def silu(x):
-    return x / (1.0 + np.exp(-x))
+def hswish(x):
+    """Hard-swish, the network's activation."""
+    return x * np.clip(x / 6.0 + 0.5, 0.0, 1.0)
 
 
 @pytest.mark.parametrize("M,N,K,tA,tB,bias,act,splits", [
@@ -46,7 +47,7 @@ def test_gemm_vs_numpy(M, N, K, tA, tB, bias, act, splits):
     if b is not None:
         ref = ref + b
     if act:
-        ref = silu(ref)
+        ref = hswish(ref)
     if C0 is not None:
         ref = ref + C0
     scale = np.sqrt(K) * max(1.0, np.abs(ref).max() / np.sqrt(K))

@@ -88,7 +88,7 @@ def test_zero_weights_give_output_bias():
 
 def test_single_path_closed_form():
     """One-hot weights route one input feature through the layers; the output
-    is the SiLU composition of that feature (plus the summed messages)."""
+    is the activation's composition of that feature (plus the summed messages)."""
     n = 3
     nf = np.zeros((n, S.NODE_FEAT))
     nf[:, 0] = [0.5, -1.0, 2.0]
@@ -100,8 +100,10 @@ def test_single_path_closed_form():
         p["W4"][0, 0] = 1.0
         p["W5"][0, 0] = 1.0
     u, _ = S.forward(S.pack(P), nf, ef, nbr)
-    # messages: silu(silu(0)) = 0 for zero edge weights, so M = 0
-    expect = S.silu(S.silu(nf[:, 0]))
+    # messages: act(act(0)) = 0 for zero edge weights, so M = 0; the
+    # hard-swish in closed form: 0.5 -> 0.5 * 0.5833.., -1 -> -1/3, 2 -> 5/3
+    expect = np.array([0.5 * (0.5 / 6 + 0.5), -1.0 * (-1.0 / 6 + 0.5), 2.0 * (2.0 / 6 + 0.5)])
+    expect = expect * np.clip(expect / 6 + 0.5, 0, 1)
     assert np.allclose(u[:, 0], expect, rtol=0, atol=1e-15)
     assert np.all(u[:, 1:] == 0)
 

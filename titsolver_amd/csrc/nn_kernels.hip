@@ -103,20 +103,20 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
       if (g.G1) x += g.G1[static_cast<size_t>(g.r1[m]) * g.N + n];
       if (g.G2) x += g.G2[static_cast<size_t>(g.r2[m]) * g.N + n];
       if (g.pre) g.pre[static_cast<size_t>(m) * g.ldpre + n] = x;
-      const float y = g.act == kActSilu ? nn_silu(x) : x;
+      const float y = g.act == kActHswish ? nn_act(x) : x;
       if (g.out32) g.out32[static_cast<size_t>(m) * g.ldo32 + n] = y;
       if (g.out16) g.out16[static_cast<size_t>(m) * g.ldo16 + n] = static_cast<bf16>(y);
     }
   }
 }
 
-__global__ void k_silu_backward(int M, int N, const float* dy, int ld_dy,
+__global__ void k_act_backward(int M, int N, const float* dy, int ld_dy,
                                 int group, const float* pre, float* dx32,
                                 bf16* dx16) {
   const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= static_cast<size_t>(M) * N) return;
   const size_t m = t / N, n = t % N;
-  const float g = dy[(m / group) * ld_dy + n] * nn_silu_grad(pre[t]);
+  const float g = dy[(m / group) * ld_dy + n] * nn_act_grad(pre[t]);
   if (dx32) dx32[t] = g;
   if (dx16) dx16[t] = static_cast<bf16>(g);
 }
@@ -205,11 +205,11 @@ void gemm_bf16(hipStream_t s, const GemmArgs& g) {
   k_gemm<<<grid, 256, 0, s>>>(g);
 }
 
-void silu_backward(hipStream_t s, int M, int N, const float* dy, int ld_dy,
+void act_backward(hipStream_t s, int M, int N, const float* dy, int ld_dy,
                    int group, const float* pre, float* dx32, bf16* dx16) {
   const size_t n = static_cast<size_t>(M) * N;
   if (n) {
-    k_silu_backward<<<blocks(n), 256, 0, s>>>(M, N, dy, ld_dy, group, pre, dx32,
+    k_act_backward<<<blocks(n), 256, 0, s>>>(M, N, dy, ld_dy, group, pre, dx32,
                                               dx16);
   }
 }

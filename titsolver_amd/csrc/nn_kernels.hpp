@@ -10,18 +10,19 @@ namespace tit_amd {
 
 typedef __bf16 bf16;
 
-enum NnAct { kActNone = 0, kActSilu = 1 };
+enum NnAct { kActNone = 0, kActHswish = 1 };
 
-// Logistic function 1 / (1 + e^-x) with the hardware exp2 and reciprocal
-// (about 1 ulp each); every SiLU of the network (training GEMM epilogues,
-// their derivative and the fused inference kernel) uses this one.
-__device__ __forceinline__ float nn_sigmoid(float x) {
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * x));
+// The network's activation, hard-swish x min(max(x / 6 + 1/2, 0), 1): three
+// full-rate VALU operations (an FMA, a clamp, a product), used by every
+// layer (training GEMM epilogues, the derivative, the fused inference kernel).
+__device__ __forceinline__ float nn_gate(float x) {
+  return __builtin_amdgcn_fmed3f(__builtin_fmaf(x, 1.0f / 6.0f, 0.5f), 0.0f, 1.0f);
 }
-__device__ __forceinline__ float nn_silu(float x) { return x * nn_sigmoid(x); }
-__device__ __forceinline__ float nn_silu_grad(float x) {
-  const float s = nn_sigmoid(x);
-  return s * (1.0f + x * (1.0f - s));
+__device__ __forceinline__ float nn_act(float x) { return x * nn_gate(x); }
+// 0 below -3, x / 3 + 1/2 on (-3, 3), 1 above 3
+__device__ __forceinline__ float nn_act_grad(float x) {
+  const float g = nn_gate(x);
+  return (x > -3.0f && x < 3.0f) ? __builtin_fmaf(x, 1.0f / 6.0f, g) : g;
 }
 
 // C[M x N] = act(op(A)[M x K] . op(B)[K x N] + bias + G1[r1[m]] + G2[r2[m]])
@@ -57,11 +58,11 @@ struct GemmArgs {
 
 void gemm_bf16(hipStream_t s, const GemmArgs& g);
 
-// dx = dy * silu'(pre) for an M x N block; dy row m is read from row
+// dx = dy * act'(pre) for an M x N block; dy row m is read from row
 // m / group (leading dimension ld_dy), so group > 1 broadcasts one upstream
 // row to `group` rows.  Writes FP32 (dx32) and/or BF16 (dx16) copies, both
 // M x N.  In place (dx32 == dy) is allowed when group == 1 and ld_dy == N.
-void silu_backward(hipStream_t s, int M, int N, const float* dy, int ld_dy,
+void act_backward(hipStream_t s, int M, int N, const float* dy, int ld_dy,
                    int group, const float* pre, float* dx32, bf16* dx16);
 // out[r][n] = sum_{k < group} in[r * group + k][n]; optional BF16 copy with
 // its own leading dimension and column offset.

@@ -729,7 +729,7 @@ void forward_round(hipStream_t s, Learned* L, Activations& A, int r, uint32_t n,
   g.B = W.w16 + RoundLayout::W1;
   g.ldb = kHidden;
   g.bias = W.w32 + RoundLayout::b1;
-  g.act = kActSilu;
+  g.act = kActHswish;
   g.pre = A.training ? A.pre1[r] : nullptr;
   g.ldpre = kHidden;
   g.out16 = A.a1[r];
@@ -745,7 +745,7 @@ void forward_round(hipStream_t s, Learned* L, Activations& A, int r, uint32_t n,
   g.B = W.w16 + RoundLayout::W2;
   g.ldb = kHidden;
   g.bias = W.w32 + RoundLayout::b2;
-  g.act = kActSilu;
+  g.act = kActHswish;
   g.pre = A.training ? A.pre2[r] : nullptr;
   g.ldpre = kHidden;
   g.out32 = A.msg;
@@ -765,7 +765,7 @@ void forward_round(hipStream_t s, Learned* L, Activations& A, int r, uint32_t n,
   g.B = W.w16 + RoundLayout::W3;
   g.ldb = kHidden;
   g.bias = W.w32 + RoundLayout::b3;
-  g.act = kActSilu;
+  g.act = kActHswish;
   g.pre = A.training ? A.pre3[r] : nullptr;
   g.ldpre = kHidden;
   g.out16 = A.a3[r];
@@ -850,23 +850,23 @@ void backward_round(hipStream_t s, Learned* L, Activations& A, int r, uint32_t n
   wgrad(s, kHidden, kOut, n, A.a4[r], A.du16, g + RoundLayout::W5);
   dgrad(s, n, kOut, kHidden, A.du16, W.w16 + RoundLayout::W5, A.dN);
   // node layer 2
-  silu_backward(s, static_cast<int>(n), kHidden, A.dN, kHidden, 1, A.pre4[r], A.dN, A.dN16);
+  act_backward(s, static_cast<int>(n), kHidden, A.dN, kHidden, 1, A.pre4[r], A.dN, A.dN16);
   column_sum(s, static_cast<int>(n), kHidden, A.dN, g + RoundLayout::b4);
   wgrad(s, kHidden, kHidden, n, A.a3[r], A.dN16, g + RoundLayout::W4);
   dgrad(s, n, kHidden, kHidden, A.dN16, W.w16 + RoundLayout::W4, A.dN);
   // node layer 1
-  silu_backward(s, static_cast<int>(n), kHidden, A.dN, kHidden, 1, A.pre3[r], A.dN, A.dN16);
+  act_backward(s, static_cast<int>(n), kHidden, A.dN, kHidden, 1, A.pre3[r], A.dN, A.dN16);
   column_sum(s, static_cast<int>(n), kHidden, A.dN, g + RoundLayout::b3);
   wgrad(s, kNodeCat, kHidden, n, A.Y[r], A.dN16, g + RoundLayout::W3);
   dgrad(s, n, kHidden, kNodeCat, A.dN16, W.w16 + RoundLayout::W3, A.dY);
   // edge layer 2 (message gradient = dY[:, kNodeIn:] broadcast to the edges)
-  silu_backward(s, static_cast<int>(E), kHidden, A.dY + kNodeIn, kNodeCat, kKnn,
+  act_backward(s, static_cast<int>(E), kHidden, A.dY + kNodeIn, kNodeCat, kKnn,
                 A.pre2[r], A.dE, A.dE16);
   column_sum(s, static_cast<int>(E), kHidden, A.dE, g + RoundLayout::b2);
   wgrad(s, kHidden, kHidden, E, A.a1[r], A.dE16, g + RoundLayout::W2);
   dgrad(s, E, kHidden, kHidden, A.dE16, W.w16 + RoundLayout::W2, A.dE);
   // edge layer 1
-  silu_backward(s, static_cast<int>(E), kHidden, A.dE, kHidden, 1, A.pre1[r], A.dE, A.dE16);
+  act_backward(s, static_cast<int>(E), kHidden, A.dE, kHidden, 1, A.pre1[r], A.dE, A.dE16);
   column_sum(s, static_cast<int>(E), kHidden, A.dE, g + RoundLayout::b1);
   wgrad(s, kEdgeIn, kHidden, E, A.X[r], A.dE16, g + RoundLayout::W1);
   if (r > 0) {
@@ -1604,7 +1604,7 @@ tit_status tit_hip_nn_gemm(int M, int N, int K, const float* A, int transA,
                            const float* B, int transB, const float* bias, int act,
                            int k_splits, float* C) {
   if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C || k_splits < 1 ||
-      (act != kActNone && act != kActSilu) || (k_splits > 1 && (bias || act))) {
+      (act != kActNone && act != kActHswish) || (k_splits > 1 && (bias || act))) {
     return TIT_E_ARG;
   }
   std::vector<void*> mem;

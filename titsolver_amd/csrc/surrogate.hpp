@@ -10,10 +10,10 @@
 //
 // Per round r = 0, 1 with node input H_r = [nf | u_{r-1}] (u_{-1} = 0):
 //   edge   x_e = [ef_e | H_a | H_b]                         (kEdgeIn)
-//          m_e = SiLU(SiLU(x_e W1 + b1) W2 + b2)            (kHidden)
+//          m_e = act(act(x_e W1 + b1) W2 + b2)              (kHidden)
 //   node   M_a = sum over the kKnn edges of a of m_e
 //          y_a = [H_a | M_a]                                (kNodeCat)
-//          u_a = SiLU(SiLU(y_a W3 + b3) W4 + b4) W5 + b5    (kOut)
+//          u_a = act(act(y_a W3 + b3) W4 + b4) W5 + b5      (kOut)
 // The last round's u holds (a_x, a_y, a_z, drho/dt) divided by their scales.
 // GEMM operands are BF16, accumulation FP32.
 #pragma once

@@ -124,6 +124,24 @@ __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
+// Accumulators that start from the bias: of the output rows
+// f = 32 ob + 8q + 4h + s (transposed layers), or of column f (direct layers).
+__device__ __forceinline__ f32x16 bias_rows(const float* b, int ob, int h) {
+  f32x16 c;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) c[4 * q + s] = b[32 * ob + 8 * q + 4 * h + s];
+  }
+  return c;
+}
+__device__ __forceinline__ f32x16 bias_all(float b) {
+  f32x16 c;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) c[k] = b;
+  return c;
+}
+
 // Tiles in XCD-major bands: workgroup b runs on XCD b % 8 and takes tiles of
 // that XCD's eighth of the particle range, so each L2 sees one region.
 __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
@@ -174,10 +192,9 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
   const int p_slot = 2 * ((slot >> 2) & 1) + (slot >> 4);
   const int k_slot = 4 * ((slot >> 3) & 1) + (slot & 3);
 
-  for (uint32_t tile = t_beg + k_in; tile < t_end; tile += per_xcd) {
+  // edge inputs x_e of this wave's 2 blocks of 32 edges of a tile
+  auto gather_x = [&](uint32_t tile, bf16x8 (&X)[2][5]) {
     const uint32_t p0 = tile * kTileP;
-    // ---------------- edge phase: wave handles edge blocks 2w, 2w + 1 --------------
-    bf16x8 X[2][5];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const int blk = 2 * wave + c;                      // 0..15
@@ -191,11 +208,18 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
       X[c][3] = ld8(A.H + static_cast<size_t>(b) * kNodeIn + 8 * h);
       X[c][4] = ld8(A.H + static_cast<size_t>(b) * kNodeIn + 16 + 8 * h);
     }
+  };
+  bf16x8 X[2][5];
+  if (t_beg + k_in < t_end) gather_x(t_beg + k_in, X);
+  for (uint32_t tile = t_beg + k_in; tile < t_end; tile += per_xcd) {
+    const uint32_t p0 = tile * kTileP;
+    // ---------------- edge phase: wave handles edge blocks 2w, 2w + 1 --------------
     // layer 1 (transposed) -> layer-2 A operands, BF16
     bf16x8 H1[2][8];  // [block][K-step of layer 2 = 2 ob + jj]
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
-      f32x16 acc[2] = {{}, {}};
+      const f32x16 b0 = bias_rows(b1, ob, h);
+      f32x16 acc[2] = {b0, b0};
 #pragma unroll
       for (int ks = 0; ks < 5; ++ks) {
         const bf16x8 a = ld8(A.frag + kOffF1 + (ob * 5 + ks) * kFrag + 8 * lane);
@@ -208,27 +232,25 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
         for (int jj = 0; jj < 2; ++jj) {
           bf16x8 v;
 #pragma unroll
-          for (int t = 0; t < 8; ++t) {
-            const int q = 2 * jj + (t >> 2), s = t & 3;
-            const int f = 32 * ob + 8 * q + 4 * h + s;
-            v[t] = static_cast<bf16>(nn_silu(acc[c][8 * jj + t] + b1[f]));
-          }
+          for (int t = 0; t < 8; ++t) v[t] = static_cast<bf16>(nn_act(acc[c][8 * jj + t]));
           H1[c][2 * ob + jj] = v;
         }
       }
     }
+    // the next tile's edge inputs load while this tile finishes
+    if (tile + per_xcd < t_end) gather_x(tile + per_xcd, X);
     // layer 2 (direct) + aggregation -> M into LDS
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
-      f32x16 acc[2] = {{}, {}};
+      const int f = 32 * ob + col;
+      const f32x16 b0 = bias_all(b2[f]);
+      f32x16 acc[2] = {b0, b0};
 #pragma unroll
       for (int ks = 0; ks < 8; ++ks) {
         const bf16x8 bw = ld8(sF + (kOffF2 - kOffF2) + (ob * 8 + ks) * kFrag + 8 * lane);
         acc[0] = mfma(H1[0][ks], bw, acc[0]);
         acc[1] = mfma(H1[1][ks], bw, acc[1]);
       }
-      const int f = 32 * ob + col;
-      const float bb = b2[f];
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const int blk = 2 * wave + c;
@@ -238,7 +260,7 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
 #pragma unroll
           for (int q = 2 * jj; q < 2 * jj + 2; ++q) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) s += nn_silu(acc[c][4 * q + e] + bb);
+            for (int e = 0; e < 4; ++e) s += nn_act(acc[c][4 * q + e]);
           }
           const int pl = 4 * blk + 2 * h + jj;  // tile-local particle
           sY[pl * kYld + kNodeIn + f] = static_cast<bf16>(s);
@@ -257,7 +279,7 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
     const int ob = wave & 3, pb = wave >> 2;
     const int prow = 32 * pb + col;  // particle of this lane's B column
     {  // layer 3: y (160) -> 128
-      f32x16 acc = {};
+      f32x16 acc = bias_rows(b3, ob, h);
 #pragma unroll
       for (int ks = 0; ks < 10; ++ks) {
         const bf16x8 a = ld8(sF + (kOffF3 - kOffF2) + (ob * 10 + ks) * kFrag + 8 * lane);
@@ -269,13 +291,13 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const int f = 32 * ob + 8 * q + 4 * h + s;
-          sH3[prow * kHld + f] = static_cast<bf16>(nn_silu(acc[4 * q + s] + b3[f]));
+          sH3[prow * kHld + f] = static_cast<bf16>(nn_act(acc[4 * q + s]));
         }
       }
     }
     __syncthreads();
     {  // layer 4: 128 -> 128, output into the y rows (y is no longer read)
-      f32x16 acc = {};
+      f32x16 acc = bias_rows(b4, ob, h);
 #pragma unroll
       for (int ks = 0; ks < 8; ++ks) {
         const bf16x8 a = ld8(sF + (kOffF4 - kOffF2) + (ob * 8 + ks) * kFrag + 8 * lane);
@@ -287,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const int f = 32 * ob + 8 * q + 4 * h + s;
-          sY[prow * kHld + f] = static_cast<bf16>(nn_silu(acc[4 * q + s] + b4[f]));
+          sY[prow * kHld + f] = static_cast<bf16>(nn_act(acc[4 * q + s]));
         }
       }
     }
