@@ -237,8 +237,6 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
         }
       }
     }
-    // the next tile's edge inputs load while this tile finishes
-    if (tile + per_xcd < t_end) gather_x(tile + per_xcd, X);
     // layer 2 (direct) + aggregation -> M into LDS
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
@@ -267,6 +265,8 @@ __global__ __launch_bounds__(kThreads) void k_round_fused(FusedArgs A) {
         }
       }
     }
+    // the next tile's edge inputs load while this tile finishes
+    if (tile + per_xcd < t_end) gather_x(tile + per_xcd, X);
     // node inputs H into y[:, 0:32]
     if (tid < kTileP * (kNodeIn / 8)) {
       const int pl = tid >> 2, g = tid & 3;
