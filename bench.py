@@ -43,7 +43,7 @@ def stage_bytes_per_particle(dim: int) -> float:
     return (2 * state + 3 * state + 3 * state) / 3.0
 
 
-def measured_traffic(nf: int):
+def measured_traffic(workload: str, nf: int):
     """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE runs (scaled by a 256 MiB copy), when they were taken on this
     same workload; None otherwise."""
@@ -52,7 +52,8 @@ def measured_traffic(nf: int):
         d = json.loads(f.read_text())
     except (OSError, ValueError):
         return None
-    return d["hbm_bytes_per_launch"] if d.get("fluid_particles") == nf else None
+    same = d.get("bench_workload") == workload and d.get("fluid_particles") == nf
+    return d["hbm_bytes_per_launch"] if same else None
 
 
 def parse():
@@ -189,7 +190,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_stage (fused pair sums + SSPRK update)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(nf),
+                     "traffic": measured_traffic(args.workload, nf),
                      "bytes_per_launch": bytes_launch,
                      "avg_launch_us": avg_stage_s * 1e6},
         "kernels": kernels,
