@@ -299,18 +299,30 @@ __device__ __forceinline__ void gather(const DevParams& P, const Pos& ps,
       if (i >= c0 && i < c0 + len) mask &= ~(1ull << (i - c0));  // not self
       count += static_cast<uint32_t>(__builtin_popcountll(mask));
       if (!mask) continue;
-      // Two register sets alternate: the next accepted candidate's loads are
-      // in flight while the current one is summed.
       PairIn<DIM> A, B;
+      if (acc.pend) {  // the odd term of an earlier batch waits for its partner
+        load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), A);
+        sum_pair<DIM, KERNEL, FLAVOUR>(P, a, A, acc);
+        if (!mask) continue;
+      }
+      // Two register sets alternate: the next accepted candidate's loads are
+      // in flight while the current one is summed; two terms enter the
+      // compensated sums together.
       load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), A);
       while (true) {
         const bool hb = mask != 0;
         if (hb) load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), B);
-        sum_pair<DIM, KERNEL, FLAVOUR>(P, a, A, acc);
-        if (!hb) break;
+        float ca, fa[DIM];
+        term_of<DIM, KERNEL, FLAVOUR>(P, a, A, ca, fa);
+        if (!hb) {
+          acc.add(ca, fa);
+          break;
+        }
         const bool ha = mask != 0;
         if (ha) load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), A);
-        sum_pair<DIM, KERNEL, FLAVOUR>(P, a, B, acc);
+        float cb, fb[DIM];
+        term_of<DIM, KERNEL, FLAVOUR>(P, a, B, cb, fb);
+        acc.add2(ca, fa, cb, fb);
         if (!ha) break;
       }
     }
