@@ -21,12 +21,16 @@ FLAVOUR_REF, FLAVOUR_CFG = 0, 1
 
 class OrcParams(C.Structure):
     _fields_ = [("dim", C.c_int), ("kernel", C.c_int), ("flavour", C.c_int),
-                ("reserved", C.c_int)] + [
+                ("post_integrate", C.c_int)] + [
         (n, C.c_double) for n in ("h", "m", "rho0", "cs0", "xi", "g", "mu",
                                   "alpha", "delta", "cfl", "c_force", "c_visc")]
 
 
 _DP = C.POINTER(C.c_double)
+
+
+class OrcShiftOut(C.Structure):
+    _fields_ = [("N", _DP), ("grad_v", _DP), ("grad_rho", _DP), ("phi", _DP), ("dr", _DP)]
 
 
 class OrcState(C.Structure):
@@ -75,6 +79,7 @@ def load():
         "orc_compute_momentum": (C.c_int, [P, S, U64, U32]),
         "orc_compute_time_step": (C.c_double, [P, S]),
         "orc_ssprk_step": (C.c_int, [P, S, C.c_int, _DP]),
+        "orc_post_integrate": (C.c_int, [P, S, U64, U32, C.POINTER(OrcShiftOut)]),
         "orc_ssprk_generic": (C.c_int, [C.c_int, C.c_double, C.c_size_t, _DP,
                                         C.c_void_p, C.c_void_p]),
         "orc_set_mutation": (None, [C.c_int]),
@@ -193,6 +198,20 @@ def compute_continuity(prm, st, off, idx):
 
 def compute_momentum(prm, st, off, idx):
     load().orc_compute_momentum(C.byref(prm), C.byref(st.c), *_csr(off, idx))
+
+
+def post_integrate(prm, st, off, idx) -> dict:
+    """FluidEquations::apply_shifts + apply_free_surface_correction on the
+    given adjacency; updates st and returns the renormalised N, grad v,
+    grad rho, the free-surface indicator phi and the applied shift dr."""
+    n, dim = st.n, st.dim
+    res = {"N": np.zeros((n, dim)), "grad_v": np.zeros((n, dim, dim)),
+           "grad_rho": np.zeros((n, dim)), "phi": np.zeros(n), "dr": np.zeros((n, dim))}
+    out = OrcShiftOut(*(_dp(res[k]) for k in ("N", "grad_v", "grad_rho", "phi", "dr")))
+    rc = load().orc_post_integrate(C.byref(prm), C.byref(st.c), *_csr(off, idx), C.byref(out))
+    if rc != 0:
+        raise RuntimeError("oracle post_integrate failed")
+    return res
 
 
 def compute_time_step(prm, st) -> float:

@@ -23,6 +23,13 @@
  *   - SSPRK: one step on u' = lambda u equals the cubic Taylor polynomial,
  *     third-order convergence.
  *   - time step: closed form on single-particle states.
+ *   - post-integration: the renormalised gradients reproduce linear velocity
+ *     and density fields exactly on a perturbed lattice, the free-surface
+ *     indicator classifies a lattice block's outer layer, interior and the
+ *     layer within 2h of the surface (phi = depth / 2h there), the shift moves
+ *     a displaced particle back toward its lattice site, the free-surface
+ *     correction keeps rho0 on a uniform block's surface, planted mistakes
+ *     5-8 each fail one of these.
  */
 #include "tit_oracle.h"
 
@@ -648,6 +655,248 @@ static int prepare(const orc_params* prm, orc_state* s, step_ws* w) {
   return orc_setup_boundary(prm, s, w->off, w->idx.data);
 }
 
+/* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
+/* Post-integration (fluid_equations.hpp:313-512), wall-free form: gamma = 1,
+ * so the face loops are empty and the velocity correction always applies. */
+
+/* FluidEquations constants (fluid_equations.hpp:518-524). */
+static const double kShiftCfl = 0.4;  /* CFL_ */
+static const double kShiftC = 0.2;    /* C_shift_ */
+static const double kPhiMax = 1.0;    /* phi_max_ */
+#define ORC_PHI_MIN DBL_MIN           /* phi_min_ = numeric_limits::min() */
+static double k_free_surface(void) { return -log(0.05) / pow2(0.01); }
+
+static int same_bits(double a, double b) { return memcmp(&a, &b, sizeof a) == 0; }
+
+/* lu (core/_mat/fact.hpp:84-107): Doolittle factors of a row-major dim x dim
+ * matrix, near-singular when a pivot is tiny; FactLU::inverse (:69-72) then
+ * solves L U X = I, forward with the unit lower factor, backward with the
+ * upper one.  Returns 0 on success. */
+static int lu_inverse(int dim, const double* A, double* inv) {
+  double LU[9];
+  for (int i = 0; i < dim; ++i) {
+    for (int j = 0; j < i; ++j) {
+      double x = A[i * dim + j];
+      for (int k = 0; k < j; ++k) x -= LU[i * dim + k] * LU[k * dim + j];
+      LU[i * dim + j] = x / LU[j * dim + j];
+    }
+    for (int j = i; j < dim; ++j) {
+      double x = A[i * dim + j];
+      for (int k = 0; k < i; ++k) x -= LU[i * dim + k] * LU[k * dim + j];
+      LU[i * dim + j] = x;
+    }
+    if (is_tiny(LU[i * dim + i])) return -1;
+  }
+  for (int c = 0; c < dim; ++c) {
+    double y[3];
+    for (int i = 0; i < dim; ++i) {
+      double x = i == c ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) x -= LU[i * dim + k] * y[k];
+      y[i] = x;
+    }
+    for (int i = dim - 1; i >= 0; --i) {
+      double x = y[i];
+      for (int k = i + 1; k < dim; ++k) x -= LU[i * dim + k] * y[k];
+      y[i] = x / LU[i * dim + i];
+    }
+    for (int i = 0; i < dim; ++i) inv[i * dim + c] = y[i];
+  }
+  return 0;
+}
+
+int orc_post_integrate(const orc_params* prm, orc_state* s, const uint64_t* off,
+                       const uint32_t* idx, orc_shift_out* out) {
+  const int dim = prm->dim, dd = dim * dim;
+  const size_t nf = s->n_fluid, n = s->n_fluid + s->n_fixed;
+  const double R = orc_kernel_radius(prm->kernel, prm->h);
+  double* N = (double*)calloc(n * dim, sizeof(double));
+  double* L = (double*)calloc(n * dd, sizeof(double));
+  double* gv = (double*)calloc(n * dd, sizeof(double));
+  double* gr = (double*)calloc(n * dim, sizeof(double));
+  double* dr = (double*)calloc(n * dim, sizeof(double));
+  double* phi = (double*)malloc(n * sizeof(double));
+  double* rho_raw = (double*)malloc(n * sizeof(double));
+  if (!N || !L || !gv || !gr || !dr || !phi || !rho_raw) {
+    free(N), free(L), free(gv), free(gr), free(dr), free(phi), free(rho_raw);
+    return -1;
+  }
+  const double* r = s->r;
+  const double* v = s->v;
+  const double* rho = s->rho;
+
+  /* apply_shifts, pair sums over unique pairs b < a (:340-353) */
+  for (size_t a = 0; a < n; ++a) {
+    for (uint64_t j = off[a]; j < off[a + 1]; ++j) {
+      const size_t b = idx[j];
+      if (b >= a) continue;
+      const double V_a = prm->m / rho[a], V_b = prm->m / rho[b];
+      double x_ab[3], gW[3];
+      for (int k = 0; k < dim; ++k) x_ab[k] = r[a * dim + k] - r[b * dim + k];
+      orc_kernel_grad(prm->kernel, dim, x_ab, prm->h, gW);
+      for (int i = 0; i < dim; ++i) {
+        N[a * dim + i] += V_b * gW[i];
+        N[b * dim + i] -= V_a * gW[i];
+        const double r_ba = r[b * dim + i] - r[a * dim + i], r_ab = -r_ba;
+        const double v_ba = v[b * dim + i] - v[a * dim + i], v_ab = v[a * dim + i] - v[b * dim + i];
+        for (int q = 0; q < dim; ++q) {
+          L[a * dd + i * dim + q] += V_b * (r_ba * gW[q]);
+          L[b * dd + i * dim + q] -= V_a * (r_ab * gW[q]);
+          gv[a * dd + i * dim + q] += V_b * (v_ba * gW[q]);
+          gv[b * dd + i * dim + q] -= V_a * (v_ab * gW[q]);
+        }
+        gr[a * dim + i] += V_b * (rho[b] - rho[a]) * gW[i];
+        gr[b * dim + i] -= V_a * (rho[a] - rho[b]) * gW[i];
+      }
+    }
+  }
+  /* renormalisation (:354-365) */
+  for (size_t a = 0; a < n; ++a) {
+    double* Na = N + a * dim;
+    for (int i = 0; i < dim; ++i) dr[a * dim + i] = Na[i];
+    double LT[9], inv[9];
+    for (int i = 0; i < dim; ++i) {
+      for (int q = 0; q < dim; ++q) LT[i * dim + q] = L[a * dd + q * dim + i];
+    }
+    if (lu_inverse(dim, LT, inv) == 0) {
+      double t[9];
+      for (int i = 0; i < dim; ++i) {
+        double x = 0.0;
+        for (int q = 0; q < dim; ++q) x += inv[i * dim + q] * Na[q];
+        t[i] = x;
+      }
+      for (int i = 0; i < dim; ++i) Na[i] = t[i];
+      double* G = gv + a * dd;
+      if (g_mutation != 5) {
+        for (int i = 0; i < dim; ++i) {
+          for (int q = 0; q < dim; ++q) {
+            double x = 0.0;
+            for (int k = 0; k < dim; ++k) x += G[i * dim + k] * inv[q * dim + k];
+            t[i * dim + q] = x;
+          }
+        }
+        for (int k = 0; k < dd; ++k) G[k] = t[k];
+      }
+      double* g = gr + a * dim;
+      for (int i = 0; i < dim; ++i) {
+        double x = 0.0;
+        for (int q = 0; q < dim; ++q) x += inv[i * dim + q] * g[q];
+        t[i] = x;
+      }
+      for (int i = 0; i < dim; ++i) g[i] = t[i];
+    }
+    double nn[3];
+    normalize(dim, Na, nn);
+    for (int i = 0; i < dim; ++i) Na[i] = nn[i];
+  }
+  /* free-surface indicator (:375-419) */
+  for (size_t a = 0; a < n; ++a) phi[a] = a < nf ? ORC_PHI_MIN : kPhiMax;
+  const double cos_fov = cos(3.14159265358979323846 / 4.0); /* std::numbers::pi */
+  for (size_t a = 0; a < n; ++a) {
+    for (uint64_t j = off[a]; j < off[a + 1]; ++j) {
+      const size_t b = idx[j];
+      if (b >= a) continue;
+      double x_ab[3];
+      for (int k = 0; k < dim; ++k) x_ab[k] = r[a * dim + k] - r[b * dim + k];
+      const double r2_ab = norm2(dim, x_ab);
+      const double thr = (R + R) / 2.0;
+      if (r2_ab > pow2(thr)) continue;
+      const double fov = pow2(cos_fov) * r2_ab;
+      if (same_bits(phi[a], ORC_PHI_MIN)) {
+        const double n_a = dot(dim, N + a * dim, x_ab);
+        const int in_cone = g_mutation == 6 ? n_a < 0.0 : n_a > 0.0;
+        if (in_cone && pow2(n_a) >= fov) phi[a] = kPhiMax;
+      }
+      if (same_bits(phi[b], ORC_PHI_MIN)) {
+        const double n_b = dot(dim, N + b * dim, x_ab);
+        const int in_cone = g_mutation == 6 ? n_b > 0.0 : n_b < 0.0;
+        if (in_cone && pow2(n_b) >= fov) phi[b] = kPhiMax;
+      }
+    }
+  }
+  /* splashes (:421-426) */
+  const uint64_t cutoff = dim == 2 ? 8 : 26;
+  for (size_t a = 0; a < nf; ++a) {
+    if (off[a + 1] - off[a] <= cutoff) phi[a] = ORC_PHI_MIN;
+  }
+  /* near-surface particles (:439-452): the nearest surface neighbour, first
+   * in list order among equals */
+  for (size_t a = 0; a < nf; ++a) {
+    if (!same_bits(phi[a], kPhiMax)) continue;
+    size_t best = n;
+    double best_d = 0.0;
+    for (uint64_t j = off[a]; j < off[a + 1]; ++j) {
+      const size_t b = idx[j];
+      if (!same_bits(phi[b], ORC_PHI_MIN)) continue;
+      double x_ab[3];
+      for (int k = 0; k < dim; ++k) x_ab[k] = r[a * dim + k] - r[b * dim + k];
+      const double d2 = norm2(dim, x_ab);
+      if (best == n || d2 < best_d) {
+        best = b;
+        best_d = d2;
+      }
+    }
+    if (best < n) {
+      double x_ab[3];
+      for (int k = 0; k < dim; ++k) x_ab[k] = r[a * dim + k] - r[best * dim + k];
+      phi[a] *= fabs(dot(dim, N + best * dim, x_ab)) / R;
+    }
+  }
+  /* shifts and field corrections (:455-470) */
+  const double coef = -kShiftCfl * kShiftC * pow2(prm->h);
+  for (size_t a = 0; a < nf; ++a) {
+    double* da = dr + a * dim;
+    if (!same_bits(phi[a], kPhiMax) && g_mutation != 7) {
+      for (int i = 0; i < dim; ++i) da[i] = 0.0;
+      continue;
+    }
+    for (int i = 0; i < dim; ++i) da[i] *= coef;
+    double dv[3];
+    for (int i = 0; i < dim; ++i) {
+      double x = 0.0;
+      for (int q = 0; q < dim; ++q) x += gv[a * dd + i * dim + q] * da[q];
+      dv[i] = x;
+    }
+    for (int i = 0; i < dim; ++i) {
+      s->r[a * dim + i] += da[i];
+      s->v[a * dim + i] += dv[i];
+    }
+    s->rho[a] += dot(dim, gr + a * dim, da);
+  }
+  for (size_t a = nf; a < n; ++a) {
+    for (int i = 0; i < dim; ++i) dr[a * dim + i] = 0.0;
+  }
+  /* apply_free_surface_correction (:483-512), on the same lists */
+  memcpy(rho_raw, s->rho, n * sizeof(double));
+  const double K = k_free_surface();
+  for (size_t a = 0; a < nf; ++a) {
+    if (same_bits(phi[a], kPhiMax)) continue;
+    double alpha = 0.0, rho_t = 0.0;
+    for (uint64_t j = off[a]; j < off[a + 1]; ++j) {
+      const size_t b = idx[j];
+      double x_ab[3];
+      for (int k = 0; k < dim; ++k) x_ab[k] = s->r[a * dim + k] - s->r[b * dim + k];
+      const double W_ab = orc_kernel_value(prm->kernel, dim, x_ab, prm->h);
+      alpha += prm->m / rho_raw[b] * W_ab;
+      rho_t += prm->m * W_ab;
+    }
+    const double ratio = fmin(1.0, alpha / 1.0);
+    if (ratio > 0.99) continue;
+    const double e = g_mutation == 8 ? K * pow2(ratio - 1.0) : -K * pow2(ratio - 1.0);
+    const double beta = exp(e);
+    const double corr = beta * 1.0 + (1.0 - beta) * alpha;
+    if (!is_tiny(corr)) s->rho[a] = rho_t / corr;
+  }
+  if (out) {
+    if (out->N) memcpy(out->N, N, n * dim * sizeof(double));
+    if (out->grad_v) memcpy(out->grad_v, gv, n * dd * sizeof(double));
+    if (out->grad_rho) memcpy(out->grad_rho, gr, n * dim * sizeof(double));
+    if (out->phi) memcpy(out->phi, phi, n * sizeof(double));
+    if (out->dr) memcpy(out->dr, dr, n * dim * sizeof(double));
+  }
+  free(N), free(L), free(gv), free(gr), free(dr), free(phi), free(rho_raw);
+  return 0;
+}
+
 /* SSPRKIntegrator::substep_, time_integrator.hpp:188-210 */
 static int substep(const orc_params* prm, orc_state* s, step_ws* w,
                    double* dt) {
@@ -665,8 +914,9 @@ static int substep(const orc_params* prm, orc_state* s, step_ws* w,
   return 0;
 }
 
-/* SSPRKIntegrator::step, time_integrator.hpp:159-184 (post_integrate, the
- * particle shifting stage, is not part of this path). */
+/* SSPRKIntegrator::step, time_integrator.hpp:159-184; post_integrate
+ * (fluid_equations.hpp:313-321: prepare, apply_shifts, free-surface
+ * correction) when prm->post_integrate is set. */
 int orc_ssprk_step(const orc_params* prm, orc_state* s, int order,
                    double* dt_out) {
   if (order != 2 && order != 3) return -1;
@@ -686,6 +936,10 @@ int orc_ssprk_step(const orc_params* prm, orc_state* s, int order,
       lincomb(nf * dim, w->v_old, s->v, wgt);
       lincomb(nf, w->rho_old, s->rho, wgt);
     }
+  }
+  if (prm->post_integrate) {
+    if (prepare(prm, s, w)) return -1;
+    if (orc_post_integrate(prm, s, w->off, w->idx.data, NULL)) return -1;
   }
   if (dt_out) *dt_out = dt;
   return 0;

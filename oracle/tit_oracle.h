@@ -40,7 +40,7 @@ typedef struct orc_params {
   int dim;      /* 2 or 3 */
   int kernel;   /* ORC_KERNEL_* */
   int flavour;  /* ORC_FLAVOUR_* */
-  int reserved;
+  int post_integrate; /* 1: orc_ssprk_step ends with orc_post_integrate */
   double h;       /* kernel width (uniform field h) */
   double m;       /* particle mass (uniform) */
   double rho0;    /* reference density */
@@ -109,10 +109,28 @@ int orc_compute_momentum(const orc_params* prm, orc_state* s,
                          const uint64_t* off, const uint32_t* idx);
 double orc_compute_time_step(const orc_params* prm, const orc_state* s);
 
+/* ---- post-integration (particle shifting, free-surface correction) ------- */
+/* Optional outputs of orc_post_integrate (any pointer may be null). */
+typedef struct orc_shift_out {
+  double* N;        /* n * dim: normalised, renormalised surface normal N */
+  double* grad_v;   /* n * dim * dim: renormalised grad v, [i][j] = dv_i/dx_j */
+  double* grad_rho; /* n * dim: renormalised grad rho */
+  double* phi;      /* n: free-surface indicator (DBL_MIN on the surface, 1 far) */
+  double* dr;       /* n * dim: the shift applied to each particle */
+} orc_shift_out;
+
+/* FluidEquations::apply_shifts + apply_free_surface_correction
+ * (fluid_equations.hpp:327-512) in the wall-free form (gamma = 1), on the
+ * adjacency of the current positions (as post_integrate's prepare builds
+ * it).  Updates r, v, rho of the fluid particles.  Returns 0 on success. */
+int orc_post_integrate(const orc_params* prm, orc_state* s, const uint64_t* off,
+                       const uint32_t* idx, orc_shift_out* out);
+
 /* ---- integrator ---------------------------------------------------------- */
 /* One SSPRK step (order 2 or 3) with a grid search before every stage, the
- * reference SSPRKIntegrator::step without post_integrate.  Returns 0 on
- * success.  *dt_out receives the step size. */
+ * reference SSPRKIntegrator::step; post_integrate (re-index, boundary,
+ * orc_post_integrate) runs at its end when prm->post_integrate is set.
+ * Returns 0 on success.  *dt_out receives the step size. */
 int orc_ssprk_step(const orc_params* prm, orc_state* s, int order,
                    double* dt_out);
 
@@ -124,7 +142,10 @@ int orc_ssprk_generic(int order, double dt, size_t n, double* u,
 
 /* Test-only: plant a known mistake (1: momentum pressure sign on the a side,
  * 2: anti-diffusive delta-SPH sign, 3: viscosity on receding pairs,
- * 4: swapped SSPRK combination weights); 0 restores the faithful oracle. */
+ * 4: swapped SSPRK combination weights, 5: grad v not renormalised,
+ * 6: free-surface visibility cone reversed, 7: shifts applied to every
+ * fluid particle, 8: free-surface blend exponent with the wrong sign); 0
+ * restores the faithful oracle. */
 void orc_set_mutation(int m);
 
 #ifdef __cplusplus
