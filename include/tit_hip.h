@@ -64,6 +64,10 @@ typedef struct tit_hip_params {
   double cfl, c_force, c_visc; /* time-step coefficients */
   double skin;      /* neighbour-cell skin in units of the support radius;
                        0 selects the default 0.1 */
+  int post_integrate; /* 1: every step ends with FluidEquations::post_integrate
+                         (fluid_equations.hpp:313-321): particle shifting and
+                         the free-surface density correction */
+  int reserved;       /* 0 */
 } tit_hip_params;
 
 /* One tank: the box its cell grid covers.  Particles that leave the box are
@@ -120,6 +124,16 @@ tit_status tit_hip_tank_clock(tit_hip_ctx* ctx, double* time, double* dt_next);
 /* FluidEquations::prepare without the wall terms: neighbour index rebuild and
  * setup_boundary (fluid_equations.hpp:99-105, :122-164). */
 tit_status tit_hip_prepare(tit_hip_ctx* ctx);
+
+/* FluidEquations::post_integrate (fluid_equations.hpp:313-321) on the
+ * current state, in the wall-free form: prepare, apply_shifts (:327-471)
+ * and apply_free_surface_correction (:480-512).  Requires a context created
+ * with params.post_integrate = 1.  phi (may be NULL) receives the
+ * free-surface indicator of every particle in the reference order (DBL_MIN
+ * on the surface, 1 far from it).  The next step size is taken again from
+ * the new state and the last stage's acceleration (zero right after an
+ * upload). */
+tit_status tit_hip_post_integrate(tit_hip_ctx* ctx, double* phi);
 
 /* FluidEquations::compute_time_step (fluid_equations.hpp:199-222) on the
  * current state and the last momentum RHS; one value per tank. */

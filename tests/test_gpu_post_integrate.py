@@ -1,0 +1,106 @@
+"""Post-integration on the device (particle shifting and the free-surface
+density correction, FluidEquations::post_integrate,
+source/tit/sph/fluid_equations.hpp:313-512) against the FP64 oracle.
+
+Both sides run in FP64 and differ only in summation order, so the
+free-surface indicator (a discrete classification) must match exactly and
+positions, velocities and densities to 1e-12 relative.  Over steps, the
+trajectory carries the FP32 pair sums of the stages: there the bars are
+those of the plain step.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import titsolver_amd as ta
+from helpers import perturbed_dam_break
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq(dim, dr, kernel=ta.KERNEL_WENDLAND_C2, flavour=ta.FLAVOUR_REF):
+    eq = ta.cfg_equations(dim, dr, 0.6, kernel=kernel, flavour=flavour)
+    if flavour == ta.FLAVOUR_REF:
+        eq.mu, eq.cfl = 1e-3, 0.4
+    return eq
+
+
+@pytest.mark.parametrize("dim,kernel", [(2, ta.KERNEL_WENDLAND_C2), (2, ta.KERNEL_WENDLAND_C4),
+                                        (3, ta.KERNEL_WENDLAND_C2)])
+def test_post_integrate_matches_oracle(dim, kernel):
+    dr = 0.02 if dim == 2 else 0.03
+    pa = perturbed_dam_break(dim, dr, seed=7, jitter=0.1)
+    eq = _eq(dim, dr, kernel)
+    ctx = ta.HipContext(eq, post_integrate=True)
+    ctx.upload([pa])
+    phi = ctx.post_integrate()
+    got = ctx.download()
+
+    prm = oracle.params(eq)
+    st = oracle.State.from_particles(pa)
+    off, idx = oracle.adjacency(st, prm)
+    oracle.setup_boundary(prm, st, off, idx)
+    ref = oracle.post_integrate(prm, st, off, idx)
+    nf = pa.n_fluid
+
+    tiny = np.finfo(np.float64).tiny
+    n_surf, n_far = int((ref["phi"][:nf] == tiny).sum()), int((ref["phi"][:nf] == 1.0).sum())
+    shifted = np.any(ref["dr"][:nf] != 0.0, axis=1)
+    print(f"dim={dim} kernel={kernel}: {nf} fluid, {n_surf} on the surface, {n_far} far, "
+          f"{shifted.sum()} shifted")
+    assert n_surf > 0 and n_far > 0 and shifted.sum() > 0
+    # classification: identical (equal bits for the surface and far values;
+    # near-surface values are products of FP64 sums)
+    assert np.array_equal(phi[:nf] == tiny, ref["phi"][:nf] == tiny)
+    assert np.array_equal(phi[:nf] == 1.0, ref["phi"][:nf] == 1.0)
+    near = (ref["phi"][:nf] != tiny) & (ref["phi"][:nf] != 1.0)
+    assert np.allclose(phi[:nf][near], ref["phi"][:nf][near], rtol=1e-12, atol=0)
+    # state after the shifts and the correction
+    r_err = np.abs(got["r"][:nf] - st.r[:nf]).max() / dr
+    v_err = np.abs(got["v"][:nf] - st.v[:nf]).max() / max(np.abs(st.v[:nf]).max(), 1e-30)
+    rho_err = np.abs(got["rho"][:nf] / st.rho[:nf] - 1.0).max()
+    print(f"  r/dr {r_err:.2e}, v {v_err:.2e}, rho {rho_err:.2e}")
+    assert r_err < 1e-12 and v_err < 1e-12 and rho_err < 1e-12
+
+
+def test_steps_with_post_integrate_match_oracle():
+    """20 SSPRK3 steps of the 2D dam break (dr = 0.02) with post_integrate,
+    reference flavour, against the oracle's step with post_integrate."""
+    dr = 0.02
+    pa = ta.dam_break(2, dr, [3.22, 2.4], [1.0, 2.0])
+    eq = _eq(2, dr)
+    ctx = ta.HipContext(eq, post_integrate=True)
+    ctx.upload([pa])
+    prm = oracle.params(eq)
+    prm.post_integrate = 1
+    st = oracle.State.from_particles(pa)
+    nf = pa.n_fluid
+    for s in range(20):
+        dt_g = ctx.step(1)
+        dt_o = oracle.ssprk_step(prm, st, 3)
+        assert abs(dt_g - dt_o) <= 1e-7 * dt_o, (s, dt_g, dt_o)
+    out = ctx.download()
+    rho_err = np.max(np.abs(out["rho"][:nf] - st.rho[:nf]) / st.rho[:nf])
+    r_err = np.max(np.abs(out["r"][:nf] - st.r[:nf])) / dr
+    print(f"20 steps with post_integrate: rho rel {rho_err:.2e}, r/dr {r_err:.2e}")
+    assert rho_err < 1e-6
+    assert r_err < 1e-5
+
+
+def test_post_integrate_graph_equals_eager():
+    """The captured two-step graph and eager launches give the same bits."""
+    dr = 0.03
+    pa = perturbed_dam_break(2, dr, seed=3, jitter=0.05)
+    eq = _eq(2, dr)
+    res = []
+    for eager in (False, True):
+        ctx = ta.HipContext(eq, post_integrate=True)
+        ctx.upload([pa])
+        ctx.set_profiling(eager)
+        ctx.step(6)
+        res.append(ctx.download())
+        ctx.close()
+    for k in ("r", "v", "rho"):
+        assert np.array_equal(res[0][k], res[1][k]), k

@@ -31,6 +31,7 @@ class Params(C.Structure):
         ("mu", C.c_double), ("alpha", C.c_double), ("delta", C.c_double),
         ("cfl", C.c_double), ("c_force", C.c_double), ("c_visc", C.c_double),
         ("skin", C.c_double),
+        ("post_integrate", C.c_int), ("reserved", C.c_int),
     ]
 
 
@@ -79,6 +80,7 @@ SIGNATURES = {
     "tit_hip_prepare": (C.c_int, [C.c_void_p]),
     "tit_hip_compute_time_step": (C.c_int, [C.c_void_p, _DP]),
     "tit_hip_compute_rhs": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "tit_hip_post_integrate": (C.c_int, [C.c_void_p, _DP]),
     "tit_hip_neighbors": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint32), C.c_size_t,
                                     C.POINTER(C.c_size_t)]),

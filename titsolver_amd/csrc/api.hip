@@ -156,6 +156,9 @@ tit_status fill_params(tit_hip_ctx* c, const tit_hip_params& p) {
   if (p.flavour == TIT_FLAVOUR_REF && p.mu > 0.0 && !(p.c_visc > 0.0)) {
     return fail(c, TIT_E_ARG, "c_visc must be positive when mu > 0");
   }
+  if (p.post_integrate != 0 && p.post_integrate != 1) {
+    return fail(c, TIT_E_ARG, "post_integrate must be 0 or 1");
+  }
   c->prm = p;
   DevParams& d = c->dp;
   d = DevParams{};
@@ -359,6 +362,7 @@ StageArgs stage_args(tit_hip_ctx* c) {
   a.rhs_drho = c->rhs_drho;
   a.rhs_count = c->rhs_count;
   for (int k = 0; k < 3; ++k) a.rhs_dv[k] = c->rhs_dv[k];
+  for (int k = 0; k < 3; ++k) a.dv_last[k] = c->prm.post_integrate ? c->rhs_dv[k] : nullptr;
   return a;
 }
 
@@ -391,6 +395,43 @@ void enqueue_step(tit_hip_ctx* c) {
   }
   launch_step_end(s, c->n_tanks, c->dt, c->vmax, c->time, c->disp,
                   c->dp.skin_half, c->counter + 1);
+  if (c->prm.post_integrate) {
+    enqueue_boundary(c, c->A);  // prepare (fluid_equations.hpp:99-105)
+    enqueue_post_integrate(c);
+  }
+}
+
+// FluidEquations::post_integrate (fluid_equations.hpp:313-321) on the state
+// A: the index is the step's (it covers the post-step positions by the skin
+// bound); the boundary values are refreshed by the caller.
+void enqueue_post_integrate(tit_hip_ctx* c) {
+  const int dim = c->prm.dim;
+  const size_t n = c->n;
+  PiArgs a{};
+  a.P = c->dp;
+  a.tanks = c->tanks;
+  a.n = c->n;
+  a.a = c->A;
+  a.b = c->B;
+  a.c = c->C;
+  a.key = c->key;
+  a.tag = c->tag;
+  a.id = c->id;
+  a.cell_off = c->cell_off;
+  double* p = c->pi_buf;
+  for (int k = 0; k < dim; ++k, p += n) a.N[k] = p;
+  for (int k = 0; k < dim; ++k, p += n) a.dr0[k] = p;
+  for (int k = 0; k < dim * dim; ++k, p += n) a.grad_v[k] = p;
+  for (int k = 0; k < dim; ++k, p += n) a.grad_rho[k] = p;
+  a.phi = p;
+  a.phi_near = p + n;
+  a.count = c->pi_count;
+  a.shift_max = c->pi_shift;
+  for (int k = 0; k < dim; ++k) a.dv_last[k] = c->rhs_dv[k];
+  a.dt_min = c->dt_min;
+  a.vmax = c->vmax;
+  ProfScope ps(c->prof, c->stream, "post_integrate");
+  launch_post_integrate(c->stream, a, c->n_tanks, c->disp, c->dp.skin_half, c->counter + 1);
 }
 
 tit_status check_launch(tit_hip_ctx* c, const char* what) {
@@ -660,6 +701,18 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->rhs_drho, n));
   TIT_HIP_TRY(c, dalloc(c, &c->rhs_count, n));
   for (int k = 0; k < dim; ++k) TIT_HIP_TRY(c, dalloc(c, &c->rhs_dv[k], n));
+  if (c->prm.post_integrate) {
+    const size_t per = static_cast<size_t>(3 * dim + dim * dim + 2);
+    TIT_HIP_TRY(c, dalloc(c, &c->pi_buf, per * n));
+    TIT_HIP_TRY(c, dalloc(c, &c->pi_count, n));
+    TIT_HIP_TRY(c, dalloc(c, &c->pi_shift, n_tanks));
+    TIT_HIP_TRY(c, hipMemsetAsync(c->pi_shift, 0, n_tanks * sizeof(unsigned long long),
+                                  c->stream));
+    // the first step's time step selection reads the uploaded dv/dt only
+    for (int k = 0; k < dim; ++k) {
+      TIT_HIP_TRY(c, hipMemsetAsync(c->rhs_dv[k], 0, n * sizeof(double), c->stream));
+    }
+  }
 
   hipStream_t s = c->stream;
   TIT_HIP_TRY(c, hipMemcpyAsync(c->stage_r, hr.data(), hr.size() * sizeof(double),
@@ -785,6 +838,31 @@ tit_status tit_hip_prepare(tit_hip_ctx* c) {
   enqueue_boundary(c, c->A);
   tit_status st = check_launch(c, "prepare");
   if (st != TIT_OK) return st;
+  if (c->prof.enabled) c->prof.collect();
+  return read_flags(c);
+}
+
+tit_status tit_hip_post_integrate(tit_hip_ctx* c, double* phi) {
+  if (!c) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "post_integrate before upload");
+  if (!c->prm.post_integrate) {
+    return fail(c, TIT_E_STATE, "context created without post_integrate");
+  }
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  enqueue_rebuild(c, kRebuildPending);
+  enqueue_boundary(c, c->A);
+  enqueue_post_integrate(c);
+  const size_t n = c->n, dim = static_cast<size_t>(c->prm.dim);
+  if (phi) {  // phi_near follows N, dr0, grad v, grad rho, phi in pi_buf
+    launch_export_f64(c->stream, c->n, c->id, c->pi_buf + (3 * dim + dim * dim + 1) * n,
+                      c->stage_rho);
+  }
+  tit_status st = check_launch(c, "post_integrate");
+  if (st != TIT_OK) return st;
+  if (phi) {
+    TIT_HIP_TRY(c, hipMemcpyAsync(phi, c->stage_rho, n * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+  }
   if (c->prof.enabled) c->prof.collect();
   return read_flags(c);
 }

@@ -58,6 +58,11 @@ struct tit_hip_ctx {
   double* rhs_drho = nullptr;
   uint32_t* rhs_count = nullptr;  // pair counts of the last right-hand side
   double* rhs_dv[3] = {nullptr, nullptr, nullptr};
+  // post-integration scratch (allocated when params.post_integrate is set);
+  // rhs_dv holds the last stage's acceleration between steps then
+  double* pi_buf = nullptr;  // N, dr0, grad v, grad rho, phi, phi_near (SoA)
+  uint32_t* pi_count = nullptr;
+  unsigned long long* pi_shift = nullptr;  // per tank
 
   // chunk schedule of the stage kernel
   std::vector<DevTank> host_tanks;
@@ -116,6 +121,7 @@ enum RebuildMode {
 void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode);
 void enqueue_boundary(tit_hip_ctx* c, const DevState& st);
 void enqueue_step(tit_hip_ctx* c);
+void enqueue_post_integrate(tit_hip_ctx* c);
 tit_status check_launch(tit_hip_ctx* c, const char* what);
 tit_status read_flags(tit_hip_ctx* c);
 // n_steps plain SSPRK steps (graph replay where possible); synchronises.

@@ -1,0 +1,510 @@
+// Post-integration step of FluidEquations (fluid_equations.hpp:313-512) in
+// the wall-free form of this path (gamma = 1, no face terms): particle
+// shifting with renormalised gradients (apply_shifts, :327-471) and the
+// free-surface density correction (apply_free_surface_correction, :480-512).
+//
+// Everything runs in FP64, as the reference does, one particle per thread;
+// the neighbour sets are those of the post-step positions (the exact
+// predicate over the 5^d cell stencil, the index covering them by the skin
+// bound of the step).  Passes:
+//   sums     N = sum V_b grad W_ab, L = sum V_b r_ba (x) grad W_ab, grad v,
+//            grad rho (every particle); renormalise by (L^T)^-1; N normalised
+//   phi      free-surface indicator: a fluid particle is "far" (1) when a
+//            neighbour lies within 45 degrees of -N behind it, splashes
+//            (<= 8 / 26 neighbours) are on the surface (DBL_MIN)
+//   shift    far particles next to a surface particle get phi = |N_b . r_ab|
+//            / 2h; particles still far are shifted by -0.08 h^2 N_raw with
+//            v and rho corrected along the shift (into B)
+//   fsc      surface and near-surface particles: Shepard-blended density on
+//            the same neighbour sets at the shifted positions (into C)
+//   commit   A = (B.r, B.v, C.rho), pair records, the next step size from
+//            the last stage's acceleration, and the shift into the
+//            displacement bound of the cell index.
+#include <cfloat>
+
+#include "pair_math.hpp"
+#include "sph_kernels.hpp"
+
+namespace tit_amd {
+
+namespace {
+
+constexpr int kPiBlock = 256;
+constexpr double kTiny = 6.0554544523933395e-06;  // cbrt(DBL_EPSILON)
+constexpr double kPhiMax = 1.0;
+
+__device__ __forceinline__ bool same_bits(double a, double b) {
+  return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+// Calls f(j, d, s2) for every particle j (self included) with
+// s2 = |r_j - r_a|^2 <= (2h)^2, d = r_j - r_a, positions from S.
+template <int DIM, class F>
+__device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, const double* xa,
+                                               const DevState& S, F&& f) {
+  const DevTank T = A.tanks[A.tag[i] >> 1];
+  const uint32_t local = A.key[i] - T.cell_off;
+  int c[3];
+  if constexpr (DIM == 2) {
+    c[0] = static_cast<int>(local / T.nc[1]);
+    c[1] = static_cast<int>(local % T.nc[1]);
+  } else {
+    const uint32_t q = local / T.nc[2];
+    c[2] = static_cast<int>(local % T.nc[2]);
+    c[1] = static_cast<int>(q % T.nc[1]);
+    c[0] = static_cast<int>(q / T.nc[1]);
+  }
+  constexpr int L = DIM - 1;
+  const int zlo = max(c[L] - kStencil, 0), zhi = min(c[L] + kStencil, T.nc[L] - 1);
+  const int ylo = DIM == 3 ? max(c[1] - kStencil, 0) : 0;
+  const int yhi = DIM == 3 ? min(c[1] + kStencil, T.nc[1] - 1) : 0;
+  for (int ix = max(c[0] - kStencil, 0); ix <= min(c[0] + kStencil, T.nc[0] - 1); ++ix) {
+    for (int iy = ylo; iy <= yhi; ++iy) {
+      const uint32_t row =
+          DIM == 2 ? T.cell_off + static_cast<uint32_t>(ix) * T.nc[1]
+                   : T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] +
+                                   static_cast<uint32_t>(iy)) * T.nc[2];
+      const uint32_t b = A.cell_off[row + zlo], e = A.cell_off[row + zhi + 1];
+      for (uint32_t j = b; j < e; ++j) {
+        double d[DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) d[k] = S.x[k][j] - xa[k];
+        double s2 = d[0] * d[0];
+#pragma unroll
+        for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
+        if (!(s2 <= A.P.R2)) continue;
+        f(j, d, s2);
+      }
+    }
+  }
+}
+
+template <int KERNEL>
+__device__ __forceinline__ double unit_deriv(double q) {
+  if (!(q < 2.0)) return 0.0;
+  const double t = 1.0 - q / 2.0;
+  if constexpr (KERNEL == kKernelC2) {
+    return -5.0 * q * (t * t * t);
+  } else {
+    const double t5 = t * t * t * t * t;
+    return -14.0 / 3.0 * q * (1.0 + 2.5 * q) * t5;
+  }
+}
+
+template <int DIM, int KERNEL>
+__device__ __forceinline__ double kernel_factor(const DevParams& P) {
+  const double omega = KERNEL == kKernelC2
+                           ? (DIM == 2 ? 7.0 / (4.0 * M_PI) : 21.0 / (16.0 * M_PI))
+                           : (DIM == 2 ? 9.0 / (4.0 * M_PI) : 495.0 / (256.0 * M_PI));
+  const double h_inv = 1.0 / P.h;
+  double hd = h_inv;  // omega / h^d (sph/kernel.hpp:142-151)
+#pragma unroll
+  for (int d = 1; d < DIM; ++d) hd *= h_inv;
+  return omega * hd;
+}
+
+template <int DIM>
+__device__ __forceinline__ void normalize_f64(const double* a, double* out) {
+  double n2 = a[0] * a[0];
+#pragma unroll
+  for (int k = 1; k < DIM; ++k) n2 = n2 + a[k] * a[k];
+  if (n2 >= kTiny * kTiny) {
+    const double s = sqrt(n2);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) out[k] = a[k] / s;
+  } else {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) out[k] = 0.0;
+  }
+}
+
+// lu + FactLU::inverse (core/_mat/fact.hpp:60-107): Doolittle factors, near
+// singular when a pivot is tiny, then L U X = I column by column.
+template <int DIM>
+__device__ __forceinline__ bool lu_inverse(const double* M, double* inv) {
+  double LU[DIM * DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) {
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      double x = M[i * DIM + j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) x -= LU[i * DIM + k] * LU[k * DIM + j];
+      LU[i * DIM + j] = x / LU[j * DIM + j];
+    }
+#pragma unroll
+    for (int j = i; j < DIM; ++j) {
+      double x = M[i * DIM + j];
+#pragma unroll
+      for (int k = 0; k < i; ++k) x -= LU[i * DIM + k] * LU[k * DIM + j];
+      LU[i * DIM + j] = x;
+    }
+    if (fabs(LU[i * DIM + i]) <= kTiny) return false;
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    double y[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+      double x = i == c ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) x -= LU[i * DIM + k] * y[k];
+      y[i] = x;
+    }
+#pragma unroll
+    for (int i = DIM - 1; i >= 0; --i) {
+      double x = y[i];
+#pragma unroll
+      for (int k = i + 1; k < DIM; ++k) x -= LU[i * DIM + k] * y[k];
+      y[i] = x / LU[i * DIM + i];
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) inv[i * DIM + c] = y[i];
+  }
+  return true;
+}
+
+// apply_shifts, sums and renormalisation (:335-365)
+template <int DIM, int KERNEL>
+__global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  if (i >= A.n) return;
+  const DevParams& P = A.P;
+  double xa[DIM], va[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    xa[k] = A.a.x[k][i];
+    va[k] = A.a.v[k][i];
+  }
+  const double rho_a = A.a.rho[i];
+  const double w = kernel_factor<DIM, KERNEL>(P), h_inv = 1.0 / P.h;
+  double N[DIM] = {}, Lm[DIM * DIM] = {}, G[DIM * DIM] = {}, g[DIM] = {};
+  uint32_t cnt = 0;
+  for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) {
+    ++cnt;
+    if (j == i) return;
+    const double rho_b = A.a.rho[j];
+    const double V_b = P.m / rho_b;
+    double x_ab[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) x_ab[k] = -d[k];
+    double nx[DIM];
+    normalize_f64<DIM>(x_ab, nx);
+    const double s = w * unit_deriv<KERNEL>(h_inv * sqrt(s2));
+    double gW[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) gW[k] = s * (nx[k] * h_inv);
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+      N[p] += V_b * gW[p];
+      const double v_ba = A.a.v[p][j] - va[p];
+#pragma unroll
+      for (int q = 0; q < DIM; ++q) {
+        Lm[p * DIM + q] += V_b * (d[p] * gW[q]);
+        G[p * DIM + q] += V_b * (v_ba * gW[q]);
+      }
+      g[p] += V_b * (rho_b - rho_a) * gW[p];
+    }
+  });
+  // renormalisation (:354-365)
+  double LT[DIM * DIM], inv[DIM * DIM];
+#pragma unroll
+  for (int p = 0; p < DIM; ++p) {
+    A.dr0[p][i] = N[p];
+#pragma unroll
+    for (int q = 0; q < DIM; ++q) LT[p * DIM + q] = Lm[q * DIM + p];
+  }
+  if (lu_inverse<DIM>(LT, inv)) {
+    double t[DIM * DIM];
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+      double x = 0.0;
+#pragma unroll
+      for (int q = 0; q < DIM; ++q) x += inv[p * DIM + q] * N[q];
+      t[p] = x;
+    }
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) N[p] = t[p];
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+#pragma unroll
+      for (int q = 0; q < DIM; ++q) {
+        double x = 0.0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) x += G[p * DIM + k] * inv[q * DIM + k];
+        t[p * DIM + q] = x;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < DIM * DIM; ++k) G[k] = t[k];
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+      double x = 0.0;
+#pragma unroll
+      for (int q = 0; q < DIM; ++q) x += inv[p * DIM + q] * g[q];
+      t[p] = x;
+    }
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) g[p] = t[p];
+  }
+  double nn[DIM];
+  normalize_f64<DIM>(N, nn);
+#pragma unroll
+  for (int p = 0; p < DIM; ++p) {
+    A.N[p][i] = nn[p];
+    A.grad_rho[p][i] = g[p];
+#pragma unroll
+    for (int q = 0; q < DIM; ++q) A.grad_v[p * DIM + q][i] = G[p * DIM + q];
+  }
+  A.count[i] = cnt;
+}
+
+// Free-surface indicator (:375-426): far when a neighbour lies inside the
+// 45-degree cone behind the particle along N; splashes are on the surface.
+template <int DIM>
+__global__ __launch_bounds__(kPiBlock) void k_pi_phi(PiArgs A) {
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  if (i >= A.n) return;
+  if (A.tag[i] & 1u) {
+    A.phi[i] = kPhiMax;
+    return;
+  }
+  double phi = DBL_MIN;
+  const uint32_t cutoff = DIM == 2 ? 8u : 26u;
+  if (A.count[i] > cutoff) {
+    double xa[DIM], Na[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      xa[k] = A.a.x[k][i];
+      Na[k] = A.N[k][i];
+    }
+    const double cos_fov = 0.7071067811865476;  // cos(pi / 4) in FP64
+    for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) {
+      if (j == i || !same_bits(phi, DBL_MIN)) return;
+      const double fov = (cos_fov * cos_fov) * s2;
+      double n_a = Na[0] * -d[0];
+#pragma unroll
+      for (int k = 1; k < DIM; ++k) n_a = n_a + Na[k] * -d[k];
+      if (n_a > 0.0 && n_a * n_a >= fov) phi = kPhiMax;
+    });
+  }
+  A.phi[i] = phi;
+}
+
+// Near-surface classification and the shifts (:439-470), into B.
+template <int DIM>
+__global__ __launch_bounds__(kPiBlock) void k_pi_shift(PiArgs A) {
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  if (i >= A.n) return;
+  const DevParams& P = A.P;
+  double xa[DIM], va[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    xa[k] = A.a.x[k][i];
+    va[k] = A.a.v[k][i];
+  }
+  double rho = A.a.rho[i];
+  double phi = A.phi[i];
+  const bool fluid = !(A.tag[i] & 1u);
+  if (fluid && same_bits(phi, kPhiMax)) {
+    // the nearest surface neighbour; the reference's lists are in index
+    // order, so equal distances go to the smaller index
+    uint32_t best = ~0u, best_id = ~0u;
+    double best_d2 = 0.0, bx[DIM] = {};
+    for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) {
+      if (!same_bits(A.phi[j], DBL_MIN)) return;
+      const uint32_t jid = A.id[j];
+      if (best == ~0u || s2 < best_d2 || (s2 == best_d2 && jid < best_id)) {
+        best = j;
+        best_id = jid;
+        best_d2 = s2;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) bx[k] = -d[k];
+      }
+    });
+    if (best != ~0u) {
+      double nb = A.N[0][best] * bx[0];
+#pragma unroll
+      for (int k = 1; k < DIM; ++k) nb = nb + A.N[k][best] * bx[k];
+      phi *= fabs(nb) / (2.0 * P.h);
+    }
+  }
+  A.phi_near[i] = phi;
+  if (fluid && same_bits(phi, kPhiMax)) {
+    const double coef = -0.4 * 0.2 * (P.h * P.h);  // -CFL_ C_shift_ h^2
+    double dr[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) dr[k] = A.dr0[k][i] * coef;
+    double dv[DIM];
+#pragma unroll
+    for (int p = 0; p < DIM; ++p) {
+      double x = 0.0;
+#pragma unroll
+      for (int q = 0; q < DIM; ++q) x += A.grad_v[p * DIM + q][i] * dr[q];
+      dv[p] = x;
+    }
+    double drho = A.grad_rho[0][i] * dr[0];
+#pragma unroll
+    for (int k = 1; k < DIM; ++k) drho = drho + A.grad_rho[k][i] * dr[k];
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      xa[k] += dr[k];
+      va[k] += dv[k];
+      d2 += dr[k] * dr[k];
+    }
+    rho += drho;
+    atomicMax(A.shift_max + (A.tag[i] >> 1), dbl_bits(sqrt(d2)));
+  }
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    A.b.x[k][i] = xa[k];
+    A.b.v[k][i] = va[k];
+  }
+  A.b.rho[i] = rho;
+}
+
+// apply_free_surface_correction (:483-512): on the neighbour sets of the
+// unshifted positions, with W at the shifted ones; densities into C.
+template <int DIM, int KERNEL>
+__global__ __launch_bounds__(kPiBlock) void k_pi_fsc(PiArgs A) {
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  if (i >= A.n) return;
+  const DevParams& P = A.P;
+  double rho = A.b.rho[i];
+  if (!(A.tag[i] & 1u) && !same_bits(A.phi_near[i], kPhiMax)) {
+    double xa[DIM], xs[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      xa[k] = A.a.x[k][i];
+      xs[k] = A.b.x[k][i];
+    }
+    const double w = kernel_factor<DIM, KERNEL>(P), h_inv = 1.0 / P.h;
+    double alpha = 0.0, rho_t = 0.0;
+    for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double*, double) {
+      double x2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        const double x = xs[k] - A.b.x[k][j];
+        x2 += x * x;
+      }
+      const double W = w * kernel_value_poly_f64<KERNEL>(h_inv * sqrt(x2));
+      alpha += P.m / A.b.rho[j] * W;
+      rho_t += P.m * W;
+    });
+    const double ratio = fmin(1.0, alpha);
+    if (!(ratio > 0.99)) {
+      const double K = -log(0.05) / (0.01 * 0.01);
+      const double beta = exp(-K * ((ratio - 1.0) * (ratio - 1.0)));
+      const double corr = beta + (1.0 - beta) * alpha;
+      if (fabs(corr) > kTiny) rho = rho_t / corr;
+    }
+  }
+  A.c.rho[i] = rho;
+}
+
+// Shift displacement into the index's bound; the next step size is taken
+// again after the shifts.
+__global__ void k_pi_bound(int n_tanks, unsigned long long* shift_max, double* disp,
+                           double skin_half, uint32_t* flags, unsigned long long* dt_min) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tanks) return;
+  const double d = disp[t] + __longlong_as_double(static_cast<long long>(shift_max[t]));
+  disp[t] = d;
+  shift_max[t] = 0ull;
+  dt_min[t] = ~0ull;
+  if (!(d <= skin_half)) atomicOr(flags, 1u);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kPiBlock) void k_pi_commit(PiArgs A) {
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  const bool in_range = i < A.n;
+  const uint32_t tg = in_range ? A.tag[i] : 1u;
+  const bool fluid = in_range && !(tg & 1u);
+  unsigned long long dtb = ~0ull;
+  float vm = 0.0f;
+  if (in_range) {
+    double v[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      v[k] = A.b.v[k][i];
+      A.a.x[k][i] = A.b.x[k][i];
+      A.a.v[k][i] = v[k];
+    }
+    const double rho = A.c.rho[i];
+    A.a.rho[i] = rho;
+    write_pair_data<DIM>(A.P, A.a, i, v, rho);
+    if (fluid) {
+      double dv[DIM], v2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        dv[k] = A.dv_last[k][i];
+        v2 += v[k] * v[k];
+      }
+      dtb = dbl_bits(particle_time_step<DIM>(A.P, rho, v, dv));
+      vm = static_cast<float>(sqrt(v2));
+    }
+  }
+  if (fluid) {
+    const uint32_t t = tg >> 1;
+    atomicMin(A.dt_min + t, dtb);
+    atomicMax(A.vmax + t, __float_as_uint(vm));
+  }
+}
+
+__global__ void k_export_f64(uint32_t n, const uint32_t* __restrict__ id,
+                             const double* __restrict__ src, double* __restrict__ dst) {
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  if (i < n) dst[id[i]] = src[i];
+}
+
+}  // namespace
+
+void launch_export_f64(hipStream_t s, uint32_t n, const uint32_t* id, const double* src,
+                       double* dst) {
+  if (n) k_export_f64<<<(n + kPiBlock - 1) / kPiBlock, kPiBlock, 0, s>>>(n, id, src, dst);
+}
+
+void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* disp,
+                           double skin_half, uint32_t* flags) {
+  if (a.n == 0) return;
+  const uint32_t nb = (a.n + kPiBlock - 1) / kPiBlock;
+  const int dim = a.P.dim, kernel = a.P.kernel;
+  if (dim == 2) {
+    if (kernel == kKernelC2) {
+      k_pi_sums<2, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
+    } else {
+      k_pi_sums<2, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
+    }
+    k_pi_phi<2><<<nb, kPiBlock, 0, s>>>(a);
+    k_pi_shift<2><<<nb, kPiBlock, 0, s>>>(a);
+    if (kernel == kKernelC2) {
+      k_pi_fsc<2, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
+    } else {
+      k_pi_fsc<2, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
+    }
+  } else {
+    if (kernel == kKernelC2) {
+      k_pi_sums<3, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
+    } else {
+      k_pi_sums<3, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
+    }
+    k_pi_phi<3><<<nb, kPiBlock, 0, s>>>(a);
+    k_pi_shift<3><<<nb, kPiBlock, 0, s>>>(a);
+    if (kernel == kKernelC2) {
+      k_pi_fsc<3, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
+    } else {
+      k_pi_fsc<3, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
+    }
+  }
+  k_pi_bound<<<(n_tanks + 63) / 64, 64, 0, s>>>(n_tanks, a.shift_max, disp, skin_half, flags,
+                                                a.dt_min);
+  if (dim == 2) {
+    k_pi_commit<2><<<nb, kPiBlock, 0, s>>>(a);
+  } else {
+    k_pi_commit<3><<<nb, kPiBlock, 0, s>>>(a);
+  }
+}
+
+}  // namespace tit_amd

@@ -40,6 +40,32 @@ struct StageArgs {
   uint32_t* overflow;          // 2D chunks whose candidates exceed lds_cap
   uint32_t* counts;            // [0] chunks, [1] overflow chunks
   uint32_t grid_lds, grid_global;  // persistent grid sizes
+  double* dv_last[3];          // final stage: its acceleration (post-integration only)
+};
+
+// Post-integration passes (post_integrate.hip): state a (post-step, its
+// positions give the neighbour sets), b (shifted), c (corrected density),
+// FP64 scratch per particle, SoA.
+struct PiArgs {
+  DevParams P;
+  const DevTank* tanks;
+  uint32_t n;
+  DevState a, b, c;
+  const uint32_t* key;
+  const uint32_t* tag;
+  const uint32_t* id;
+  const uint32_t* cell_off;
+  double* N[3];          // renormalised, normalised N
+  double* dr0[3];        // raw N (the unscaled shift)
+  double* grad_v[9];     // renormalised grad v, [p * dim + q] = dv_p / dx_q
+  double* grad_rho[3];   // renormalised grad rho
+  uint32_t* count;       // neighbours, self included
+  double* phi;           // free-surface indicator after the cone test
+  double* phi_near;      // after the near-surface classification
+  unsigned long long* shift_max;  // per tank, FP64 bits
+  double* dv_last[3];    // the last stage's acceleration
+  unsigned long long* dt_min;
+  uint32_t* vmax;
 };
 
 struct BoundaryArgs {
@@ -91,6 +117,13 @@ void launch_restore_order(hipStream_t s, int dim, uint32_t n, const uint32_t* pe
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
                          const uint32_t* key, uint32_t* cell_off, const uint32_t* ctl);
 void launch_boundary(hipStream_t s, const BoundaryArgs& a);
+// post_integrate.hip: the passes in order, the shift into disp[] (flags bit 0
+// when it passes skin_half), the next step size into dt_min.
+void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* disp,
+                           double skin_half, uint32_t* flags);
+// dst[id[i]] = src[i] (FP64).
+void launch_export_f64(hipStream_t s, uint32_t n, const uint32_t* id, const double* src,
+                       double* dst);
 void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final);
 uint32_t stage_lds_capacity(int dim);
 void stage_grids(const DevParams& P, int n_cu, uint32_t* grid_lds,

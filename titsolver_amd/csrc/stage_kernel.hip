@@ -623,7 +623,13 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
       }
       A.out.rho[i] = rhon;
       write_pair_data<DIM>(P, A.out, i, vn, rhon);
-      if constexpr (FINAL) dtmin = dbl_bits(particle_time_step<DIM>(P, rhon, vn, dv));
+      if constexpr (FINAL) {
+        dtmin = dbl_bits(particle_time_step<DIM>(P, rhon, vn, dv));
+        if (A.dv_last[0]) {  // kept for the time step after post-integration
+#pragma unroll
+          for (int k = 0; k < DIM; ++k) A.dv_last[k][i] = dv[k];
+        }
+      }
     }
   }
   if constexpr (KIND != kRhsOnly) {

@@ -56,7 +56,8 @@ class FluidEquations:
     c_force: float = 0.25
     c_visc: float = 0.125
 
-    def to_params(self, order: int = 3, skin: float = 0.0) -> _abi.Params:
+    def to_params(self, order: int = 3, skin: float = 0.0,
+                  post_integrate: bool = False) -> _abi.Params:
         p = _abi.Params()
         p.dim, p.kernel, p.flavour, p.ssprk_order = (self.dim, self.kernel,
                                                     self.flavour, order)
@@ -64,6 +65,7 @@ class FluidEquations:
                      "delta", "cfl", "c_force", "c_visc"):
             setattr(p, name, float(getattr(self, name)))
         p.skin = float(skin)
+        p.post_integrate = 1 if post_integrate else 0
         return p
 
 
@@ -134,10 +136,10 @@ class HipContext:
     """One tit_hip_ctx: device buffers, stream and captured step graph."""
 
     def __init__(self, equations: FluidEquations, order: int = 3,
-                 device: int = 0, skin: float = 0.0):
+                 device: int = 0, skin: float = 0.0, post_integrate: bool = False):
         self.lib = _abi.load()
         self.eq = equations
-        self.params = equations.to_params(order, skin)
+        self.params = equations.to_params(order, skin, post_integrate)
         h = C.c_void_p()
         st = self.lib.tit_hip_create(device, C.byref(self.params), C.byref(h))
         if st != 0:
@@ -213,6 +215,13 @@ class HipContext:
         dv = np.empty((self.n, self.eq.dim))
         self._check(self.lib.tit_hip_compute_rhs(self.h, _dp(drho), _dp(dv)))
         return drho, dv
+
+    def post_integrate(self) -> np.ndarray:
+        """FluidEquations::post_integrate on the current state (context made
+        with post_integrate=True); returns the free-surface indicator phi."""
+        phi = np.empty(self.n)
+        self._check(self.lib.tit_hip_post_integrate(self.h, _dp(phi)))
+        return phi
 
     def neighbors(self):
         off = np.zeros(self.n + 1, dtype=np.uint64)
