@@ -89,6 +89,9 @@ def parse():
     ap.add_argument("--learned-steps", type=int, default=50,
                     help="timed learned steps (4 dt each; judge every 10)")
     ap.add_argument("--train-iters", type=int, default=2400)
+    ap.add_argument("--post-integrate", action="store_true",
+                    help="end every step of the workload with post_integrate "
+                         "(particle shifting and the free-surface correction)")
     ap.add_argument("--eager", action="store_true",
                     help="no hipGraph replay: every launch eager with HIP events "
                          "(for profiler runs)")
@@ -128,11 +131,13 @@ def main():
     global EAGER
     EAGER = args.eager
     arrays, eq, desc = build_workload(args)
+    if args.post_integrate:
+        desc += " + post_integrate (particle shifting, free-surface correction)"
     pa = arrays[0]
     nf = sum(a.n_fluid for a in arrays)
     n_fixed = sum(a.n_fixed for a in arrays)
     dim = eq.dim
-    ctx = make_ctx(eq, arrays)
+    ctx = make_ctx(eq, arrays, args.post_integrate)
     print(f"bench: {desc}: {nf} fluid + {n_fixed} fixed particles uploaded; "
           f"warmup {args.warmup}", file=sys.stderr, flush=True)
     ctx.step(args.warmup)
@@ -202,16 +207,17 @@ def main():
     if args.copy_probe_mb > 0:
         out["copy_probe"] = copy_probe(args.copy_probe_mb)
     if not args.no_cpu and args.cpu_steps > 0 and len(arrays) == 1:
-        out["cpu_baseline"], out["parity"] = cpu_baseline(pa, eq, args.cpu_steps)
+        out["cpu_baseline"], out["parity"] = cpu_baseline(pa, eq, args.cpu_steps,
+                                                          args.post_integrate)
     print(json.dumps(out), flush=True)
 
 
 EAGER = False
 
 
-def make_ctx(eq, arrays):
+def make_ctx(eq, arrays, post_integrate=False):
     import titsolver_amd as ta
-    ctx = ta.HipContext(eq, order=3)
+    ctx = ta.HipContext(eq, order=3, post_integrate=post_integrate)
     ctx.upload(arrays)
     if EAGER:
         ctx.set_profiling(True)
@@ -334,7 +340,7 @@ def learned_leg(args):
     return res
 
 
-def cpu_baseline(pa, eq, steps):
+def cpu_baseline(pa, eq, steps, post_integrate=False):
     """FP64 oracle (single thread) on the same workload for a few steps, and
     the GPU's error against it after the same steps from the same state."""
     import oracle
@@ -344,6 +350,7 @@ def cpu_baseline(pa, eq, steps):
     print(f"bench: CPU oracle {steps} steps on {pa.n} particles", file=sys.stderr,
           flush=True)
     prm = oracle.params(eq)
+    prm.post_integrate = 1 if post_integrate else 0
     st = oracle.State.from_particles(pa)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -353,7 +360,7 @@ def cpu_baseline(pa, eq, steps):
            "kind": "port",
            "sample": f"{steps} SSPRK3 steps of the same {nf}-fluid-particle dam break "
                      f"(FP64 oracle, grid search per stage), {wall:.1f} s"}
-    ctx = ta.HipContext(eq, order=3)
+    ctx = ta.HipContext(eq, order=3, post_integrate=post_integrate)
     ctx.upload([pa])
     ctx.step(steps)
     g = ctx.download()

@@ -425,13 +425,17 @@ void enqueue_post_integrate(tit_hip_ctx* c) {
   for (int k = 0; k < dim; ++k, p += n) a.grad_rho[k] = p;
   a.phi = p;
   a.phi_near = p + n;
+  a.vol = p + 2 * n;
   a.count = c->pi_count;
+  a.cell_mark = c->pi_cells;
+  a.n_cells = c->n_cells;
   a.shift_max = c->pi_shift;
   for (int k = 0; k < dim; ++k) a.dv_last[k] = c->rhs_dv[k];
   a.dt_min = c->dt_min;
   a.vmax = c->vmax;
   ProfScope ps(c->prof, c->stream, "post_integrate");
-  launch_post_integrate(c->stream, a, c->n_tanks, c->disp, c->dp.skin_half, c->counter + 1);
+  launch_post_integrate(c->stream, a, c->n_tanks, c->disp, c->dp.skin_half, c->counter + 1,
+                        c->n_cu);
 }
 
 tit_status check_launch(tit_hip_ctx* c, const char* what) {
@@ -702,9 +706,10 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->rhs_count, n));
   for (int k = 0; k < dim; ++k) TIT_HIP_TRY(c, dalloc(c, &c->rhs_dv[k], n));
   if (c->prm.post_integrate) {
-    const size_t per = static_cast<size_t>(3 * dim + dim * dim + 2);
+    const size_t per = static_cast<size_t>(3 * dim + dim * dim + 3);
     TIT_HIP_TRY(c, dalloc(c, &c->pi_buf, per * n));
     TIT_HIP_TRY(c, dalloc(c, &c->pi_count, n));
+    TIT_HIP_TRY(c, dalloc(c, &c->pi_cells, static_cast<size_t>(c->n_cells) + 1));
     TIT_HIP_TRY(c, dalloc(c, &c->pi_shift, n_tanks));
     TIT_HIP_TRY(c, hipMemsetAsync(c->pi_shift, 0, n_tanks * sizeof(unsigned long long),
                                   c->stream));

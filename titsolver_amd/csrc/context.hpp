@@ -62,6 +62,7 @@ struct tit_hip_ctx {
   // rhs_dv holds the last stage's acceleration between steps then
   double* pi_buf = nullptr;  // N, dr0, grad v, grad rho, phi, phi_near (SoA)
   uint32_t* pi_count = nullptr;
+  uint32_t* pi_cells = nullptr;  // per cell mark
   unsigned long long* pi_shift = nullptr;  // per tank
 
   // chunk schedule of the stage kernel

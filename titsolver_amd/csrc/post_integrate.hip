@@ -20,7 +20,9 @@
 //   commit   A = (B.r, B.v, C.rho), pair records, the next step size from
 //            the last stage's acceleration, and the shift into the
 //            displacement bound of the cell index.
+#include <algorithm>
 #include <cfloat>
+#include <type_traits>
 
 #include "pair_math.hpp"
 #include "sph_kernels.hpp"
@@ -37,9 +39,44 @@ __device__ __forceinline__ bool same_bits(double a, double b) {
   return __double_as_longlong(a) == __double_as_longlong(b);
 }
 
+// Per-tank atomic min / max of a wave's values (every lane of the wave must
+// call it; `part` marks the lanes with a value).  A tank's particles are
+// contiguous, so the participating lanes of a wave nearly always share one
+// tank: then the wave reduces first and one lane updates the tank's slot.
+template <bool MAX, class T>
+__device__ __forceinline__ void tank_atomic(bool part, uint32_t t, T v, T* slot) {
+  const unsigned long long m = __ballot(part);
+  if (m == 0) return;
+  const int first = __ffsll(static_cast<long long>(m)) - 1;
+  const uint32_t t0 = __shfl(t, first);
+  if (__all(!part || t == t0)) {
+    T r = part ? v : (MAX ? T(0) : ~T(0));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const T o = __shfl_xor(r, off);
+      r = MAX ? (o > r ? o : r) : (o < r ? o : r);
+    }
+    if ((threadIdx.x & 63) == static_cast<unsigned>(first)) {
+      if constexpr (MAX) {
+        atomicMax(slot + t0, r);
+      } else {
+        atomicMin(slot + t0, r);
+      }
+    }
+  } else if (part) {
+    if constexpr (MAX) {
+      atomicMax(slot + t, v);
+    } else {
+      atomicMin(slot + t, v);
+    }
+  }
+}
+
 // Calls f(j, d, s2) for every particle j (self included) with
-// s2 = |r_j - r_a|^2 <= (2h)^2, d = r_j - r_a, positions from S.
-template <int DIM, class F>
+// s2 = |r_j - r_a|^2 <= (2h)^2, d = r_j - r_a, positions from S; a callback
+// that returns bool stops the walk by returning true.  With ONLY_MARKED,
+// only the stencil columns with a marked cell are visited.
+template <int DIM, bool ONLY_MARKED = false, class F>
 __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, const double* xa,
                                                const DevState& S, F&& f) {
   const DevTank T = A.tanks[A.tag[i] >> 1];
@@ -64,6 +101,11 @@ __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, cons
           DIM == 2 ? T.cell_off + static_cast<uint32_t>(ix) * T.nc[1]
                    : T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] +
                                    static_cast<uint32_t>(iy)) * T.nc[2];
+      if constexpr (ONLY_MARKED) {
+        bool any = false;
+        for (int z = zlo; z <= zhi; ++z) any = any || A.cell_mark[row + z] != 0u;
+        if (!any) continue;
+      }
       const uint32_t b = A.cell_off[row + zlo], e = A.cell_off[row + zhi + 1];
       for (uint32_t j = b; j < e; ++j) {
         double d[DIM];
@@ -73,7 +115,11 @@ __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, cons
 #pragma unroll
         for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
         if (!(s2 <= A.P.R2)) continue;
-        f(j, d, s2);
+        if constexpr (std::is_same_v<decltype(f(j, d, s2)), bool>) {
+          if (f(j, d, s2)) return;
+        } else {
+          f(j, d, s2);
+        }
       }
     }
   }
@@ -164,6 +210,11 @@ __device__ __forceinline__ bool lu_inverse(const double* M, double* inv) {
   return true;
 }
 
+__global__ __launch_bounds__(kPiBlock) void k_pi_volume(PiArgs A) {
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  if (i < A.n) A.vol[i] = A.P.m / A.a.rho[i];
+}
+
 // apply_shifts, sums and renormalisation (:335-365)
 template <int DIM, int KERNEL>
 __global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
@@ -184,13 +235,14 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
     ++cnt;
     if (j == i) return;
     const double rho_b = A.a.rho[j];
-    const double V_b = P.m / rho_b;
-    double x_ab[DIM];
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) x_ab[k] = -d[k];
+    const double V_b = A.vol[j];
+    // unit direction of r_ab (zero below the tiny norm, as normalize does)
+    const double r = sqrt(s2);
+    const double inv_r = s2 >= kTiny * kTiny ? 1.0 / r : 0.0;
     double nx[DIM];
-    normalize_f64<DIM>(x_ab, nx);
-    const double s = w * unit_deriv<KERNEL>(h_inv * sqrt(s2));
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) nx[k] = -d[k] * inv_r;
+    const double s = w * unit_deriv<KERNEL>(h_inv * r);
     double gW[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) gW[k] = s * (nx[k] * h_inv);
@@ -279,23 +331,70 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_phi(PiArgs A) {
       Na[k] = A.N[k][i];
     }
     const double cos_fov = 0.7071067811865476;  // cos(pi / 4) in FP64
-    for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) {
-      if (j == i || !same_bits(phi, DBL_MIN)) return;
+    for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) -> bool {
+      if (j == i) return false;
       const double fov = (cos_fov * cos_fov) * s2;
       double n_a = Na[0] * -d[0];
 #pragma unroll
       for (int k = 1; k < DIM; ++k) n_a = n_a + Na[k] * -d[k];
-      if (n_a > 0.0 && n_a * n_a >= fov) phi = kPhiMax;
+      if (n_a > 0.0 && n_a * n_a >= fov) {
+        phi = kPhiMax;
+        return true;  // any one neighbour in the cone decides
+      }
+      return false;
     });
   }
   A.phi[i] = phi;
+  if (same_bits(phi, DBL_MIN)) A.cell_mark[A.key[i]] = 1u;  // a surface particle's cell
+}
+
+__global__ __launch_bounds__(kPiBlock) void k_pi_clear(uint32_t n, uint32_t* mark) {
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  if (i < n) mark[i] = 0u;
+}
+
+// A lane's running per-tank value: the lanes of a persistent workgroup walk
+// a contiguous range of particles, so a lane's tank changes only at tank
+// boundaries (then the old tank's value goes out with one atomic); at the
+// end each wave combines its lanes' values (one atomic when they share a
+// tank).
+template <bool MAX, class T>
+struct TankRun {
+  uint32_t t = ~0u;
+  T v = MAX ? T(0) : ~T(0);
+  __device__ __forceinline__ void add(uint32_t tank, T x, T* slot) {
+    if (tank != t) {
+      if (t != ~0u) {
+        if constexpr (MAX) {
+          atomicMax(slot + t, v);
+        } else {
+          atomicMin(slot + t, v);
+        }
+      }
+      t = tank;
+      v = x;
+    } else {
+      v = MAX ? (x > v ? x : v) : (x < v ? x : v);
+    }
+  }
+  __device__ __forceinline__ void flush(T* slot) { tank_atomic<MAX>(t != ~0u, t, v, slot); }
+};
+
+// This workgroup's contiguous share of the n particles, walked in steps of
+// the workgroup size (a uniform trip count).
+struct Share {
+  uint32_t beg, end;
+};
+__device__ __forceinline__ Share share(uint32_t n) {
+  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t beg = min(n, blockIdx.x * per);
+  return Share{beg, min(n, beg + per)};
 }
 
 // Near-surface classification and the shifts (:439-470), into B.
 template <int DIM>
-__global__ __launch_bounds__(kPiBlock) void k_pi_shift(PiArgs A) {
-  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
-  if (i >= A.n) return;
+__device__ __forceinline__ void shift_one(const PiArgs& A, uint32_t i,
+                                          TankRun<true, unsigned long long>& run) {
   const DevParams& P = A.P;
   double xa[DIM], va[DIM];
 #pragma unroll
@@ -311,7 +410,8 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_shift(PiArgs A) {
     // order, so equal distances go to the smaller index
     uint32_t best = ~0u, best_id = ~0u;
     double best_d2 = 0.0, bx[DIM] = {};
-    for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) {
+    // only columns with a surface particle's cell can hold one
+    for_neighbours<DIM, true>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) {
       if (!same_bits(A.phi[j], DBL_MIN)) return;
       const uint32_t jid = A.id[j];
       if (best == ~0u || s2 < best_d2 || (s2 == best_d2 && jid < best_id)) {
@@ -354,7 +454,7 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_shift(PiArgs A) {
       d2 += dr[k] * dr[k];
     }
     rho += drho;
-    atomicMax(A.shift_max + (A.tag[i] >> 1), dbl_bits(sqrt(d2)));
+    run.add(A.tag[i] >> 1, dbl_bits(sqrt(d2)), A.shift_max);
   }
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
@@ -362,6 +462,17 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_shift(PiArgs A) {
     A.b.v[k][i] = va[k];
   }
   A.b.rho[i] = rho;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kPiBlock) void k_pi_shift(PiArgs A) {
+  const Share sh = share(A.n);
+  TankRun<true, unsigned long long> run;
+  for (uint32_t base = sh.beg; base < sh.end; base += kPiBlock) {
+    const uint32_t i = base + threadIdx.x;
+    if (i < sh.end) shift_one<DIM>(A, i, run);
+  }
+  run.flush(A.shift_max);
 }
 
 // apply_free_surface_correction (:483-512): on the neighbour sets of the
@@ -418,13 +529,12 @@ __global__ void k_pi_bound(int n_tanks, unsigned long long* shift_max, double* d
 
 template <int DIM>
 __global__ __launch_bounds__(kPiBlock) void k_pi_commit(PiArgs A) {
-  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
-  const bool in_range = i < A.n;
-  const uint32_t tg = in_range ? A.tag[i] : 1u;
-  const bool fluid = in_range && !(tg & 1u);
-  unsigned long long dtb = ~0ull;
-  float vm = 0.0f;
-  if (in_range) {
+  const Share sh = share(A.n);
+  TankRun<false, unsigned long long> dt_run;
+  TankRun<true, uint32_t> v_run;
+  for (uint32_t base = sh.beg; base < sh.end; base += kPiBlock) {
+    const uint32_t i = base + threadIdx.x;
+    if (i >= sh.end) continue;
     double v[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
@@ -435,22 +545,20 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_commit(PiArgs A) {
     const double rho = A.c.rho[i];
     A.a.rho[i] = rho;
     write_pair_data<DIM>(A.P, A.a, i, v, rho);
-    if (fluid) {
+    const uint32_t tg = A.tag[i];
+    if (!(tg & 1u)) {
       double dv[DIM], v2 = 0.0;
 #pragma unroll
       for (int k = 0; k < DIM; ++k) {
         dv[k] = A.dv_last[k][i];
         v2 += v[k] * v[k];
       }
-      dtb = dbl_bits(particle_time_step<DIM>(A.P, rho, v, dv));
-      vm = static_cast<float>(sqrt(v2));
+      dt_run.add(tg >> 1, dbl_bits(particle_time_step<DIM>(A.P, rho, v, dv)), A.dt_min);
+      v_run.add(tg >> 1, __float_as_uint(static_cast<float>(sqrt(v2))), A.vmax);
     }
   }
-  if (fluid) {
-    const uint32_t t = tg >> 1;
-    atomicMin(A.dt_min + t, dtb);
-    atomicMax(A.vmax + t, __float_as_uint(vm));
-  }
+  dt_run.flush(A.dt_min);
+  v_run.flush(A.vmax);
 }
 
 __global__ void k_export_f64(uint32_t n, const uint32_t* __restrict__ id,
@@ -467,18 +575,22 @@ void launch_export_f64(hipStream_t s, uint32_t n, const uint32_t* id, const doub
 }
 
 void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* disp,
-                           double skin_half, uint32_t* flags) {
+                           double skin_half, uint32_t* flags, int n_cu) {
   if (a.n == 0) return;
   const uint32_t nb = (a.n + kPiBlock - 1) / kPiBlock;
+  // the reducing passes run persistent: 8 workgroups per CU at most
+  const uint32_t np = std::min<uint32_t>(nb, 8u * static_cast<uint32_t>(std::max(n_cu, 1)));
   const int dim = a.P.dim, kernel = a.P.kernel;
+  k_pi_volume<<<nb, kPiBlock, 0, s>>>(a);
   if (dim == 2) {
     if (kernel == kKernelC2) {
       k_pi_sums<2, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
     } else {
       k_pi_sums<2, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
     }
+    k_pi_clear<<<(a.n_cells + kPiBlock - 1) / kPiBlock, kPiBlock, 0, s>>>(a.n_cells, a.cell_mark);
     k_pi_phi<2><<<nb, kPiBlock, 0, s>>>(a);
-    k_pi_shift<2><<<nb, kPiBlock, 0, s>>>(a);
+    k_pi_shift<2><<<np, kPiBlock, 0, s>>>(a);
     if (kernel == kKernelC2) {
       k_pi_fsc<2, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
     } else {
@@ -490,8 +602,9 @@ void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* 
     } else {
       k_pi_sums<3, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
     }
+    k_pi_clear<<<(a.n_cells + kPiBlock - 1) / kPiBlock, kPiBlock, 0, s>>>(a.n_cells, a.cell_mark);
     k_pi_phi<3><<<nb, kPiBlock, 0, s>>>(a);
-    k_pi_shift<3><<<nb, kPiBlock, 0, s>>>(a);
+    k_pi_shift<3><<<np, kPiBlock, 0, s>>>(a);
     if (kernel == kKernelC2) {
       k_pi_fsc<3, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
     } else {
@@ -501,9 +614,9 @@ void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* 
   k_pi_bound<<<(n_tanks + 63) / 64, 64, 0, s>>>(n_tanks, a.shift_max, disp, skin_half, flags,
                                                 a.dt_min);
   if (dim == 2) {
-    k_pi_commit<2><<<nb, kPiBlock, 0, s>>>(a);
+    k_pi_commit<2><<<np, kPiBlock, 0, s>>>(a);
   } else {
-    k_pi_commit<3><<<nb, kPiBlock, 0, s>>>(a);
+    k_pi_commit<3><<<np, kPiBlock, 0, s>>>(a);
   }
 }
 

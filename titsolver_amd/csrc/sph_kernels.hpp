@@ -62,6 +62,9 @@ struct PiArgs {
   uint32_t* count;       // neighbours, self included
   double* phi;           // free-surface indicator after the cone test
   double* phi_near;      // after the near-surface classification
+  uint32_t* cell_mark;   // per cell: holds a surface particle
+  double* vol;           // m / rho per particle
+  uint32_t n_cells;
   unsigned long long* shift_max;  // per tank, FP64 bits
   double* dv_last[3];    // the last stage's acceleration
   unsigned long long* dt_min;
@@ -120,7 +123,7 @@ void launch_boundary(hipStream_t s, const BoundaryArgs& a);
 // post_integrate.hip: the passes in order, the shift into disp[] (flags bit 0
 // when it passes skin_half), the next step size into dt_min.
 void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* disp,
-                           double skin_half, uint32_t* flags);
+                           double skin_half, uint32_t* flags, int n_cu);
 // dst[id[i]] = src[i] (FP64).
 void launch_export_f64(hipStream_t s, uint32_t n, const uint32_t* id, const double* src,
                        double* dst);
