@@ -31,6 +31,7 @@ namespace {
 constexpr int kCols2 = 2 * kStencil + 1;
 constexpr int kRowsMax = 96;   // LDS cell-offset rows of one chunk
 constexpr int kWords = 5;      // acceptance masks held per lane (LDS path)
+constexpr int kWordsG = 24;    // 32-candidate masks held per lane (global path)
 constexpr uint32_t kWordSpan = 1u << 22;  // candidate global-index span of an LDS chunk
 static_assert(kStageLdsCap <= 1024, "LDS index packs into 10 bits");
 
@@ -69,7 +70,9 @@ struct GlobalPos {
   const DevTank& T;
   const DevState& S;
   const uint32_t* coff;
-  int cx, cy;  // target column
+  int cx, cy;    // target column
+  uint32_t* wm;  // this wave's [kWordsG][64] 32-candidate acceptance masks
+  uint32_t* wj;  // global index of each mask's bit 0
 
   __device__ __forceinline__ bool range(int k, int w0, int w1, uint32_t& b,
                                         uint32_t& e) const {
@@ -465,6 +468,137 @@ __device__ __forceinline__ void gather_words(const DevParams& P, const LdsPos& p
   }
 }
 
+// Global-memory path, the same scheme with 32-candidate masks (the 3D stencil
+// has 25 short column ranges): all columns' accepted candidates are summed
+// in one loop per lane, in the same order as gather().
+template <int DIM, int KERNEL, int FLAVOUR>
+__device__ __forceinline__ void sum_words_g(const DevParams& P, const GlobalPos<DIM>& ps,
+                                            const DevState& S, const Target<DIM>& a,
+                                            uint32_t n_acc, Sums<DIM>& acc) {
+  const int lane = threadIdx.x & 63;
+  int w = 0;
+  uint32_t m = 0, m_next = ps.wm[lane], j0 = 0, j_next = ps.wj[lane];
+  // the next of the n_acc accepted candidates (stored masks are never empty)
+  auto take = [&](PairIn<DIM>& p) {
+    if (m == 0) {
+      m = m_next;
+      j0 = j_next;
+      w = min(w + 1, kWordsG - 1);
+      m_next = ps.wm[w * 64 + lane];
+      j_next = ps.wj[w * 64 + lane];
+    }
+    const uint32_t j = j0 + static_cast<uint32_t>(__builtin_ctz(m));
+    m &= m - 1u;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) p.x[k] = S.x[k][j];
+    p.pv = S.pv[j];
+    p.aux = S.aux[j];
+  };
+  uint32_t rem = n_acc;
+  PairIn<DIM> A, B;
+  if (rem == 0) return;
+  if (acc.pend) {  // an odd term of an earlier batch of masks waits for its partner
+    take(A);
+    sum_pair<DIM, KERNEL, FLAVOUR>(P, a, A, acc);
+    if (--rem == 0) return;
+  }
+  take(A);
+  --rem;
+  while (true) {
+    const bool hb = rem > 0;
+    if (hb) {
+      take(B);
+      --rem;
+    }
+    float ca, fa[DIM];
+    term_of<DIM, KERNEL, FLAVOUR>(P, a, A, ca, fa);
+    if (!hb) {
+      acc.add(ca, fa);
+      break;
+    }
+    const bool ha = rem > 0;
+    if (ha) {
+      take(A);
+      --rem;
+    }
+    float cb, fb[DIM];
+    term_of<DIM, KERNEL, FLAVOUR>(P, a, B, cb, fb);
+    acc.add2(ca, fa, cb, fb);
+    if (!ha) break;
+  }
+}
+
+template <int DIM, int KERNEL, int FLAVOUR, int NCOL>
+__device__ __forceinline__ void gather_words_g(const DevParams& P, const GlobalPos<DIM>& ps,
+                                               const DevState& S, const Target<DIM>& a,
+                                               uint32_t i, int zc, const Windows<DIM, NCOL>& W,
+                                               Sums<DIM>& acc, uint32_t& count) {
+  const int lane = threadIdx.x & 63;
+  int k = 0;
+  uint32_t c0 = 0, e = 0;
+  bool more = true;
+  while (more) {
+    int nw = 0;
+    uint32_t n_acc = 0;
+    while (nw < kWordsG) {
+      if (c0 >= e) {  // next non-empty column range
+        bool found = false;
+        for (; k < NCOL && !found; ++k) {
+          const uint32_t v = W.get(k);
+          if ((v & 7u) == 7u) continue;
+          const int w0 = zc - kStencil + static_cast<int>(v & 7u);
+          const int w1 = zc - kStencil + static_cast<int>(v >> 3);
+          uint32_t b, eb;
+          if (ps.range(k, w0, w1, b, eb) && b < eb) {
+            found = true;
+            c0 = b;
+            e = eb;
+          }
+        }
+        if (!found) {
+          more = false;
+          break;
+        }
+      }
+      const uint32_t len = min(e - c0, 32u);
+      uint32_t mask = 0;
+      uint32_t t = 0;
+      for (; t + 4 <= len; t += 4) {
+        double x0[DIM], x1[DIM], x2[DIM], x3[DIM];
+#pragma unroll
+        for (int q = 0; q < DIM; ++q) {
+          x0[q] = S.x[q][c0 + t];
+          x1[q] = S.x[q][c0 + t + 1];
+          x2[q] = S.x[q][c0 + t + 2];
+          x3[q] = S.x[q][c0 + t + 3];
+        }
+        const uint32_t mm = (within<DIM>(P, x0, a.x) ? 1u : 0u) |
+                            (within<DIM>(P, x1, a.x) ? 2u : 0u) |
+                            (within<DIM>(P, x2, a.x) ? 4u : 0u) |
+                            (within<DIM>(P, x3, a.x) ? 8u : 0u);
+        mask |= mm << t;
+      }
+      for (; t < len; ++t) {
+        double x0[DIM];
+#pragma unroll
+        for (int q = 0; q < DIM; ++q) x0[q] = S.x[q][c0 + t];
+        if (within<DIM>(P, x0, a.x)) mask |= 1u << t;
+      }
+      if (i >= c0 && i < c0 + len) mask &= ~(1u << (i - c0));  // not self
+      const uint32_t na = static_cast<uint32_t>(__builtin_popcount(mask));
+      count += na;
+      n_acc += na;
+      if (mask) {
+        ps.wm[nw * 64 + lane] = mask;
+        ps.wj[nw * 64 + lane] = c0;
+        ++nw;
+      }
+      c0 += len;
+    }
+    sum_words_g<DIM, KERNEL, FLAVOUR>(P, ps, S, a, n_acc, acc);
+  }
+}
+
 __device__ __forceinline__ float wave_max_f32(float v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
@@ -579,7 +713,7 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
     if constexpr (std::is_same_v<Pos, LdsPos>) {
       gather_words<KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, W, acc, count);
     } else {
-      gather<DIM, KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, W, acc, count);
+      gather_words_g<DIM, KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, W, acc, count);
     }
     acc.finish();
     if constexpr (KIND == kRhsOnly) {
@@ -724,13 +858,16 @@ __global__ __launch_bounds__(NT) void k_stage_lds(StageArgs A) {
 
 // Chunks with positions from global memory: 3D, and 2D overflow chunks.
 template <int DIM, int KERNEL, int FLAVOUR, int KIND, bool FINAL, int NT>
-__global__ __launch_bounds__(NT) void k_stage_global(StageArgs A, const uint32_t* list,
-                                                     const uint32_t* count) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_stage_global(
+    StageArgs A, const uint32_t* list, const uint32_t* count) {
+  __shared__ uint32_t s_wm[NT / 64][kWordsG * 64];
+  __shared__ uint32_t s_wj[NT / 64][kWordsG * 64];
   const Band band = xcd_band(*count);
   WaveReduce red;
   for (uint32_t q = band.lo; q < band.hi; q += band.step) {
     const ChunkInfo c = chunk_info<DIM>(A, list ? list[q] : q);
-    const GlobalPos<DIM> ps{c.T, A.in, A.cell_off, c.cx, c.cy};
+    const GlobalPos<DIM> ps{c.T,  A.in, A.cell_off, c.cx, c.cy, s_wm[threadIdx.x >> 6],
+                            s_wj[threadIdx.x >> 6]};
     chunk_targets<DIM, KERNEL, FLAVOUR, KIND, FINAL>(A, c, ps, red);
   }
   red.flush(A);
