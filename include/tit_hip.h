@@ -85,14 +85,43 @@ typedef struct tit_hip_host_view {
   const double* dv_dt;  /* (n_fluid + n_fixed) * dim, may be NULL (zero):
                            the previous momentum RHS, read by the first
                            time-step selection only */
-  const double* normal; /* n_fixed * dim wall normals of the fixed particles */
+  const double* normal; /* n_fixed * dim wall normals of the fixed particles
+                           (ignored, may be NULL, with tit_hip_set_walls) */
   size_t n_fluid;
   size_t n_fixed;
 } tit_hip_host_view;
 
+/* Wall boundary of the semi-analytic formulation, 2D: the tessellated wall
+ * surface and a closed polygon for containment, as FluidEquations receives
+ * them (fluid_equations.hpp:58-69; built in titwcsph/wcsph.cpp:55-86).
+ * Face k joins vertices faces[2k], faces[2k+1]; its normal
+ * normalize(cross(b - a)), cross(v) = (v1, -v0), points into the fluid.
+ * Fixed particle k of the uploaded tank must sit at vertex k
+ * (wcsph.cpp:110-113). */
+typedef struct tit_hip_walls {
+  const double* verts;    /* n_verts * 2 */
+  uint32_t n_verts;
+  uint32_t n_faces;
+  const uint32_t* faces;  /* n_faces * 2 */
+  const double* cverts;   /* n_cverts * 2, containment polygon */
+  uint32_t n_cverts;
+  uint32_t n_cfaces;
+  const uint32_t* cfaces; /* n_cfaces * 2 */
+} tit_hip_walls;
+
 /* Creates a context on HIP device `device`. */
 tit_status tit_hip_create(int device, const tit_hip_params* params,
                           tit_hip_ctx** out);
+
+/* Switches the context to the semi-analytic wall boundary (2D, REF terms,
+ * one tank, no post_integrate): before tit_hip_upload.  Every prepare then
+ * runs compute_gamma (fluid_equations.hpp:168-193), setup_boundary takes
+ * normalize(grad gamma) as the wall normal, the fixed particles carry the
+ * masses m gamma set by FluidEquations::initialize (:75-89), and the
+ * continuity and momentum equations gain the face terms and the 1/gamma
+ * factors (:232-305).  Replaces: constructing FluidEquations with a domain
+ * and containment function (fluid_equations.hpp:58-69). */
+tit_status tit_hip_set_walls(tit_hip_ctx* ctx, const tit_hip_walls* walls);
 
 /* Copies one tank's particles to the device (replaces any previous state).
  * Replaces: the ParticleArray field vectors that SSPRKIntegrator::step

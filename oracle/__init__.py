@@ -19,14 +19,22 @@ KERNEL_WENDLAND_C2, KERNEL_WENDLAND_C4 = 0, 1
 FLAVOUR_REF, FLAVOUR_CFG = 0, 1
 
 
+_DP = C.POINTER(C.c_double)
+_U32P = C.POINTER(C.c_uint32)
+
+
+class OrcWalls(C.Structure):
+    _fields_ = [("n_verts", C.c_size_t), ("verts", _DP), ("n_faces", C.c_size_t),
+                ("faces", _U32P), ("n_cverts", C.c_size_t), ("cverts", _DP),
+                ("n_cfaces", C.c_size_t), ("cfaces", _U32P)]
+
+
 class OrcParams(C.Structure):
     _fields_ = [("dim", C.c_int), ("kernel", C.c_int), ("flavour", C.c_int),
                 ("post_integrate", C.c_int)] + [
         (n, C.c_double) for n in ("h", "m", "rho0", "cs0", "xi", "g", "mu",
-                                  "alpha", "delta", "cfl", "c_force", "c_visc")]
-
-
-_DP = C.POINTER(C.c_double)
+                                  "alpha", "delta", "cfl", "c_force", "c_visc")] + [
+        ("walls", C.POINTER(OrcWalls))]
 
 
 class OrcShiftOut(C.Structure):
@@ -36,7 +44,8 @@ class OrcShiftOut(C.Structure):
 class OrcState(C.Structure):
     _fields_ = [("n_fluid", C.c_size_t), ("n_fixed", C.c_size_t),
                 ("r", _DP), ("v", _DP), ("rho", _DP), ("drho_dt", _DP),
-                ("dv_dt", _DP), ("p", _DP), ("cs", _DP), ("normal", _DP)]
+                ("dv_dt", _DP), ("p", _DP), ("cs", _DP), ("normal", _DP),
+                ("gamma", _DP), ("grad_gamma", _DP), ("mass", _DP)]
 
 
 _lib = None
@@ -83,6 +92,15 @@ def load():
         "orc_ssprk_generic": (C.c_int, [C.c_int, C.c_double, C.c_size_t, _DP,
                                         C.c_void_p, C.c_void_p]),
         "orc_set_mutation": (None, [C.c_int]),
+        "orc_kernel_unit_flux": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double]),
+        "orc_kernel_unit_antigrad_flux": (C.c_double, [C.c_int, C.c_double, C.c_double,
+                                                       C.c_double]),
+        "orc_face_flux": (None, [C.c_int, _DP, _DP, _DP, C.c_double, _DP]),
+        "orc_face_antigrad_flux": (C.c_double, [C.c_int, _DP, _DP, _DP, C.c_double]),
+        "orc_face_intersects": (C.c_int, [_DP, _DP, _DP, C.c_double]),
+        "orc_winding": (C.c_double, [C.POINTER(OrcWalls), _DP]),
+        "orc_compute_gamma": (C.c_int, [P, S]),
+        "orc_initialize": (C.c_int, [P, S]),
         "orc_eos_pressure_array": (None, [P, C.c_size_t, _DP, _DP]),
     }
     for name, (res, args) in sig.items():
@@ -98,13 +116,32 @@ def _dp(a):
     return a.ctypes.data_as(_DP)
 
 
-def params(eq) -> OrcParams:
-    """OrcParams from a titsolver_amd.FluidEquations (same field names)."""
+class Walls:
+    """Wall surface and containment polygon (2D), owned by numpy arrays; any
+    object with verts, faces, cverts, cfaces (titsolver_amd.Walls) converts."""
+
+    def __init__(self, walls):
+        self.verts = np.ascontiguousarray(walls.verts, dtype=np.float64).reshape(-1, 2).copy()
+        self.faces = np.ascontiguousarray(walls.faces, dtype=np.uint32).reshape(-1, 2).copy()
+        self.cverts = np.ascontiguousarray(walls.cverts, dtype=np.float64).reshape(-1, 2).copy()
+        self.cfaces = np.ascontiguousarray(walls.cfaces, dtype=np.uint32).reshape(-1, 2).copy()
+        u32 = lambda a: a.ctypes.data_as(_U32P)  # noqa: E731
+        self.c = OrcWalls(len(self.verts), _dp(self.verts), len(self.faces), u32(self.faces),
+                          len(self.cverts), _dp(self.cverts), len(self.cfaces), u32(self.cfaces))
+
+
+def params(eq, walls=None) -> OrcParams:
+    """OrcParams from a titsolver_amd.FluidEquations (same field names); with
+    walls, the semi-analytic wall form."""
     p = OrcParams()
     p.dim, p.kernel, p.flavour = eq.dim, eq.kernel, eq.flavour
     for n in ("h", "m", "rho0", "cs0", "xi", "g", "mu", "alpha", "delta", "cfl",
               "c_force", "c_visc"):
         setattr(p, n, float(getattr(eq, n)))
+    if walls is not None:
+        w = walls if isinstance(walls, Walls) else Walls(walls)
+        p._walls = w  # keeps the arrays alive
+        p.walls = C.pointer(w.c)
     return p
 
 
@@ -169,9 +206,13 @@ class State:
         self.normal = np.ascontiguousarray(normal, dtype=np.float64).reshape(-1, self.dim).copy()
         if self.normal.shape[0] == 0:
             self.normal = np.zeros((1, self.dim))
+        self.gamma = np.ones(self.n)
+        self.grad_gamma = np.zeros_like(r)
+        self.mass = np.zeros(self.n)
         self.c = OrcState(self.n_fluid, self.n - self.n_fluid, _dp(self.r), _dp(self.v),
                           _dp(self.rho), _dp(self.drho_dt), _dp(self.dv_dt),
-                          _dp(self.p), _dp(self.cs), _dp(self.normal))
+                          _dp(self.p), _dp(self.cs), _dp(self.normal), _dp(self.gamma),
+                          _dp(self.grad_gamma), _dp(self.mass))
 
     @classmethod
     def from_particles(cls, pa):
@@ -186,6 +227,35 @@ def _csr(off, idx):
 def adjacency(st: State, prm: OrcParams):
     """Sorted neighbour lists as ParticleMesh builds them (hint h, radius 2h)."""
     return grid_search(st.r, 2.0 * prm.h, prm.h)
+
+
+def initialize(prm, st):
+    """FluidEquations::initialize: masses, and with walls gamma and the fixed
+    particles' mass fractions."""
+    if load().orc_initialize(C.byref(prm), C.byref(st.c)) != 0:
+        raise RuntimeError("oracle initialize failed")
+
+
+def compute_gamma(prm, st):
+    if load().orc_compute_gamma(C.byref(prm), C.byref(st.c)) != 0:
+        raise RuntimeError("oracle compute_gamma failed")
+
+
+def unit_flux(kernel, eta, z0, z1, antigrad=False):
+    fn = load().orc_kernel_unit_antigrad_flux if antigrad else load().orc_kernel_unit_flux
+    return fn(int(kernel), float(eta), float(z0), float(z1))
+
+
+def face_flux(kernel, a, b, x, h):
+    a, b, x = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, x))
+    out = np.zeros(2)
+    load().orc_face_flux(int(kernel), _dp(a), _dp(b), _dp(x), float(h), _dp(out))
+    return out
+
+
+def face_antigrad_flux(kernel, a, b, x, h):
+    a, b, x = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, x))
+    return load().orc_face_antigrad_flux(int(kernel), _dp(a), _dp(b), _dp(x), float(h))
 
 
 def setup_boundary(prm, st, off, idx):

@@ -413,9 +413,232 @@ long long orc_naive_search(int dim, size_t n, const double* pts,
 }
 
 /* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
-/* Fluid equations (sph/fluid_equations.hpp), wall-free form: gamma = 1 and
- * grad gamma = 0 everywhere, so the face loops vanish; fixed particles carry
- * an explicit wall normal in place of normalize(grad_gamma). */
+/* Semi-analytic wall, 2D (sph/kernel.hpp:193-314, geom/segment.hpp:66-112,
+ * fluid_equations.hpp:168-193).
+ *
+ * The unit kernels as monomials, f(q) = sum_k a_k q^k on [0, 2], and their
+ * tail moments M(q) = int_q^2 s f(s) ds = M0 + sum_j b_j q^(j+2) (no linear
+ * term: M'(q) = -q f(q)).  The kernel's face integrals in unit coordinates,
+ * for a face at distance eta from the point, spanning z in [z0, z1] along it
+ * (rho = sqrt(z^2 + eta^2)):
+ *   unit_flux          = int f(rho) dz              = sum_k a_k J_k
+ *   unit_antigrad_flux = eta int M(rho)/rho^2 dz    = M0 atan2(z, eta) + eta sum_j b_j J_j
+ * with J_n(z) = int rho^n dz: J_0 = z, J_-1 = asinh(z/eta) = L,
+ * J_n = (z rho^n + n eta^2 J_(n-2)) / (n + 1), each evaluated between the
+ * clipped ends as unit_segment_integral does (kernel.hpp:282-312).  The
+ * tests check both against numerical quadrature of the defining integrals
+ * and the resulting gamma against a direct area integral of W. */
+typedef struct unit_poly {
+  int na;
+  double a[9];
+  double M0;
+  int nb;
+  double b[9];
+} unit_poly;
+static const unit_poly kPolyC2 = {
+    6, {1.0, 0.0, -2.5, 2.5, -15.0 / 16.0, 1.0 / 8.0}, 2.0 / 7.0,
+    6, {-0.5, 0.0, 5.0 / 8.0, -0.5, 5.0 / 32.0, -1.0 / 56.0}};
+static const unit_poly kPolyC4 = {
+    9, {1.0, 0.0, -7.0 / 3.0, 0.0, 35.0 / 8.0, -14.0 / 3.0, 35.0 / 16.0, -0.5, 35.0 / 768.0},
+    2.0 / 9.0,
+    9, {-0.5, 0.0, 7.0 / 12.0, 0.0, -35.0 / 48.0, 2.0 / 3.0, -35.0 / 128.0, 1.0 / 18.0,
+        -7.0 / 1536.0}};
+static const unit_poly* unit_poly_of(int kernel) {
+  return kernel == ORC_KERNEL_WENDLAND_C4 ? &kPolyC4 : &kPolyC2;
+}
+
+/* Primitive of the flux (which = 0) or of the antigradient flux (which = 1)
+ * at z. */
+static double segment_primitive(const unit_poly* up, int which, double eta, double z) {
+  const double rho = sqrt(pow2(z) + pow2(eta));
+  const double L = is_tiny(eta) ? 0.0 : copysign(log1p((fabs(z) + pow2(z) / (rho + eta)) / eta), z);
+  const double e2 = pow2(eta);
+  double J[9];
+  double rp = rho; /* rho^n */
+  J[0] = z;
+  J[1] = (z * rho + e2 * L) / 2.0;
+  for (int k = 2; k < 9; ++k) {
+    rp = rp * rho;
+    J[k] = (z * rp + (double)k * e2 * J[k - 2]) / (double)(k + 1);
+  }
+  double r = 0.0;
+  if (which == 0) {
+    for (int k = 0; k < up->na; ++k) r += up->a[k] * J[k];
+    return r;
+  }
+  for (int k = 0; k < up->nb; ++k) r += up->b[k] * J[k];
+  return up->M0 * atan2(z, eta) + eta * r;
+}
+
+static double unit_segment_integral(int kernel, int which, double eta, double z_min,
+                                    double z_max) {
+  const double cutoff = 2.0;
+  if (eta >= cutoff) return 0.0;
+  const double z_clip = sqrt(pow2(cutoff) - pow2(eta));
+  const double z_lo = z_min > -z_clip ? z_min : -z_clip;
+  const double z_hi = z_max < z_clip ? z_max : z_clip;
+  if (z_lo >= z_hi) return 0.0;
+  const unit_poly* up = unit_poly_of(kernel);
+  return segment_primitive(up, which, eta, z_hi) - segment_primitive(up, which, eta, z_lo);
+}
+
+double orc_kernel_unit_flux(int kernel, double eta, double z0, double z1) {
+  return unit_segment_integral(kernel, 0, eta, z0, z1);
+}
+double orc_kernel_unit_antigrad_flux(int kernel, double eta, double z0, double z1) {
+  return unit_segment_integral(kernel, 1, eta, z0, z1);
+}
+
+/* Segment::normal = normalize(cross(b - a)), cross(v) = (v1, -v0), and the
+ * unit direction of b - a (geom/segment.hpp:54-75, core/_vec/vec.hpp:692). */
+static void face_frame(const double* a, const double* b, double* n, double* e) {
+  const double ba[2] = {b[0] - a[0], b[1] - a[1]};
+  const double wn[2] = {ba[1], -ba[0]};
+  normalize(2, wn, n);
+  normalize(2, ba, e);
+}
+
+/* Kernel::flux (kernel.hpp:193-205): n w/h unit_flux(|d|, z_a, z_b),
+ * d = (x - a).n / h, z = (vertex - x).e / h (Segment::project). */
+void orc_face_flux(int kernel, const double* a, const double* b, const double* x,
+                   double h, double* out) {
+  double n[2], e[2];
+  face_frame(a, b, n, e);
+  const double hi = 1.0 / h;
+  const double w = orc_kernel_weight(kernel, 2) * hi;
+  const double xa[2] = {x[0] - a[0], x[1] - a[1]};
+  const double ax[2] = {a[0] - x[0], a[1] - x[1]};
+  const double bx[2] = {b[0] - x[0], b[1] - x[1]};
+  const double d = dot(2, xa, n) * hi;
+  double f = orc_kernel_unit_flux(kernel, fabs(d), dot(2, ax, e) * hi, dot(2, bx, e) * hi);
+  if (g_mutation == 9) f = -f; /* planted: face normal reversed */
+  for (int k = 0; k < 2; ++k) out[k] = n[k] * w * f;
+}
+
+/* Kernel::antigrad_flux (kernel.hpp:207-220): copysign(w, d) unit_antigrad_flux. */
+double orc_face_antigrad_flux(int kernel, const double* a, const double* b, const double* x,
+                              double h) {
+  double n[2], e[2];
+  face_frame(a, b, n, e);
+  const double hi = 1.0 / h;
+  const double w = orc_kernel_weight(kernel, 2);
+  const double xa[2] = {x[0] - a[0], x[1] - a[1]};
+  const double ax[2] = {a[0] - x[0], a[1] - x[1]};
+  const double bx[2] = {b[0] - x[0], b[1] - x[1]};
+  const double d = dot(2, xa, n) * hi;
+  return copysign(w, d) *
+         orc_kernel_unit_antigrad_flux(kernel, fabs(d), dot(2, ax, e) * hi, dot(2, bx, e) * hi);
+}
+
+/* Segment::intersects(BSphere) (geom/segment.hpp:98-112, bbox.hpp:62-65,
+ * bsphere.hpp:47-54): the boxes overlap (inclusive) and the point of the
+ * segment closest to the centre lies within the radius (inclusive).  clamp
+ * returns vertex a when |b - a|^2 is tiny, as the reference does. */
+int orc_face_intersects(const double* a, const double* b, const double* c, double R) {
+  for (int k = 0; k < 2; ++k) {
+    const double lo = a[k] < b[k] ? a[k] : b[k];
+    const double hi = a[k] < b[k] ? b[k] : a[k];
+    if (!(c[k] - R <= hi && lo <= c[k] + R)) return 0;
+  }
+  const double ba[2] = {b[0] - a[0], b[1] - a[1]};
+  const double len2 = norm2(2, ba);
+  double p[2] = {a[0], a[1]};
+  if (!is_tiny(len2)) {
+    const double ca[2] = {c[0] - a[0], c[1] - a[1]};
+    const double t = dot(2, ca, ba) / len2;
+    if (t > 1.0) {
+      p[0] = b[0];
+      p[1] = b[1];
+    } else if (!(t < 0.0)) {
+      p[0] = a[0] + t * ba[0];
+      p[1] = a[1] + t * ba[1];
+    }
+  }
+  const double pc[2] = {p[0] - c[0], p[1] - c[1]};
+  return norm2(2, pc) <= pow2(R);
+}
+
+/* Generalised winding number of the containment polygon around x, the exact
+ * sum of Segment::winding_number (geom/segment.hpp:77-82); the reference's
+ * fast estimator falls back to this sum near the threshold
+ * (winding/fast_winding.hpp:80-92).  contains: winding > 1/2. */
+double orc_winding(const orc_walls* w, const double* x) {
+  const double two_pi = 6.283185307179586476925286766559;
+  double r = 0.0;
+  for (size_t f = 0; f < w->n_cfaces; ++f) {
+    const double* a = w->cverts + 2 * w->cfaces[2 * f];
+    const double* b = w->cverts + 2 * w->cfaces[2 * f + 1];
+    const double ap[2] = {a[0] - x[0], a[1] - x[1]};
+    const double bp[2] = {b[0] - x[0], b[1] - x[1]};
+    r += atan2(ap[0] * bp[1] - ap[1] * bp[0], dot(2, ap, bp)) / two_pi;
+  }
+  return r;
+}
+
+static const double* wall_vert(const orc_walls* w, size_t f, int k) {
+  return w->verts + 2 * w->faces[2 * f + k];
+}
+
+/* compute_gamma (fluid_equations.hpp:168-193) for every particle, over the
+ * faces within the kernel radius in ascending order (ParticleMesh::search_,
+ * particle_mesh.hpp:150-162). */
+int orc_compute_gamma(const orc_params* prm, orc_state* s) {
+  const orc_walls* w = prm->walls;
+  if (!w || prm->dim != 2 || !s->gamma || !s->grad_gamma) return -1;
+  const size_t n = s->n_fluid + s->n_fixed;
+  const double h = prm->h, R = orc_kernel_radius(prm->kernel, h);
+  for (size_t i = 0; i < n; ++i) {
+    const double* x = s->r + 2 * i;
+    double gg[2] = {0.0, 0.0};
+    for (size_t f = 0; f < w->n_faces; ++f) {
+      if (!orc_face_intersects(wall_vert(w, f, 0), wall_vert(w, f, 1), x, R)) continue;
+      double fl[2];
+      orc_face_flux(prm->kernel, wall_vert(w, f, 0), wall_vert(w, f, 1), x, h, fl);
+      gg[0] += fl[0];
+      gg[1] += fl[1];
+    }
+    double gamma = orc_winding(w, x) > 0.5 ? 1.0 : 0.0;
+    const double ng = norm(2, gg);
+    if (!is_tiny(ng)) {
+      const double na[2] = {gg[0] / ng, gg[1] / ng};
+      const double sgn = 2.0 * gamma - 1.0;
+      const double xs[2] = {x[0] + sgn * na[0] * pow2(h), x[1] + sgn * na[1] * pow2(h)};
+      for (size_t f = 0; f < w->n_faces; ++f) {
+        if (!orc_face_intersects(wall_vert(w, f, 0), wall_vert(w, f, 1), x, R)) continue;
+        gamma -= orc_face_antigrad_flux(prm->kernel, wall_vert(w, f, 0), wall_vert(w, f, 1), xs, h);
+      }
+    }
+    s->gamma[i] = gamma;
+    s->grad_gamma[2 * i] = gg[0];
+    s->grad_gamma[2 * i + 1] = gg[1];
+  }
+  return 0;
+}
+
+/* FluidEquations::initialize (fluid_equations.hpp:75-89): gamma, then the
+ * fixed particles' masses become m gamma. */
+int orc_initialize(const orc_params* prm, orc_state* s) {
+  if (!s->mass) return -1;
+  const size_t n = s->n_fluid + s->n_fixed;
+  for (size_t i = 0; i < n; ++i) s->mass[i] = prm->m;
+  if (!prm->walls) return 0;
+  if (prm->walls->n_verts != s->n_fixed) return -1;
+  if (orc_compute_gamma(prm, s)) return -1;
+  for (size_t e = s->n_fluid; e < n; ++e) s->mass[e] = prm->m * s->gamma[e];
+  return 0;
+}
+
+/* Mass of particle i: per particle once initialised with walls, else m. */
+static double mass_of(const orc_params* prm, const orc_state* s, size_t i) {
+  return (prm->walls && s->mass) ? s->mass[i] : prm->m;
+}
+
+/* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
+/* Fluid equations (sph/fluid_equations.hpp).  Without walls (prm->walls
+ * null): gamma = 1 and grad gamma = 0 everywhere, so the face loops vanish,
+ * and fixed particles carry an explicit wall normal in place of
+ * normalize(grad_gamma).  With walls (2D, REF terms): the face loops over the
+ * wall segments, pair terms divided by gamma_a, per-particle masses. */
 
 /* setup_boundary, fluid_equations.hpp:122-164 */
 int orc_setup_boundary(const orc_params* prm, orc_state* s,
@@ -426,11 +649,15 @@ int orc_setup_boundary(const orc_params* prm, orc_state* s,
     for (int k = 0; k < dim; ++k) s->v[e * dim + k] = 0.0; /* no-slip */
     double S_e = 0.0, H_e = 0.0;
     double n_e[3] = {0.0, 0.0, 0.0};
-    normalize(dim, s->normal + (e - nf) * dim, n_e);
+    if (prm->walls) {
+      normalize(dim, s->grad_gamma + e * dim, n_e); /* normalize(grad_gamma[e]) */
+    } else {
+      normalize(dim, s->normal + (e - nf) * dim, n_e);
+    }
     for (uint64_t j = off[e]; j < off[e + 1]; ++j) {
       const size_t b = idx[j];
       if (b >= nf) continue; /* fluid neighbours only */
-      const double V_b = prm->m / s->rho[b];
+      const double V_b = mass_of(prm, s, b) / s->rho[b];
       double r_be[3] = {0.0, 0.0, 0.0};
       for (int k = 0; k < dim; ++k) r_be[k] = s->r[b * dim + k] - s->r[e * dim + k];
       const double W_be = orc_kernel_value(prm->kernel, dim, r_be, prm->h);
@@ -446,6 +673,63 @@ int orc_setup_boundary(const orc_params* prm, orc_state* s,
 /* compute_continuity, fluid_equations.hpp:232-260.  The pair loop runs over
  * unique pairs (b < a) of the sorted adjacency with symmetric scatter, as
  * block_for_each over ParticleMesh::block_pairs does, in ascending order. */
+/* gamma_a (1 without walls). */
+static double gamma_of(const orc_params* prm, const orc_state* s, size_t a) {
+  return prm->walls ? s->gamma[a] : 1.0;
+}
+
+/* Face terms of compute_continuity (fluid_equations.hpp:243-247): the face's
+ * fixed particles are the ones at its vertices, their fields averaged. */
+static void wall_continuity(const orc_params* prm, orc_state* s, size_t a) {
+  const orc_walls* w = prm->walls;
+  const size_t nf = s->n_fluid;
+  const double* x = s->r + 2 * a;
+  const double R = orc_kernel_radius(prm->kernel, prm->h);
+  for (size_t f = 0; f < w->n_faces; ++f) {
+    if (!orc_face_intersects(wall_vert(w, f, 0), wall_vert(w, f, 1), x, R)) continue;
+    double gg[2];
+    orc_face_flux(prm->kernel, wall_vert(w, f, 0), wall_vert(w, f, 1), x, prm->h, gg);
+    const size_t s0 = nf + w->faces[2 * f], s1 = nf + w->faces[2 * f + 1];
+    const double rho_s = (s->rho[s0] + s->rho[s1]) / 2.0;
+    double v_as[2];
+    for (int k = 0; k < 2; ++k) v_as[k] = s->v[2 * a + k] - (s->v[2 * s0 + k] + s->v[2 * s1 + k]) / 2.0;
+    s->drho_dt[a] -= rho_s * dot(2, v_as, gg) / s->gamma[a];
+  }
+}
+
+/* Face terms of compute_momentum (fluid_equations.hpp:277-291). */
+static void wall_momentum(const orc_params* prm, orc_state* s, size_t a) {
+  const orc_walls* w = prm->walls;
+  const size_t nf = s->n_fluid;
+  const double* x = s->r + 2 * a;
+  const double h = prm->h, R = orc_kernel_radius(prm->kernel, h);
+  for (size_t f = 0; f < w->n_faces; ++f) {
+    if (!orc_face_intersects(wall_vert(w, f, 0), wall_vert(w, f, 1), x, R)) continue;
+    double gg[2];
+    orc_face_flux(prm->kernel, wall_vert(w, f, 0), wall_vert(w, f, 1), x, h, gg);
+    const size_t s0 = nf + w->faces[2 * f], s1 = nf + w->faces[2 * f + 1];
+    const double rho_s = (s->rho[s0] + s->rho[s1]) / 2.0;
+    const double p_s = (s->p[s0] + s->p[s1]) / 2.0;
+    const double P_as = rho_s * (s->p[a] / pow2(s->rho[a]) + p_s / pow2(rho_s));
+    double v_as[2], r_as[2], n_s[2], t[2], t_as[2];
+    for (int k = 0; k < 2; ++k) {
+      v_as[k] = s->v[2 * a + k] - (s->v[2 * s0 + k] + s->v[2 * s1 + k]) / 2.0;
+      r_as[k] = s->r[2 * a + k] - (s->r[2 * s0 + k] + s->r[2 * s1 + k]) / 2.0;
+    }
+    normalize(2, gg, n_s);
+    const double vn = dot(2, v_as, n_s);
+    for (int k = 0; k < 2; ++k) t[k] = v_as[k] - vn * n_s[k];
+    normalize(2, t, t_as);
+    const double rn = dot(2, r_as, n_s);
+    const double dr_as = h / 2.0 > rn ? h / 2.0 : rn;
+    const double pc = 2.0 * prm->mu / (s->rho[a] * dr_as) * dot(2, v_as, t_as);
+    const double ng = norm(2, gg);
+    for (int k = 0; k < 2; ++k) {
+      s->dv_dt[2 * a + k] += (P_as * gg[k] - pc * t_as[k] * ng) / s->gamma[a];
+    }
+  }
+}
+
 int orc_compute_continuity(const orc_params* prm, orc_state* s,
                            const uint64_t* off, const uint32_t* idx) {
   const int dim = prm->dim;
@@ -453,6 +737,9 @@ int orc_compute_continuity(const orc_params* prm, orc_state* s,
   const double m = prm->m;
   for (size_t a = 0; a < n; ++a) s->cs[a] = orc_eos_sound_speed(prm, s->rho[a]);
   for (size_t a = 0; a < n; ++a) s->drho_dt[a] = 0.0;
+  if (prm->walls) {
+    for (size_t a = 0; a < s->n_fluid; ++a) wall_continuity(prm, s, a);
+  }
   for (size_t a = 0; a < n; ++a) {
     for (uint64_t j = off[a]; j < off[a + 1]; ++j) {
       const size_t b = idx[j];
@@ -474,8 +761,8 @@ int orc_compute_continuity(const orc_params* prm, orc_state* s,
           ta[k] = v_ab[k] + Psi[k] / s->rho[b];
           tb[k] = -v_ab[k] + Psi[k] / s->rho[a];
         }
-        s->drho_dt[a] += m * dot(dim, ta, grad);
-        s->drho_dt[b] -= m * dot(dim, tb, grad);
+        s->drho_dt[a] += mass_of(prm, s, b) / gamma_of(prm, s, a) * dot(dim, ta, grad);
+        s->drho_dt[b] -= mass_of(prm, s, a) / gamma_of(prm, s, b) * dot(dim, tb, grad);
       } else {
         /* delta-SPH (Molteni & Colagrossi 2009):
          *   psi_ab = 2 (rho_a - rho_b) r_ab / |r_ab|^2,
@@ -509,6 +796,9 @@ int orc_compute_momentum(const orc_params* prm, orc_state* s,
     for (int k = 0; k < dim; ++k) s->dv_dt[a * dim + k] = 0.0;
     if (a < nf) s->dv_dt[a * dim + 1] = -prm->g; /* unit<1>(r, -g) */
   }
+  if (prm->walls && g_mutation != 10) { /* 10 planted: face terms dropped */
+    for (size_t a = 0; a < nf; ++a) wall_momentum(prm, s, a);
+  }
   for (size_t a = 0; a < n; ++a) {
     for (uint64_t j = off[a]; j < off[a + 1]; ++j) {
       const size_t b = idx[j];
@@ -539,8 +829,8 @@ int orc_compute_momentum(const orc_params* prm, orc_state* s,
                   (rho_m * (norm2(dim, r_ab) + eta2));
         }
       }
-      double fa = m * (Pi_ab - P_ab);
-      const double fb = m * (Pi_ab - P_ab);
+      double fa = mass_of(prm, s, b) / gamma_of(prm, s, a) * (Pi_ab - P_ab);
+      const double fb = mass_of(prm, s, a) / gamma_of(prm, s, b) * (Pi_ab - P_ab);
       if (g_mutation == 1) fa = m * (Pi_ab + P_ab); /* planted: sign slip */
       for (int k = 0; k < dim; ++k) {
         s->dv_dt[a * dim + k] += fa * grad[k];
@@ -643,15 +933,17 @@ static int ws_reserve(step_ws* w, size_t n, int dim) {
   return 0;
 }
 
-/* FluidEquations::prepare without gamma: index + setup_boundary
- * (fluid_equations.hpp:99-105).  The search uses GridSearch{h} with radius
- * 2h as titwcsph does (wcsph.cpp:145-154, fluid_equations.hpp:112-115). */
+/* FluidEquations::prepare: index + compute_gamma (with walls) +
+ * setup_boundary (fluid_equations.hpp:99-105).  The search uses
+ * GridSearch{h} with radius 2h as titwcsph does (wcsph.cpp:145-154,
+ * fluid_equations.hpp:112-115). */
 static int prepare(const orc_params* prm, orc_state* s, step_ws* w) {
   const size_t n = s->n_fluid + s->n_fixed;
   if (build_adjacency(prm->dim, n, s->r, orc_kernel_radius(prm->kernel, prm->h),
                       prm->h, w->off, &w->idx)) {
     return -1;
   }
+  if (prm->walls && orc_compute_gamma(prm, s)) return -1;
   return orc_setup_boundary(prm, s, w->off, w->idx.data);
 }
 
@@ -920,6 +1212,10 @@ static int substep(const orc_params* prm, orc_state* s, step_ws* w,
 int orc_ssprk_step(const orc_params* prm, orc_state* s, int order,
                    double* dt_out) {
   if (order != 2 && order != 3) return -1;
+  /* the post-integration passes are restated in the wall-free form only */
+  if (prm->walls && (prm->post_integrate || prm->dim != 2 || prm->flavour != ORC_FLAVOUR_REF)) {
+    return -1;
+  }
   const int dim = prm->dim;
   const size_t n = s->n_fluid + s->n_fixed, nf = s->n_fluid;
   step_ws* w = &g_ws;

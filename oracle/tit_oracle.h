@@ -36,6 +36,22 @@ enum { ORC_KERNEL_WENDLAND_C2 = 0, ORC_KERNEL_WENDLAND_C4 = 1 };
  * viscosity alpha. */
 enum { ORC_FLAVOUR_REF = 0, ORC_FLAVOUR_CFG = 1 };
 
+/* Wall boundary of the semi-analytic formulation, 2D (fluid_equations.hpp:
+ * 58-69, wcsph.cpp:55-86, 110-113): the tessellated wall polyline, whose
+ * vertex k carries fixed particle n_fluid + k, each face's normal
+ * normalize(cross(b - a)) pointing into the fluid; and a closed polygon whose
+ * winding number decides containment. */
+typedef struct orc_walls {
+  size_t n_verts;
+  const double* verts;    /* n_verts * 2 */
+  size_t n_faces;
+  const uint32_t* faces;  /* n_faces * 2 vertex indices (a, b) */
+  size_t n_cverts;
+  const double* cverts;   /* containment polygon */
+  size_t n_cfaces;
+  const uint32_t* cfaces;
+} orc_walls;
+
 typedef struct orc_params {
   int dim;      /* 2 or 3 */
   int kernel;   /* ORC_KERNEL_* */
@@ -53,6 +69,7 @@ typedef struct orc_params {
   double cfl;     /* acoustic CFL number */
   double c_force; /* force time-step coefficient */
   double c_visc;  /* viscous time-step coefficient */
+  const orc_walls* walls; /* null: wall-free form (fixed particles carry normals) */
 } orc_params;
 
 typedef struct orc_state {
@@ -66,6 +83,9 @@ typedef struct orc_state {
   double* p;        /* n */
   double* cs;       /* n */
   const double* normal; /* n_fixed * dim, wall normals of fixed particles */
+  double* gamma;        /* n (walls only) */
+  double* grad_gamma;   /* n * dim (walls only) */
+  double* mass;         /* n: per-particle masses (walls only, orc_initialize) */
 } orc_state;
 
 /* ---- smoothing kernel ---------------------------------------------------- */
@@ -109,6 +129,25 @@ int orc_compute_momentum(const orc_params* prm, orc_state* s,
                          const uint64_t* off, const uint32_t* idx);
 double orc_compute_time_step(const orc_params* prm, const orc_state* s);
 
+/* ---- semi-analytic wall (2D) ------------------------------------------- */
+/* KernelGen::unit_flux / unit_antigrad_flux over a segment in unit
+ * coordinates (sph/kernel.hpp:252-312). */
+double orc_kernel_unit_flux(int kernel, double eta, double z0, double z1);
+double orc_kernel_unit_antigrad_flux(int kernel, double eta, double z0, double z1);
+/* Kernel::flux and antigrad_flux of segment (a, b) at x (kernel.hpp:193-220). */
+void orc_face_flux(int kernel, const double* a, const double* b, const double* x,
+                   double h, double* out);
+double orc_face_antigrad_flux(int kernel, const double* a, const double* b,
+                              const double* x, double h);
+/* Segment::intersects(BSphere{c, R}) (geom/segment.hpp:98-112). */
+int orc_face_intersects(const double* a, const double* b, const double* c, double R);
+/* Exact generalised winding number of the containment polygon at x. */
+double orc_winding(const orc_walls* w, const double* x);
+/* compute_gamma (fluid_equations.hpp:168-193) for every particle. */
+int orc_compute_gamma(const orc_params* prm, orc_state* s);
+/* FluidEquations::initialize (fluid_equations.hpp:75-89): s->mass = m, and
+ * with walls gamma and the fixed particles' masses m gamma. */
+int orc_initialize(const orc_params* prm, orc_state* s);
 /* ---- post-integration (particle shifting, free-surface correction) ------- */
 /* Optional outputs of orc_post_integrate (any pointer may be null). */
 typedef struct orc_shift_out {
@@ -144,7 +183,9 @@ int orc_ssprk_generic(int order, double dt, size_t n, double* u,
  * 2: anti-diffusive delta-SPH sign, 3: viscosity on receding pairs,
  * 4: swapped SSPRK combination weights, 5: grad v not renormalised,
  * 6: free-surface visibility cone reversed, 7: shifts applied to every
- * fluid particle, 8: free-surface blend exponent with the wrong sign); 0
+ * fluid particle, 8: free-surface blend exponent with the wrong sign,
+ * 9: wall face normals reversed in the kernel flux, 10: wall face terms of
+ * the momentum equation dropped); 0
  * restores the faithful oracle. */
 void orc_set_mutation(int m);
 

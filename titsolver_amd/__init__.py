@@ -2,5 +2,6 @@
 from ._abi import (FLAVOUR_CFG, FLAVOUR_REF, KERNEL_WENDLAND_C2,  # noqa: F401
                    KERNEL_WENDLAND_C4, LIB_PATH, load)
 from .sph import (FluidEquations, HipContext, LearnedStep,  # noqa: F401
-                  ParticleArray, SSPRKIntegrator, TitError, cfg_equations,
-                  dam_break, nn_gemm, ref_equations)
+                  ParticleArray, SSPRKIntegrator, TitError, Walls, box_walls,
+                  cfg_equations, dam_break, dam_break_walls, nn_gemm,
+                  ref_equations, tessellate)

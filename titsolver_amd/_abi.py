@@ -48,6 +48,13 @@ class HostView(C.Structure):
     ]
 
 
+class Walls(C.Structure):
+    _fields_ = [("verts", C.POINTER(C.c_double)), ("n_verts", C.c_uint32),
+                ("n_faces", C.c_uint32), ("faces", C.POINTER(C.c_uint32)),
+                ("cverts", C.POINTER(C.c_double)), ("n_cverts", C.c_uint32),
+                ("n_cfaces", C.c_uint32), ("cfaces", C.POINTER(C.c_uint32))]
+
+
 class Schedule(C.Structure):
     _fields_ = [("chunk_targets", C.c_int), ("lds_capacity", C.c_int),
                 ("occupancy", C.c_double)]
@@ -81,6 +88,7 @@ SIGNATURES = {
     "tit_hip_compute_time_step": (C.c_int, [C.c_void_p, _DP]),
     "tit_hip_compute_rhs": (C.c_int, [C.c_void_p, _DP, _DP]),
     "tit_hip_post_integrate": (C.c_int, [C.c_void_p, _DP]),
+    "tit_hip_set_walls": (C.c_int, [C.c_void_p, C.POINTER(Walls)]),
     "tit_hip_neighbors": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint32), C.c_size_t,
                                     C.POINTER(C.c_size_t)]),

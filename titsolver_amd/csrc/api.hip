@@ -276,8 +276,36 @@ void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
   a.fixed_list = c->fixed_list;
   a.normal = c->normal;
   a.n_fixed = c->n_fixed;
-  ProfScope ps(c->prof, c->stream, "boundary");
-  launch_boundary(c->stream, a);
+  if (c->walls) {
+    a.wall_mu = c->wall_mu;
+    a.wall_rho = c->wall_rho;
+    a.wall_id0 = c->wall_id0;
+  }
+  {
+    ProfScope ps(c->prof, c->stream, "boundary");
+    launch_boundary(c->stream, a);
+  }
+  if (c->walls) {  // compute_gamma + the face terms of this state
+    tit_amd::WallArgs w{};
+    w.P = c->dp;
+    w.n = c->n;
+    w.s = st;
+    w.tag = c->tag;
+    w.faces = c->wall_face_d;
+    w.cell_faces_off = c->wall_cell_off;
+    w.cell_faces = c->wall_cell_faces;
+    for (int k = 0; k < 2; ++k) {
+      w.lo[k] = c->wall_lo[k];
+      w.inv_cw[k] = c->wall_inv_cw[k];
+      w.nc[k] = c->wall_nc[k];
+    }
+    w.cedges = c->wall_cedges;
+    w.n_cedges = c->wall_n_cedges;
+    w.vert_rho = c->wall_rho;
+    w.out = c->wall_out;
+    ProfScope ps(c->prof, c->stream, "wall");
+    tit_amd::launch_wall(c->stream, w);
+  }
 }
 
 void forget(tit_hip_ctx* c, void* q) {
@@ -363,6 +391,7 @@ StageArgs stage_args(tit_hip_ctx* c) {
   a.rhs_count = c->rhs_count;
   for (int k = 0; k < 3; ++k) a.rhs_dv[k] = c->rhs_dv[k];
   for (int k = 0; k < 3; ++k) a.dv_last[k] = c->prm.post_integrate ? c->rhs_dv[k] : nullptr;
+  a.wall = c->wall_out;
   return a;
 }
 
@@ -532,6 +561,40 @@ tit_status tit_hip_create(int device, const tit_hip_params* params,
   return TIT_OK;
 }
 
+tit_status tit_hip_set_walls(tit_hip_ctx* c, const tit_hip_walls* w) {
+  if (!c || !w) return TIT_E_ARG;
+  if (c->uploaded) return fail(c, TIT_E_STATE, "set_walls after upload");
+  if (c->prm.dim != 2) return fail(c, TIT_E_ARG, "walls are 2D only");
+  if (c->prm.flavour != TIT_FLAVOUR_REF) {
+    return fail(c, TIT_E_ARG, "walls need the reference flavour");
+  }
+  if (c->prm.post_integrate) {
+    return fail(c, TIT_E_ARG, "post_integrate is restated in the wall-free form only");
+  }
+  if (!w->verts || w->n_verts < 2 || !w->faces || w->n_faces == 0 || !w->cverts ||
+      w->n_cverts < 3 || !w->cfaces || w->n_cfaces < 3) {
+    return fail(c, TIT_E_ARG, "walls: empty surface or containment polygon");
+  }
+  for (uint32_t k = 0; k < 2 * w->n_faces; ++k) {
+    if (w->faces[k] >= w->n_verts) return fail(c, TIT_E_ARG, "walls: face vertex out of range");
+  }
+  for (uint32_t k = 0; k < 2 * w->n_cfaces; ++k) {
+    if (w->cfaces[k] >= w->n_cverts) {
+      return fail(c, TIT_E_ARG, "walls: containment vertex out of range");
+    }
+  }
+  for (uint32_t k = 0; k < 2 * w->n_verts; ++k) {
+    if (!std::isfinite(w->verts[k])) return fail(c, TIT_E_ARG, "walls: non-finite vertex");
+  }
+  c->wall_verts.assign(w->verts, w->verts + 2 * static_cast<size_t>(w->n_verts));
+  c->wall_faces.assign(w->faces, w->faces + 2 * static_cast<size_t>(w->n_faces));
+  c->wall_cverts.assign(w->cverts, w->cverts + 2 * static_cast<size_t>(w->n_cverts));
+  c->wall_cfaces.assign(w->cfaces, w->cfaces + 2 * static_cast<size_t>(w->n_cfaces));
+  c->walls = true;
+  c->dp.flavour = tit_amd::kFlavourWall;
+  return TIT_OK;
+}
+
 void tit_hip_destroy(tit_hip_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
@@ -568,7 +631,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   for (size_t t = 0; t < n_tanks; ++t) {
     const tit_hip_host_view& v = views[t];
     if (!v.r || !v.rho) return fail(c, TIT_E_ARG, "r and rho are required");
-    if (v.n_fixed > 0 && !v.normal) {
+    if (v.n_fixed > 0 && !v.normal && !c->walls) {
       return fail(c, TIT_E_ARG, "fixed particles need wall normals");
     }
     n_total += v.n_fluid + v.n_fixed;
@@ -624,7 +687,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
     if (v.v) std::memcpy(&hv[o * dim], v.v, nt * dim * sizeof(double));
     std::memcpy(&hrho[o], v.rho, nt * sizeof(double));
     if (v.dv_dt) std::memcpy(&hdv[o * dim], v.dv_dt, nt * dim * sizeof(double));
-    if (v.n_fixed) {
+    if (v.n_fixed && v.normal) {
       std::memcpy(&hn[(o + v.n_fluid) * dim], v.normal,
                   v.n_fixed * dim * sizeof(double));
     }
@@ -644,6 +707,41 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
     if (!(x > 0.0) || !std::isfinite(x)) {
       return fail(c, TIT_E_ARG, "density must be positive and finite");
     }
+  }
+
+  // Semi-analytic wall: faces, face grid, and the fixed particles' gamma (their
+  // mass fractions) and grad gamma (their wall normals), FluidEquations::
+  // initialize (fluid_equations.hpp:75-89).
+  tit_amd::WallHost wh;
+  std::vector<double> wall_gamma;
+  if (c->walls) {
+    if (n_tanks != 1) return fail(c, TIT_E_ARG, "walls need a single tank");
+    const tit_hip_host_view& v = views[0];
+    const uint32_t nv = static_cast<uint32_t>(c->wall_verts.size() / 2);
+    if (v.n_fixed != nv) {
+      return fail(c, TIT_E_ARG, "walls: one fixed particle per wall vertex is required");
+    }
+    for (uint32_t k = 0; k < nv; ++k) {
+      for (int d = 0; d < 2; ++d) {
+        if (hr[(v.n_fluid + k) * 2 + d] != c->wall_verts[2 * k + d]) {
+          return fail(c, TIT_E_ARG, "walls: fixed particle k must sit at wall vertex k");
+        }
+      }
+    }
+    std::string err;
+    if (!tit_amd::wall_build(c->wall_verts.data(), nv, c->wall_faces.data(),
+                             static_cast<uint32_t>(c->wall_faces.size() / 2),
+                             c->wall_cverts.data(), static_cast<uint32_t>(c->wall_cverts.size() / 2),
+                             c->wall_cfaces.data(), static_cast<uint32_t>(c->wall_cfaces.size() / 2),
+                             tanks[0].lo, tanks[0].hi, c->prm.h, &wh, &err)) {
+      return fail(c, TIT_E_ARG, "walls: " + err);
+    }
+    wall_gamma.assign(nv, 0.0);
+    std::vector<double> grad(2 * static_cast<size_t>(nv), 0.0);
+    tit_amd::wall_fixed_gamma(c->dp.kernel, wh, c->prm.h, &hr[v.n_fluid * 2], nv,
+                              wall_gamma.data(), grad.data());
+    std::memcpy(&hn[v.n_fluid * 2], grad.data(), grad.size() * sizeof(double));
+    c->wall_id0 = static_cast<uint32_t>(v.n_fluid);
   }
 
   // Mean occupancy of the occupied cells (sets the default strip length).
@@ -720,6 +818,34 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   }
 
   hipStream_t s = c->stream;
+  if (c->walls) {
+    const size_t nv = wall_gamma.size();
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_face_d, std::max<size_t>(wh.faces.size(), 1)));
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_cell_off, wh.cell_off.size()));
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_cell_faces, std::max<size_t>(wh.cell_faces.size(), 1)));
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_cedges, std::max<size_t>(wh.cedges.size(), 4)));
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_mu, std::max<size_t>(nv, 1)));
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_rho, std::max<size_t>(nv, 1)));
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_out, 4 * static_cast<size_t>(n)));
+    TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_face_d, wh.faces.data(),
+                                  wh.faces.size() * sizeof(tit_amd::WallFace),
+                                  hipMemcpyHostToDevice, s));
+    TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_cell_off, wh.cell_off.data(),
+                                  wh.cell_off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_cell_faces, wh.cell_faces.data(),
+                                  wh.cell_faces.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_cedges, wh.cedges.data(),
+                                  wh.cedges.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_mu, wall_gamma.data(), nv * sizeof(double),
+                                  hipMemcpyHostToDevice, s));
+    TIT_HIP_TRY(c, hipMemsetAsync(c->wall_rho, 0, nv * sizeof(double), s));
+    c->wall_n_cedges = static_cast<uint32_t>(wh.cedges.size() / 4);
+    for (int k = 0; k < 2; ++k) {
+      c->wall_lo[k] = wh.lo[k];
+      c->wall_inv_cw[k] = wh.inv_cw[k];
+      c->wall_nc[k] = wh.nc[k];
+    }
+  }
   TIT_HIP_TRY(c, hipMemcpyAsync(c->stage_r, hr.data(), hr.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
   TIT_HIP_TRY(c, hipMemcpyAsync(c->stage_v, hv.data(), hv.size() * sizeof(double),
@@ -877,6 +1003,7 @@ tit_status tit_hip_compute_rhs(tit_hip_ctx* c, double* drho_dt, double* dv_dt) {
   if (!c->uploaded) return fail(c, TIT_E_STATE, "compute_rhs before upload");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
   enqueue_rebuild(c, kRebuildPending);
+  if (c->walls) enqueue_boundary(c, c->A);  // gamma and face terms of this state
   enqueue_stage(c, c->A, c->A, c->A, kRhsOnly, 1.0, false);
   // reuse the staging buffers: drho -> stage_rho, dv -> stage_v
   launch_export_rhs(c->stream, c->prm.dim, c->n, c->id, c->rhs_drho, c->rhs_dv,
@@ -896,6 +1023,7 @@ tit_status tit_hip_pair_counts(tit_hip_ctx* c, uint32_t* count) {
   if (!c->uploaded) return fail(c, TIT_E_STATE, "pair_counts before upload");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
   enqueue_rebuild(c, kRebuildPending);
+  if (c->walls) enqueue_boundary(c, c->A);
   enqueue_stage(c, c->A, c->A, c->A, kRhsOnly, 1.0, false);
   uint32_t* out = reinterpret_cast<uint32_t*>(c->stage_rho);  // n doubles >= n u32
   launch_export_u32(c->stream, c->n, c->id, c->rhs_count, out);

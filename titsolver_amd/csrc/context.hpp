@@ -7,6 +7,7 @@
 
 #include "../../include/tit_hip.h"
 #include "sph_kernels.hpp"
+#include "wall.hpp"
 
 namespace tit_amd {
 struct Learned;
@@ -64,6 +65,21 @@ struct tit_hip_ctx {
   uint32_t* pi_count = nullptr;
   uint32_t* pi_cells = nullptr;  // per cell mark
   unsigned long long* pi_shift = nullptr;  // per tank
+  // semi-analytic wall (tit_hip_set_walls; one 2D tank, REF terms)
+  bool walls = false;
+  std::vector<double> wall_verts, wall_cverts;
+  std::vector<uint32_t> wall_faces, wall_cfaces;
+  tit_amd::WallFace* wall_face_d = nullptr;
+  uint32_t* wall_cell_off = nullptr;
+  uint32_t* wall_cell_faces = nullptr;
+  double* wall_cedges = nullptr;
+  uint32_t wall_n_cedges = 0;
+  double wall_lo[2] = {0.0, 0.0}, wall_inv_cw[2] = {0.0, 0.0};
+  int wall_nc[2] = {0, 0};
+  double* wall_mu = nullptr;   // per vertex: gamma at set-up (mass fraction)
+  double* wall_rho = nullptr;  // per vertex: density of this stage
+  double* wall_out = nullptr;  // per slot: face terms / gamma, 1 / gamma
+  uint32_t wall_id0 = 0;       // id of the fixed particle at vertex 0
 
   // chunk schedule of the stage kernel
   std::vector<DevTank> host_tanks;

@@ -20,6 +20,10 @@ constexpr int kKernelC2 = 0;
 constexpr int kKernelC4 = 1;
 constexpr int kFlavourRef = 0;
 constexpr int kFlavourCfg = 1;
+// REF terms with the semi-analytic wall (2D): per-particle masses (fixed
+// particles carry m gamma), pair sums divided by gamma_a, and the face terms
+// added per particle.
+constexpr int kFlavourWall = 2;
 
 // Uniform parameters, passed to every kernel by value.
 struct DevParams {

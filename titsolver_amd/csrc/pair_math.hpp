@@ -12,7 +12,10 @@
 //          for approaching pairs (Monaghan 1992), else 0
 //     REF  Pi_ab = 2 mu (v_ab . r_ab) / (rho_a rho_b |r_ab|^2)   (:299-300)
 // The uniform factors m gsc, kd, kv are folded in once (FP64, then rounded),
-// so a pair costs the arithmetic that varies per pair only.
+// so a pair costs the arithmetic that varies per pair only.  With walls
+// (kFlavourWall) the masses differ per particle (fixed particles carry
+// m gamma, fluid_equations.hpp:87-88): m_b = V_b rho_b comes from the pair
+// record (V_b = m_b / rho_b in its third slot) and multiplies the whole term.
 #pragma once
 
 #include "internal.hpp"
@@ -139,7 +142,7 @@ __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
     a.rb = P.rho0_f + drho;
     a.cv = aux.y;
   }
-  a.pa = P.mg * aux.x;
+  a.pa = FLAVOUR == kFlavourWall ? aux.x : P.mg * aux.x;
 }
 
 // The kernel-gradient polynomial P(q) of grad W_ab = gsc P(q) r_ab as a
@@ -186,10 +189,20 @@ __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>&
     const float rbar = __builtin_fmaf(pvb.w, 0.5f, a.rb);
     const float num = __builtin_fmaf(csb, P.kv, a.cv) * fminf(vr, 0.0f);
     Pi = num * __builtin_amdgcn_rcpf(rbar * (r2 + P.eta2));
-  } else {
+  } else if constexpr (FLAVOUR == kFlavourRef) {
     c = __builtin_fmaf(P.mg, vr, fmaxf(a.cv, csb) * drk * Vb * r);
     const float rb = P.rho0_f + pvb.w;
     Pi = (P.kv * vr) * __builtin_amdgcn_rcpf(a.rb * rb * r2);
+  } else {  // kFlavourWall: m_b gsc = V_b rho_b gsc (kd = gsc here)
+    const float rb = P.rho0_f + pvb.w;
+    const float mb = Vb * rb * P.kd;
+    c = __builtin_fmaf(mb, vr, fmaxf(a.cv, csb) * drk * Vb * r);
+    Pi = (P.two_mu * vr) * __builtin_amdgcn_rcpf(a.rb * rb * r2);
+    cont = c * G;
+    const float coef = ((Pi - a.pa) - Pb) * (mb * G);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) mom[k] = coef * x[k];
+    return;
   }
   cont = c * G;
   const float coef = __builtin_fmaf(Pb, -P.mg, Pi - a.pa) * G;

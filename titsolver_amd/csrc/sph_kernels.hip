@@ -352,6 +352,11 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     A.s.rho[i] = rho_e;
     const double zero[DIM] = {};
     write_pair_data<DIM>(P, A.s, i, zero, rho_e);
+    if (A.wall_rho) {
+      const uint32_t k = A.id[i] - A.wall_id0;
+      A.wall_rho[k] = rho_e;
+      A.s.pv[i].z = static_cast<float>(A.wall_mu[k] * P.m / rho_e);  // V_e = m_e / rho_e
+    }
 #pragma unroll
     for (int d = 0; d < DIM; ++d) A.s.v[d][i] = 0.0;
   }

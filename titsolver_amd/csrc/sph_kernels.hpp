@@ -41,6 +41,9 @@ struct StageArgs {
   uint32_t* counts;            // [0] chunks, [1] overflow chunks
   uint32_t grid_lds, grid_global;  // persistent grid sizes
   double* dv_last[3];          // final stage: its acceleration (post-integration only)
+  // semi-analytic wall (kFlavourWall): per slot, the face terms of drho/dt
+  // and dv/dt already divided by gamma, then 1 / gamma
+  const double* wall;
 };
 
 // Post-integration passes (post_integrate.hip): state a (post-step, its
@@ -82,6 +85,12 @@ struct BoundaryArgs {
   const uint32_t* fixed_list;
   const double* normal;  // reference order, dim per particle (fixed only)
   uint32_t n_fixed;
+  // semi-analytic wall: fixed particle id0 + k sits at wall vertex k; its
+  // mass fraction gamma_k (mass m gamma_k) and its density, written here for
+  // the face terms
+  const double* wall_mu;
+  double* wall_rho;
+  uint32_t wall_id0;
 };
 
 // Device-side control of the cell rebuild.  ctl[0]: the rebuild of this step
@@ -120,6 +129,27 @@ void launch_restore_order(hipStream_t s, int dim, uint32_t n, const uint32_t* pe
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
                          const uint32_t* key, uint32_t* cell_off, const uint32_t* ctl);
 void launch_boundary(hipStream_t s, const BoundaryArgs& a);
+
+// Semi-analytic wall (wall.hip): per fluid particle of the stage input state,
+// gamma, grad gamma and the face terms, into out (4 per slot: face drho/dt and
+// dv/dt already divided by gamma, then 1 / gamma).
+struct WallFace;
+struct WallArgs {
+  DevParams P;
+  uint32_t n;
+  DevState s;
+  const uint32_t* tag;
+  const WallFace* faces;
+  const uint32_t* cell_faces_off;  // CSR of ascending face indices per grid cell
+  const uint32_t* cell_faces;
+  double lo[2], inv_cw[2];
+  int nc[2];
+  const double* cedges;            // containment polygon, 4 per edge
+  uint32_t n_cedges;
+  const double* vert_rho;          // density of the fixed particle at each vertex
+  double* out;
+};
+void launch_wall(hipStream_t s, const WallArgs& a);
 // post_integrate.hip: the passes in order, the shift into disp[] (flags bit 0
 // when it passes skin_half), the next step size into dt_min.
 void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* disp,

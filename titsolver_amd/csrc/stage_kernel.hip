@@ -719,10 +719,18 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
     if constexpr (KIND == kRhsOnly) {
       if (A.rhs_count) A.rhs_count[i] = count;
     }
-    const double drho = static_cast<double>(acc.rho.value());
+    double drho = static_cast<double>(acc.rho.value());
     double dv[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) dv[k] = static_cast<double>(acc.v[k].value());
+    if constexpr (FLAVOUR == kFlavourWall) {
+      // face terms (already divided by gamma_a) + pair sums / gamma_a
+      const double* wt = A.wall + 4 * static_cast<size_t>(i);
+      const double ig = wt[3];
+      drho = wt[0] + drho * ig;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) dv[k] = wt[1 + k] + dv[k] * ig;
+    }
     dv[1] = dv[1] - P.g;
     if constexpr (KIND == kRhsOnly) {
       A.rhs_drho[i] = drho;
@@ -1010,6 +1018,8 @@ template <int DIM, int KERNEL>
 void dispatch_flavour(hipStream_t s, const StageArgs& a, int kind, bool final) {
   if (a.P.flavour == kFlavourCfg) {
     dispatch_kind<DIM, KERNEL, kFlavourCfg>(s, a, kind, final);
+  } else if (a.P.flavour == kFlavourWall) {
+    if constexpr (DIM == 2) dispatch_kind<DIM, KERNEL, kFlavourWall>(s, a, kind, final);
   } else {
     dispatch_kind<DIM, KERNEL, kFlavourRef>(s, a, kind, final);
   }

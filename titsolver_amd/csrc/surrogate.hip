@@ -1168,6 +1168,7 @@ extern "C" {
 tit_status tit_hip_learned_init(tit_hip_ctx* c, uint64_t seed) {
   if (!c) return TIT_E_ARG;
   if (!c->uploaded) return fail(c, TIT_E_STATE, "learned init before upload");
+  if (c->walls) return fail(c, TIT_E_ARG, "the learned step runs without walls");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
   destroy_learned(c);
   auto* L = new (std::nothrow) Learned();

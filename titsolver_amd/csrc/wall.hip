@@ -1,0 +1,273 @@
+// Semi-analytic wall boundary, 2D (fluid_equations.hpp:168-193, 243-247,
+// 277-291).  Per stage, after the boundary extrapolation, one thread per
+// fluid particle:
+//   grad gamma_a = sum over faces within 2h (ascending index) of Kernel::flux;
+//   gamma_a      = [winding number > 1/2] - sum of Kernel::antigrad_flux at
+//                  r_a + (2 gamma - 1) n_a h^2, when |grad gamma_a| is not tiny;
+//   face terms   drho: - rho_s v_as . grad gamma_as / gamma_a
+//                dv:   (P_as grad gamma_as - Pi_as |grad gamma_as|) / gamma_a
+// with s the face's two vertex (fixed) particles, their fields averaged.  The
+// stage kernel adds these to its pair sums divided by gamma_a.  The faces
+// within 2h come from a static grid of cells of width 2h listing every face
+// whose box reaches the cell grown by 2h; the exact segment-sphere predicate
+// then selects them in ascending order, as the reference's sorted face search
+// does (particle_mesh.hpp:150-162).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "internal.hpp"
+#include "sph_kernels.hpp"
+#include "wall.hpp"
+
+namespace tit_amd {
+
+namespace {
+
+constexpr int kWallBlock = 256;
+
+__device__ __forceinline__ void normalize2(double x, double y, double* out) {
+  const double n2 = x * x + y * y;
+  if (n2 >= kWallTiny * kWallTiny) {
+    const double s = sqrt(n2);
+    out[0] = x / s;
+    out[1] = y / s;
+  } else {
+    out[0] = 0.0;
+    out[1] = 0.0;
+  }
+}
+
+template <int KERNEL>
+__global__ __launch_bounds__(kWallBlock) void k_wall(WallArgs A) {
+  const uint32_t i = blockIdx.x * kWallBlock + threadIdx.x;
+  if (i >= A.n) return;
+  if (A.tag[i] & 1u) return;  // fluid particles only
+  const DevParams& P = A.P;
+  const double x = A.s.x[0][i], y = A.s.x[1][i];
+  const double h = P.h, R = 2.0 * h;
+
+  // this particle's grid cell (escaped particles clamp to the border)
+  int c[2];
+  const double xy[2] = {x, y};
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double f = (xy[k] - A.lo[k]) * A.inv_cw[k];
+    c[k] = !(f > 0.0) ? 0 : (f >= static_cast<double>(A.nc[k] - 1) ? A.nc[k] - 1 : static_cast<int>(f));
+  }
+  const uint32_t cell = static_cast<uint32_t>(c[0]) * static_cast<uint32_t>(A.nc[1]) +
+                        static_cast<uint32_t>(c[1]);
+  const uint32_t fb = A.cell_faces_off[cell], fe = A.cell_faces_off[cell + 1];
+
+  double gg[2] = {0.0, 0.0};
+  uint32_t n_acc = 0;
+  for (uint32_t j = fb; j < fe; ++j) {
+    const WallFace f = A.faces[A.cell_faces[j]];
+    if (!wall_face_intersects(f, x, y, R)) continue;
+    double fl[2];
+    wall_face_flux<KERNEL>(f, x, y, h, fl);
+    gg[0] += fl[0];
+    gg[1] += fl[1];
+    ++n_acc;
+  }
+  double gamma = wall_winding(A.cedges, A.n_cedges, x, y) > 0.5 ? 1.0 : 0.0;
+  const double ng = sqrt(gg[0] * gg[0] + gg[1] * gg[1]);
+  double out[4] = {0.0, 0.0, 0.0, 0.0};
+  if (n_acc > 0 && !(fabs(ng) <= kWallTiny)) {
+    const double nax = gg[0] / ng, nay = gg[1] / ng;
+    const double sgn = 2.0 * gamma - 1.0;
+    const double xs = x + sgn * nax * (h * h), ys = y + sgn * nay * (h * h);
+    for (uint32_t j = fb; j < fe; ++j) {
+      const WallFace f = A.faces[A.cell_faces[j]];
+      if (!wall_face_intersects(f, x, y, R)) continue;
+      gamma -= wall_face_antigrad_flux<KERNEL>(f, xs, ys, h);
+    }
+  }
+  if (n_acc > 0) {
+    const double rho_a = A.s.rho[i];
+    const double va[2] = {A.s.v[0][i], A.s.v[1][i]};
+    const double pa = eos_pressure(P, rho_a) / (rho_a * rho_a);
+    for (uint32_t j = fb; j < fe; ++j) {
+      const WallFace f = A.faces[A.cell_faces[j]];
+      if (!wall_face_intersects(f, x, y, R)) continue;
+      double g[2];
+      wall_face_flux<KERNEL>(f, x, y, h, g);
+      const double r0 = A.vert_rho[f.s0], r1 = A.vert_rho[f.s1];
+      const double rho_s = (r0 + r1) / 2.0;
+      const double p_s = (eos_pressure(P, r0) + eos_pressure(P, r1)) / 2.0;
+      // the wall particles do not move: v_as = v_a, r_as = r_a - face midpoint
+      const double vg = va[0] * g[0] + va[1] * g[1];
+      out[0] -= rho_s * vg / gamma;
+      const double P_as = rho_s * (pa + p_s / (rho_s * rho_s));
+      double n_s[2], t_as[2];
+      normalize2(g[0], g[1], n_s);
+      const double vn = va[0] * n_s[0] + va[1] * n_s[1];
+      normalize2(va[0] - vn * n_s[0], va[1] - vn * n_s[1], t_as);
+      const double rax = x - (f.ax + f.bx) / 2.0, ray = y - (f.ay + f.by) / 2.0;
+      const double rn = rax * n_s[0] + ray * n_s[1];
+      const double dr_as = h / 2.0 > rn ? h / 2.0 : rn;
+      const double pc = 2.0 * P.mu / (rho_a * dr_as) * (va[0] * t_as[0] + va[1] * t_as[1]);
+      const double gn = sqrt(g[0] * g[0] + g[1] * g[1]);
+      out[1] += (P_as * g[0] - pc * t_as[0] * gn) / gamma;
+      out[2] += (P_as * g[1] - pc * t_as[1] * gn) / gamma;
+    }
+  }
+  out[3] = 1.0 / gamma;
+  double* o = A.out + 4 * static_cast<size_t>(i);
+  o[0] = out[0];
+  o[1] = out[1];
+  o[2] = out[2];
+  o[3] = out[3];
+}
+
+template <int KERNEL>
+void fixed_gamma(const WallHost& W, double h, const double* r, uint32_t n, double* gamma,
+                 double* grad) {
+  const double R = 2.0 * h;
+  for (uint32_t e = 0; e < n; ++e) {
+    const double x = r[2 * e], y = r[2 * e + 1];
+    int c[2];
+    const double xy[2] = {x, y};
+    for (int k = 0; k < 2; ++k) {
+      const double f = (xy[k] - W.lo[k]) * W.inv_cw[k];
+      c[k] = !(f > 0.0) ? 0 : (f >= static_cast<double>(W.nc[k] - 1) ? W.nc[k] - 1 : static_cast<int>(f));
+    }
+    const uint32_t cell = static_cast<uint32_t>(c[0]) * static_cast<uint32_t>(W.nc[1]) +
+                          static_cast<uint32_t>(c[1]);
+    double gg[2] = {0.0, 0.0};
+    for (uint32_t j = W.cell_off[cell]; j < W.cell_off[cell + 1]; ++j) {
+      const WallFace& f = W.faces[W.cell_faces[j]];
+      if (!wall_face_intersects(f, x, y, R)) continue;
+      double fl[2];
+      wall_face_flux<KERNEL>(f, x, y, h, fl);
+      gg[0] += fl[0];
+      gg[1] += fl[1];
+    }
+    double g = wall_winding(W.cedges.data(), static_cast<uint32_t>(W.cedges.size() / 4), x, y) > 0.5
+                   ? 1.0
+                   : 0.0;
+    const double ng = std::sqrt(gg[0] * gg[0] + gg[1] * gg[1]);
+    if (!(std::fabs(ng) <= kWallTiny)) {
+      const double nax = gg[0] / ng, nay = gg[1] / ng;
+      const double sgn = 2.0 * g - 1.0;
+      const double xs = x + sgn * nax * (h * h), ys = y + sgn * nay * (h * h);
+      for (uint32_t j = W.cell_off[cell]; j < W.cell_off[cell + 1]; ++j) {
+        const WallFace& f = W.faces[W.cell_faces[j]];
+        if (!wall_face_intersects(f, x, y, R)) continue;
+        g -= wall_face_antigrad_flux<KERNEL>(f, xs, ys, h);
+      }
+    }
+    gamma[e] = g;
+    grad[2 * e] = gg[0];
+    grad[2 * e + 1] = gg[1];
+  }
+}
+
+}  // namespace
+
+bool wall_build(const double* verts, uint32_t n_verts, const uint32_t* faces, uint32_t n_faces,
+                const double* cverts, uint32_t n_cverts, const uint32_t* cfaces,
+                uint32_t n_cfaces, const double* lo, const double* hi, double h, WallHost* W,
+                std::string* err) {
+  W->faces.clear();
+  W->cedges.clear();
+  for (uint32_t f = 0; f < n_faces; ++f) {
+    const uint32_t ia = faces[2 * f], ib = faces[2 * f + 1];
+    if (ia >= n_verts || ib >= n_verts) {
+      *err = "wall face names a vertex out of range";
+      return false;
+    }
+    WallFace w{};
+    w.ax = verts[2 * ia];
+    w.ay = verts[2 * ia + 1];
+    w.bx = verts[2 * ib];
+    w.by = verts[2 * ib + 1];
+    // normal = normalize(cross(b - a)), cross(v) = (v1, -v0); e = normalize(b - a)
+    const double bax = w.bx - w.ax, bay = w.by - w.ay;
+    const double wnx = bay, wny = -bax;
+    const double n2 = wnx * wnx + wny * wny;
+    if (n2 >= kWallTiny * kWallTiny) {
+      const double s = std::sqrt(n2);
+      w.nx = wnx / s;
+      w.ny = wny / s;
+    }
+    const double e2 = bax * bax + bay * bay;
+    if (e2 >= kWallTiny * kWallTiny) {
+      const double s = std::sqrt(e2);
+      w.ex = bax / s;
+      w.ey = bay / s;
+    }
+    w.s0 = ia;
+    w.s1 = ib;
+    W->faces.push_back(w);
+  }
+  for (uint32_t f = 0; f < n_cfaces; ++f) {
+    const uint32_t ia = cfaces[2 * f], ib = cfaces[2 * f + 1];
+    if (ia >= n_cverts || ib >= n_cverts) {
+      *err = "containment face names a vertex out of range";
+      return false;
+    }
+    W->cedges.insert(W->cedges.end(),
+                     {cverts[2 * ia], cverts[2 * ia + 1], cverts[2 * ib], cverts[2 * ib + 1]});
+  }
+  // grid of cells of width >= 2h over the tank box, at most 4096 per axis
+  const double R = 2.0 * h;
+  for (int k = 0; k < 2; ++k) {
+    const double span = hi[k] - lo[k];
+    if (!(span > 0.0)) {
+      *err = "empty tank box";
+      return false;
+    }
+    const double cw = std::max(R, span / 4096.0);
+    W->nc[k] = std::max(1, static_cast<int>(std::ceil(span / cw)));
+    W->lo[k] = lo[k];
+    W->inv_cw[k] = 1.0 / cw;
+  }
+  const size_t n_cells = static_cast<size_t>(W->nc[0]) * static_cast<size_t>(W->nc[1]);
+  std::vector<std::vector<uint32_t>> lists(n_cells);
+  const double margin = R * (1.0 + 1e-6);
+  for (uint32_t f = 0; f < n_faces; ++f) {
+    const WallFace& w = W->faces[f];
+    int c0[2], c1[2];
+    const double fa[2] = {w.ax, w.ay}, fb2[2] = {w.bx, w.by};
+    for (int k = 0; k < 2; ++k) {
+      const double l = std::min(fa[k], fb2[k]) - margin, u = std::max(fa[k], fb2[k]) + margin;
+      const double fl = (l - W->lo[k]) * W->inv_cw[k], fu = (u - W->lo[k]) * W->inv_cw[k];
+      c0[k] = fl <= 0.0 ? 0 : std::min(W->nc[k] - 1, static_cast<int>(fl));
+      c1[k] = fu <= 0.0 ? 0 : std::min(W->nc[k] - 1, static_cast<int>(fu));
+    }
+    for (int ix = c0[0]; ix <= c1[0]; ++ix) {
+      for (int iy = c0[1]; iy <= c1[1]; ++iy) {
+        lists[static_cast<size_t>(ix) * W->nc[1] + iy].push_back(f);
+      }
+    }
+  }
+  W->cell_off.assign(n_cells + 1, 0u);
+  W->cell_faces.clear();
+  for (size_t c = 0; c < n_cells; ++c) {
+    W->cell_faces.insert(W->cell_faces.end(), lists[c].begin(), lists[c].end());
+    W->cell_off[c + 1] = static_cast<uint32_t>(W->cell_faces.size());
+  }
+  return true;
+}
+
+void wall_fixed_gamma(int kernel, const WallHost& W, double h, const double* r, uint32_t n,
+                      double* gamma, double* grad) {
+  if (kernel == kKernelC4) {
+    fixed_gamma<kKernelC4>(W, h, r, n, gamma, grad);
+  } else {
+    fixed_gamma<kKernelC2>(W, h, r, n, gamma, grad);
+  }
+}
+
+void launch_wall(hipStream_t s, const WallArgs& a) {
+  if (a.n == 0) return;
+  const uint32_t g = (a.n + kWallBlock - 1) / kWallBlock;
+  if (a.P.kernel == kKernelC4) {
+    k_wall<kKernelC4><<<g, kWallBlock, 0, s>>>(a);
+  } else {
+    k_wall<kKernelC2><<<g, kWallBlock, 0, s>>>(a);
+  }
+}
+
+}  // namespace tit_amd

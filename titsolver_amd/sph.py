@@ -132,11 +132,77 @@ def dam_break(dim: int, dr: float, tank, column, layers: int = 3,
                          tuple(hi))
 
 
+@dataclass
+class Walls:
+    """Semi-analytic wall boundary, 2D: the wall surface (vertices, faces as
+    vertex pairs, each face's normal normalize(cross(b - a)) pointing into the
+    fluid) and a closed polygon whose winding number decides containment, as
+    FluidEquations takes them (fluid_equations.hpp:58-69)."""
+    verts: np.ndarray   # (n_verts, 2)
+    faces: np.ndarray   # (n_faces, 2) uint32
+    cverts: np.ndarray  # (n_cverts, 2)
+    cfaces: np.ndarray  # (n_cfaces, 2) uint32
+
+
+def tessellate(verts, faces, d_max: float):
+    """geom::tessellate (geom/tessellation.hpp:28-62): every face split into
+    n = max(1, ceil(length / d_max)) equal segments; the new vertices follow
+    the original ones, face by face."""
+    if not d_max > 0.0:
+        raise ValueError("maximum segment length must be positive")
+    out_v = [tuple(map(float, v)) for v in np.asarray(verts, dtype=np.float64)]
+    out_f = []
+    for ia, ib in np.asarray(faces, dtype=np.int64):
+        a, b = np.asarray(out_v[ia]), np.asarray(out_v[ib])
+        ba = b - a
+        length = math.sqrt(ba[1] * ba[1] + ba[0] * ba[0])  # norm(cross(ba))
+        n = max(1, int(math.ceil(length / d_max)))
+        prev = int(ia)
+        for i in range(1, n):
+            t = float(i) / float(n)
+            out_v.append(tuple(a + t * ba))
+            out_f.append((prev, len(out_v) - 1))
+            prev = len(out_v) - 1
+        out_f.append((prev, int(ib)))
+    return np.asarray(out_v, dtype=np.float64), np.asarray(out_f, dtype=np.uint32)
+
+
+def box_walls(width: float, height: float, dr: float) -> Walls:
+    """The closed pool of titwcsph (wcsph.cpp:55-86): the wall surface
+    (0, H) -> (W, H) -> (W, 0) -> (0, 0) -> (0, H), tessellated to dr, and the
+    counter-clockwise box (0, 0), (W, 0), (W, H), (0, H) for containment."""
+    sv = [(0.0, height), (width, height), (width, 0.0), (0.0, 0.0)]
+    sq = [(0, 1), (1, 2), (2, 3), (3, 0)]
+    verts, faces = tessellate(sv, sq, dr)
+    cverts = np.asarray([(0.0, 0.0), (width, 0.0), (width, height), (0.0, height)])
+    return Walls(verts, faces, cverts, np.asarray(sq, dtype=np.uint32))
+
+
+def dam_break_walls(dr: float, tank, column, rho0: float = 1000.0, g: float = 9.81,
+                    cs0: float = 62.64, margin: float | None = None):
+    """The dam break of titwcsph with the semi-analytic wall: the fluid
+    lattice r = dr (i + 1, j + 1) with the hydrostatic start, and one fixed
+    particle at each wall vertex (wcsph.cpp:103-142).  Returns
+    (ParticleArray, Walls)."""
+    fluid = dam_break(2, dr, tank, column, layers=0, rho0=rho0, g=g, cs0=cs0)
+    walls = box_walls(float(tank[0]), float(tank[1]), dr)
+    nf, nv = fluid.n_fluid, len(walls.verts)
+    r = np.concatenate([fluid.r[:nf], walls.verts])
+    rho = np.concatenate([fluid.rho[:nf], np.full(nv, rho0)])
+    if margin is None:
+        margin = 4.0 * dr
+    lo = (-margin, -margin)
+    hi = (float(tank[0]) + margin, float(tank[1]) + margin)
+    return ParticleArray(2, r, rho, nf, np.zeros((nv, 2)), lo, hi), walls
+
+
 class HipContext:
-    """One tit_hip_ctx: device buffers, stream and captured step graph."""
+    """One tit_hip_ctx: device buffers, stream and captured step graph.  With
+    walls (a Walls), the semi-analytic wall boundary (tit_hip_set_walls)."""
 
     def __init__(self, equations: FluidEquations, order: int = 3,
-                 device: int = 0, skin: float = 0.0, post_integrate: bool = False):
+                 device: int = 0, skin: float = 0.0, post_integrate: bool = False,
+                 walls: Walls | None = None):
         self.lib = _abi.load()
         self.eq = equations
         self.params = equations.to_params(order, skin, post_integrate)
@@ -146,6 +212,16 @@ class HipContext:
             raise TitError(st, "tit_hip_create failed (is a GPU visible?)")
         self.h = h
         self.sizes: list[tuple[int, int]] = []
+        if walls is not None:
+            self._walls = [np.ascontiguousarray(walls.verts, dtype=np.float64),
+                           np.ascontiguousarray(walls.faces, dtype=np.uint32),
+                           np.ascontiguousarray(walls.cverts, dtype=np.float64),
+                           np.ascontiguousarray(walls.cfaces, dtype=np.uint32)]
+            v, f, cv, cf = self._walls
+            u32 = C.POINTER(C.c_uint32)
+            w = _abi.Walls(_dp(v), len(v), len(f), f.ctypes.data_as(u32), _dp(cv), len(cv),
+                           len(cf), cf.ctypes.data_as(u32))
+            self._check(self.lib.tit_hip_set_walls(self.h, C.byref(w)))
 
     def _check(self, st: int):
         if st != 0:
