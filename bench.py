@@ -63,10 +63,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dr", type=float, default=None,
                     help="particle spacing (default: the workload's)")
-    ap.add_argument("--workload", choices=["dam2d", "dam3d", "batched"], default="dam2d",
+    ap.add_argument("--workload", choices=["dam2d", "dam3d", "batched", "dam2d_walls"],
+                    default="dam2d",
                     help="dam2d: 2D dam break (default, dr = 1 mm, 2.0 M fluid); "
                          "dam3d: 3D dam break (dr = 1 cm, h = 1.3 dr, 2.0 M fluid); "
-                         "batched: 256 seeded 2D tanks (dr = 5 mm)")
+                         "batched: 256 seeded 2D tanks (dr = 5 mm); "
+                         "dam2d_walls: the 2D dam break in the closed pool with the "
+                         "semi-analytic wall, reference terms, Wendland C4, h = 2 dr "
+                         "(dr = 1 mm, 2.0 M fluid)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default stage schedule")
     ap.add_argument("--profile-steps", type=int, default=60,
@@ -130,14 +134,14 @@ def main():
 
     global EAGER
     EAGER = args.eager
-    arrays, eq, desc = build_workload(args)
+    arrays, eq, desc, walls = build_workload(args)
     if args.post_integrate:
         desc += " + post_integrate (particle shifting, free-surface correction)"
     pa = arrays[0]
     nf = sum(a.n_fluid for a in arrays)
     n_fixed = sum(a.n_fixed for a in arrays)
     dim = eq.dim
-    ctx = make_ctx(eq, arrays, args.post_integrate)
+    ctx = make_ctx(eq, arrays, args.post_integrate, walls)
     print(f"bench: {desc}: {nf} fluid + {n_fixed} fixed particles uploaded; "
           f"warmup {args.warmup}", file=sys.stderr, flush=True)
     ctx.step(args.warmup)
@@ -187,8 +191,7 @@ def main():
         "dtype": "f64 state/integrator, f32 pair sums (Kahan)",
         "data": "synthetic (lattice dam break, hydrostatic start)",
         "config": {"workload": f"{desc}, {nf} fluid + {n_fixed} fixed particles, "
-                               f"Wendland C2 h={eq.h / args.dr:g}dr, delta-SPH 0.1, "
-                               "alpha 0.02, SSPRK3 CFL 0.25",
+                               + terms_of(eq, args.dr),
                    "fluid_particles": nf, "fixed_particles": n_fixed,
                    "sim_time_s": float(t_step[0]), "dt_next_s": float(dt_next[0]),
                    "schedule": sched, "autotune": tuned},
@@ -208,16 +211,26 @@ def main():
         out["copy_probe"] = copy_probe(args.copy_probe_mb)
     if not args.no_cpu and args.cpu_steps > 0 and len(arrays) == 1:
         out["cpu_baseline"], out["parity"] = cpu_baseline(pa, eq, args.cpu_steps,
-                                                          args.post_integrate)
+                                                          args.post_integrate, walls)
     print(json.dumps(out), flush=True)
 
 
 EAGER = False
 
 
-def make_ctx(eq, arrays, post_integrate=False):
+def terms_of(eq, dr):
     import titsolver_amd as ta
-    ctx = ta.HipContext(eq, order=3, post_integrate=post_integrate)
+    kern = "Wendland C4" if eq.kernel == ta.KERNEL_WENDLAND_C4 else "Wendland C2"
+    if eq.flavour == ta.FLAVOUR_CFG:
+        terms = f"delta-SPH {eq.delta:g}, alpha {eq.alpha:g}"
+    else:
+        terms = f"Ferrari diffusion, mu {eq.mu:g}"
+    return f"{kern} h={eq.h / dr:g}dr, {terms}, SSPRK3 CFL {eq.cfl:g}"
+
+
+def make_ctx(eq, arrays, post_integrate=False, walls=None):
+    import titsolver_amd as ta
+    ctx = ta.HipContext(eq, order=3, post_integrate=post_integrate, walls=walls)
     ctx.upload(arrays)
     if EAGER:
         ctx.set_profiling(True)
@@ -230,7 +243,16 @@ def build_workload(args):
         args.dr = args.dr or 0.001
         arrays = [ta.dam_break(2, args.dr, TANK, COLUMN)]
         eq = ta.cfg_equations(2, args.dr, COLUMN[1])
-        return arrays, eq, f"2D dam break dr={args.dr} m"
+        return arrays, eq, f"2D dam break dr={args.dr} m", None
+    if args.workload == "dam2d_walls":
+        # titwcsph's set-up: closed pool, one wall particle per wall vertex,
+        # Wendland C4, h = 2 dr, cs0 = 20 sqrt(g H), mu = 1e-3, CFL 0.4
+        # (wcsph.cpp:35-113)
+        args.dr = args.dr or 0.001
+        pa, walls = ta.dam_break_walls(args.dr, TANK, COLUMN)
+        eq = ta.ref_equations(2, args.dr, COLUMN[1])
+        return [pa], eq, (f"2D dam break in the closed {TANK[0]} x {TANK[1]} m pool, "
+                          f"semi-analytic wall, dr={args.dr} m"), walls
     if args.workload == "dam3d":
         # tank 3.22 x 2.4 (open top) x 1.0 m, water 1.0 x 2.0 x 1.0 m
         args.dr = args.dr or 0.01
@@ -238,11 +260,11 @@ def build_workload(args):
         tank3 = [TANK[0], TANK[1], (k + 1) * args.dr]
         arrays = [ta.dam_break(3, args.dr, tank3, [COLUMN[0], COLUMN[1], k * args.dr])]
         eq = ta.cfg_equations(3, args.dr, COLUMN[1], h_factor=1.3)
-        return arrays, eq, f"3D dam break dr={args.dr} m"
+        return arrays, eq, f"3D dam break dr={args.dr} m", None
     args.dr = args.dr or 0.005
     arrays = seeded_tanks(0, args.dr, n_tanks=256)
     eq = ta.cfg_equations(2, args.dr, 2.0)
-    return arrays, eq, f"256 batched 2D dam breaks (seeded columns) dr={args.dr} m"
+    return arrays, eq, f"256 batched 2D dam breaks (seeded columns) dr={args.dr} m", None
 
 
 def seeded_tanks(seed: int, dr: float, n_tanks: int = 0, min_fluid: int = 0):
@@ -340,7 +362,7 @@ def learned_leg(args):
     return res
 
 
-def cpu_baseline(pa, eq, steps, post_integrate=False):
+def cpu_baseline(pa, eq, steps, post_integrate=False, walls=None):
     """FP64 oracle (single thread) on the same workload for a few steps, and
     the GPU's error against it after the same steps from the same state."""
     import oracle
@@ -349,9 +371,10 @@ def cpu_baseline(pa, eq, steps, post_integrate=False):
     nf = pa.n_fluid
     print(f"bench: CPU oracle {steps} steps on {pa.n} particles", file=sys.stderr,
           flush=True)
-    prm = oracle.params(eq)
+    prm = oracle.params(eq, walls)
     prm.post_integrate = 1 if post_integrate else 0
     st = oracle.State.from_particles(pa)
+    oracle.initialize(prm, st)
     t0 = time.perf_counter()
     for _ in range(steps):
         oracle.ssprk_step(prm, st, 3)
@@ -360,7 +383,7 @@ def cpu_baseline(pa, eq, steps, post_integrate=False):
            "kind": "port",
            "sample": f"{steps} SSPRK3 steps of the same {nf}-fluid-particle dam break "
                      f"(FP64 oracle, grid search per stage), {wall:.1f} s"}
-    ctx = ta.HipContext(eq, order=3, post_integrate=post_integrate)
+    ctx = ta.HipContext(eq, order=3, post_integrate=post_integrate, walls=walls)
     ctx.upload([pa])
     ctx.step(steps)
     g = ctx.download()

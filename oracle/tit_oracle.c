@@ -579,6 +579,106 @@ static const double* wall_vert(const orc_walls* w, size_t f, int k) {
   return w->verts + 2 * w->faces[2 * f + k];
 }
 
+/* Candidate faces of a point: a uniform grid of cells of width >= R over the
+ * faces' box, each cell listing (ascending) the faces whose box grown by R
+ * reaches it, as GridFaceIndex does (geom/face_search/grid_face_search.hpp:
+ * 43-111).  The exact predicate then selects the faces, so the sets and their
+ * order are those of a scan over every face.  Rebuilt when the wall or R
+ * changes. */
+typedef struct face_grid {
+  const orc_walls* w;
+  const double* verts;
+  const uint32_t* faces;
+  size_t n_faces;
+  double R;
+  double lo[2], inv[2];
+  int nc[2];
+  uint32_t* off;
+  uint32_t* idx;
+} face_grid;
+static face_grid g_fg;
+
+static int face_cell(const face_grid* g, int k, double x) {
+  const double f = (x - g->lo[k]) * g->inv[k];
+  if (!(f > 0.0)) return 0;
+  if (f >= (double)(g->nc[k] - 1)) return g->nc[k] - 1;
+  return (int)f;
+}
+
+static const face_grid* faces_near(const orc_walls* w, double R) {
+  face_grid* g = &g_fg;
+  if (g->w == w && g->verts == w->verts && g->faces == w->faces && g->n_faces == w->n_faces &&
+      g->R == R && g->off) {
+    return g;
+  }
+  free(g->off);
+  free(g->idx);
+  memset(g, 0, sizeof *g);
+  double lo[2] = {DBL_MAX, DBL_MAX}, hi[2] = {-DBL_MAX, -DBL_MAX};
+  for (size_t v = 0; v < w->n_verts; ++v) {
+    for (int k = 0; k < 2; ++k) {
+      const double x = w->verts[2 * v + k];
+      if (x < lo[k]) lo[k] = x;
+      if (x > hi[k]) hi[k] = x;
+    }
+  }
+  for (int k = 0; k < 2; ++k) {
+    const double span = hi[k] - lo[k] + 2.0 * R;
+    double cw = span / 2048.0 > R ? span / 2048.0 : R;
+    g->nc[k] = (int)ceil(span / cw);
+    if (g->nc[k] < 1) g->nc[k] = 1;
+    g->lo[k] = lo[k] - R;
+    g->inv[k] = 1.0 / cw;
+  }
+  const size_t nc = (size_t)g->nc[0] * (size_t)g->nc[1];
+  g->off = (uint32_t*)calloc(nc + 1, sizeof(uint32_t));
+  const double m = R * (1.0 + 1e-6);
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1) {
+      for (size_t c = 0; c < nc; ++c) g->off[c + 1] += g->off[c];
+      g->idx = (uint32_t*)malloc((g->off[nc] ? g->off[nc] : 1) * sizeof(uint32_t));
+    }
+    uint32_t* fill = NULL;
+    if (pass == 1) {
+      fill = (uint32_t*)malloc((nc ? nc : 1) * sizeof(uint32_t));
+      memcpy(fill, g->off, nc * sizeof(uint32_t));
+    }
+    for (size_t f = 0; f < w->n_faces; ++f) {
+      const double* a = wall_vert(w, f, 0);
+      const double* b = wall_vert(w, f, 1);
+      int c0[2], c1[2];
+      for (int k = 0; k < 2; ++k) {
+        c0[k] = face_cell(g, k, (a[k] < b[k] ? a[k] : b[k]) - m);
+        c1[k] = face_cell(g, k, (a[k] < b[k] ? b[k] : a[k]) + m);
+      }
+      for (int ix = c0[0]; ix <= c1[0]; ++ix) {
+        for (int iy = c0[1]; iy <= c1[1]; ++iy) {
+          const size_t c = (size_t)ix * g->nc[1] + iy;
+          if (pass == 0) {
+            g->off[c + 1]++;
+          } else {
+            g->idx[fill[c]++] = (uint32_t)f;
+          }
+        }
+      }
+    }
+    free(fill);
+  }
+  g->w = w;
+  g->verts = w->verts;
+  g->faces = w->faces;
+  g->n_faces = w->n_faces;
+  g->R = R;
+  return g;
+}
+
+/* The candidate range of x: faces idx[*b .. *e). */
+static void face_range(const face_grid* g, const double* x, uint32_t* b, uint32_t* e) {
+  const size_t c = (size_t)face_cell(g, 0, x[0]) * g->nc[1] + face_cell(g, 1, x[1]);
+  *b = g->off[c];
+  *e = g->off[c + 1];
+}
+
 /* compute_gamma (fluid_equations.hpp:168-193) for every particle, over the
  * faces within the kernel radius in ascending order (ParticleMesh::search_,
  * particle_mesh.hpp:150-162). */
@@ -587,10 +687,14 @@ int orc_compute_gamma(const orc_params* prm, orc_state* s) {
   if (!w || prm->dim != 2 || !s->gamma || !s->grad_gamma) return -1;
   const size_t n = s->n_fluid + s->n_fixed;
   const double h = prm->h, R = orc_kernel_radius(prm->kernel, h);
+  const face_grid* fg = faces_near(w, R);
   for (size_t i = 0; i < n; ++i) {
     const double* x = s->r + 2 * i;
     double gg[2] = {0.0, 0.0};
-    for (size_t f = 0; f < w->n_faces; ++f) {
+    uint32_t jb, je;
+    face_range(fg, x, &jb, &je);
+    for (uint32_t j = jb; j < je; ++j) {
+      const size_t f = fg->idx[j];
       if (!orc_face_intersects(wall_vert(w, f, 0), wall_vert(w, f, 1), x, R)) continue;
       double fl[2];
       orc_face_flux(prm->kernel, wall_vert(w, f, 0), wall_vert(w, f, 1), x, h, fl);
@@ -603,7 +707,8 @@ int orc_compute_gamma(const orc_params* prm, orc_state* s) {
       const double na[2] = {gg[0] / ng, gg[1] / ng};
       const double sgn = 2.0 * gamma - 1.0;
       const double xs[2] = {x[0] + sgn * na[0] * pow2(h), x[1] + sgn * na[1] * pow2(h)};
-      for (size_t f = 0; f < w->n_faces; ++f) {
+      for (uint32_t j = jb; j < je; ++j) {
+        const size_t f = fg->idx[j];
         if (!orc_face_intersects(wall_vert(w, f, 0), wall_vert(w, f, 1), x, R)) continue;
         gamma -= orc_face_antigrad_flux(prm->kernel, wall_vert(w, f, 0), wall_vert(w, f, 1), xs, h);
       }
@@ -685,7 +790,11 @@ static void wall_continuity(const orc_params* prm, orc_state* s, size_t a) {
   const size_t nf = s->n_fluid;
   const double* x = s->r + 2 * a;
   const double R = orc_kernel_radius(prm->kernel, prm->h);
-  for (size_t f = 0; f < w->n_faces; ++f) {
+  const face_grid* fg = faces_near(w, R);
+  uint32_t jb, je;
+  face_range(fg, x, &jb, &je);
+  for (uint32_t j = jb; j < je; ++j) {
+    const size_t f = fg->idx[j];
     if (!orc_face_intersects(wall_vert(w, f, 0), wall_vert(w, f, 1), x, R)) continue;
     double gg[2];
     orc_face_flux(prm->kernel, wall_vert(w, f, 0), wall_vert(w, f, 1), x, prm->h, gg);
@@ -703,7 +812,11 @@ static void wall_momentum(const orc_params* prm, orc_state* s, size_t a) {
   const size_t nf = s->n_fluid;
   const double* x = s->r + 2 * a;
   const double h = prm->h, R = orc_kernel_radius(prm->kernel, h);
-  for (size_t f = 0; f < w->n_faces; ++f) {
+  const face_grid* fg = faces_near(w, R);
+  uint32_t jb, je;
+  face_range(fg, x, &jb, &je);
+  for (uint32_t j = jb; j < je; ++j) {
+    const size_t f = fg->idx[j];
     if (!orc_face_intersects(wall_vert(w, f, 0), wall_vert(w, f, 1), x, R)) continue;
     double gg[2];
     orc_face_flux(prm->kernel, wall_vert(w, f, 0), wall_vert(w, f, 1), x, h, gg);

@@ -288,3 +288,25 @@ def test_box_walls_tessellation_and_orientation():
     lib = oracle.load()
     assert abs(lib.orc_winding(C.byref(cw.c), oracle._dp(inside)) - 1.0) < 1e-12
     assert abs(lib.orc_winding(C.byref(cw.c), oracle._dp(outside))) < 1e-12
+
+
+def test_face_grid_sums_equal_a_scan_of_every_face():
+    """The oracle's face grid selects the faces that a scan over every face
+    with the exact predicate selects, in the same order: grad gamma equal bits."""
+    rng = np.random.default_rng(3)
+    pts = np.stack([rng.uniform(-0.01, W_BOX + 0.01, 400), rng.uniform(-0.01, H_BOX + 0.01, 400)],
+                   axis=1)
+    pts[:100, 1] = rng.uniform(0.0, 0.03, 100)
+    st, h = _gamma_state(ta.KERNEL_WENDLAND_C4, pts)
+    walls = ta.box_walls(W_BOX, H_BOX, DR)
+    lib = oracle.load()
+    R = 2.0 * h
+    for i, x in enumerate(pts):
+        g = np.zeros(2)
+        for fa, fb in walls.faces:
+            a = np.ascontiguousarray(walls.verts[fa])
+            b = np.ascontiguousarray(walls.verts[fb])
+            xx = np.ascontiguousarray(x)
+            if lib.orc_face_intersects(oracle._dp(a), oracle._dp(b), oracle._dp(xx), R):
+                g = g + oracle.face_flux(ta.KERNEL_WENDLAND_C4, a, b, xx, h)
+        assert np.array_equal(g, st.grad_gamma[i]), (x, g, st.grad_gamma[i])

@@ -294,6 +294,7 @@ void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
     w.faces = c->wall_face_d;
     w.cell_faces_off = c->wall_cell_off;
     w.cell_faces = c->wall_cell_faces;
+    w.cell_inside = c->wall_cell_inside;
     for (int k = 0; k < 2; ++k) {
       w.lo[k] = c->wall_lo[k];
       w.inv_cw[k] = c->wall_inv_cw[k];
@@ -303,8 +304,10 @@ void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
     w.n_cedges = c->wall_n_cedges;
     w.vert_rho = c->wall_rho;
     w.out = c->wall_out;
+    w.near = c->wall_near;
+    w.near_count = c->wall_near_count;
     ProfScope ps(c->prof, c->stream, "wall");
-    tit_amd::launch_wall(c->stream, w);
+    tit_amd::launch_wall(c->stream, w, c->n_cu);
   }
 }
 
@@ -823,10 +826,13 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
     TIT_HIP_TRY(c, dalloc(c, &c->wall_face_d, std::max<size_t>(wh.faces.size(), 1)));
     TIT_HIP_TRY(c, dalloc(c, &c->wall_cell_off, wh.cell_off.size()));
     TIT_HIP_TRY(c, dalloc(c, &c->wall_cell_faces, std::max<size_t>(wh.cell_faces.size(), 1)));
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_cell_inside, wh.cell_inside.size()));
     TIT_HIP_TRY(c, dalloc(c, &c->wall_cedges, std::max<size_t>(wh.cedges.size(), 4)));
     TIT_HIP_TRY(c, dalloc(c, &c->wall_mu, std::max<size_t>(nv, 1)));
     TIT_HIP_TRY(c, dalloc(c, &c->wall_rho, std::max<size_t>(nv, 1)));
     TIT_HIP_TRY(c, dalloc(c, &c->wall_out, 4 * static_cast<size_t>(n)));
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_near, static_cast<size_t>(n)));
+    TIT_HIP_TRY(c, dalloc(c, &c->wall_near_count, 1));
     TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_face_d, wh.faces.data(),
                                   wh.faces.size() * sizeof(tit_amd::WallFace),
                                   hipMemcpyHostToDevice, s));
@@ -834,6 +840,8 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
                                   wh.cell_off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_cell_faces, wh.cell_faces.data(),
                                   wh.cell_faces.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_cell_inside, wh.cell_inside.data(),
+                                  wh.cell_inside.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_cedges, wh.cedges.data(),
                                   wh.cedges.size() * sizeof(double), hipMemcpyHostToDevice, s));
     TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_mu, wall_gamma.data(), nv * sizeof(double),

@@ -72,6 +72,7 @@ struct tit_hip_ctx {
   tit_amd::WallFace* wall_face_d = nullptr;
   uint32_t* wall_cell_off = nullptr;
   uint32_t* wall_cell_faces = nullptr;
+  uint32_t* wall_cell_inside = nullptr;
   double* wall_cedges = nullptr;
   uint32_t wall_n_cedges = 0;
   double wall_lo[2] = {0.0, 0.0}, wall_inv_cw[2] = {0.0, 0.0};
@@ -79,6 +80,8 @@ struct tit_hip_ctx {
   double* wall_mu = nullptr;   // per vertex: gamma at set-up (mass fraction)
   double* wall_rho = nullptr;  // per vertex: density of this stage
   double* wall_out = nullptr;  // per slot: face terms / gamma, 1 / gamma
+  uint32_t* wall_near = nullptr;  // per slot: list of particles near a wall
+  uint32_t* wall_near_count = nullptr;
   uint32_t wall_id0 = 0;       // id of the fixed particle at vertex 0
 
   // chunk schedule of the stage kernel

@@ -142,14 +142,17 @@ struct WallArgs {
   const WallFace* faces;
   const uint32_t* cell_faces_off;  // CSR of ascending face indices per grid cell
   const uint32_t* cell_faces;
+  const uint32_t* cell_inside;     // cells without faces: wholly contained
   double lo[2], inv_cw[2];
   int nc[2];
   const double* cedges;            // containment polygon, 4 per edge
   uint32_t n_cedges;
   const double* vert_rho;          // density of the fixed particle at each vertex
   double* out;
+  uint32_t* near;                  // scratch: slots of the particles near a wall
+  uint32_t* near_count;
 };
-void launch_wall(hipStream_t s, const WallArgs& a);
+void launch_wall(hipStream_t s, const WallArgs& a, int n_cu);
 // post_integrate.hip: the passes in order, the shift into disp[] (flags bit 0
 // when it passes skin_half), the next step size into dt_min.
 void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* disp,

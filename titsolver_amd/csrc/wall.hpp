@@ -191,6 +191,9 @@ struct WallHost {
   std::vector<WallFace> faces;
   std::vector<double> cedges;  // 4 per containment edge
   std::vector<uint32_t> cell_off, cell_faces;
+  // per cell without candidate faces: 1 when the whole cell is contained (no
+  // face crosses it, so the winding number is that of its centre)
+  std::vector<uint32_t> cell_inside;
   double lo[2] = {0.0, 0.0}, inv_cw[2] = {0.0, 0.0};
   int nc[2] = {0, 0};
 };
