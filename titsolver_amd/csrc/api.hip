@@ -255,6 +255,7 @@ void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode) {
   {
     ProfScope ps(c->prof, s, "cell_offsets");
     launch_cell_offsets(s, c->n_cells, c->n, c->key, c->cell_off, rc.ctl);
+    launch_cell_fluid(s, c->n_cells, c->cell_off, c->tag, c->cell_fluid, rc.ctl);
   }
   {
     ProfScope ps(c->prof, s, "build_chunks");
@@ -275,6 +276,7 @@ void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
   a.cell_off = c->cell_off;
   a.fixed_list = c->fixed_list;
   a.normal = c->normal;
+  a.cell_fluid = c->cell_fluid;
   a.n_fixed = c->n_fixed;
   if (c->walls) {
     a.wall_mu = c->wall_mu;
@@ -784,6 +786,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->tag, n));
   TIT_HIP_TRY(c, dalloc(c, &c->tag_alt, n));
   TIT_HIP_TRY(c, dalloc(c, &c->cell_off, static_cast<size_t>(c->n_cells) + 1));
+  TIT_HIP_TRY(c, dalloc(c, &c->cell_fluid, std::max<uint32_t>(c->n_cells, 1)));
   TIT_HIP_TRY(c, dalloc(c, &c->fixed_list, std::max<uint32_t>(c->n_fixed, 1)));
   TIT_HIP_TRY(c, dalloc(c, &c->counter, 4));
   TIT_HIP_TRY(c, dalloc(c, &c->rebuild_ctl, 4));
@@ -884,6 +887,8 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   const uint32_t ctl0[4] = {0u, 1u, 0u, 0u};
   TIT_HIP_TRY(c, hipMemcpyAsync(c->rebuild_ctl, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, s));
   TIT_HIP_TRY(c, hipMemsetAsync(c->tank_need, 0, n_tanks * sizeof(uint32_t), s));
+  // every cell may hold fluid until the first rebuild marks them
+  TIT_HIP_TRY(c, hipMemsetAsync(c->cell_fluid, 1, std::max<uint32_t>(c->n_cells, 1), s));
   TIT_HIP_TRY(c, hipMemsetAsync(c->rebuild_count, 0, sizeof(unsigned long long), s));
   // First step size from the uploaded state and dv/dt (zero by default).
   double* d_dv = nullptr;

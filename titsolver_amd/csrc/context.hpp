@@ -51,6 +51,7 @@ struct tit_hip_ctx {
   SortScratch sort;
   DevTank* tanks = nullptr;
   double* normal = nullptr;
+  uint8_t* cell_fluid = nullptr;  // per cell: holds fluid (rebuilt with the index)
   double *dt = nullptr, *time = nullptr, *disp = nullptr;
   unsigned long long* dt_min = nullptr;
   uint32_t* vmax = nullptr;

@@ -228,6 +228,22 @@ __global__ __launch_bounds__(kBlock) void k_cell_offsets(
   for (int64_t c = prev + 1; c <= cur; ++c) off[c] = i;
 }
 
+// Marks the cells that hold a fluid particle (after the cell offsets).  A
+// fixed particle with no marked cell in its stencil has no fluid neighbour:
+// fluid particles stay within skin/2 of the cell they were sorted into.
+__global__ __launch_bounds__(kBlock) void k_cell_fluid(uint32_t n_cells,
+                                                       const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ tag,
+                                                       uint8_t* __restrict__ fluid,
+                                                       const uint32_t* __restrict__ ctl) {
+  if (!ctl[0]) return;
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= n_cells) return;
+  uint8_t any = 0;
+  for (uint32_t j = off[c]; j < off[c + 1]; ++j) any |= static_cast<uint8_t>(!(tag[j] & 1u));
+  fluid[c] = any;
+}
+
 // ---- boundary extrapolation ---------------------------------------------------
 // FluidEquations::setup_boundary (source/tit/sph/fluid_equations.hpp:122-164):
 // Shepard average of H(rho_b) + g (r_be . n_e) n_e[1] over fluid neighbours,
@@ -314,9 +330,38 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     rb[q] = b;
     rl[q + 1] = rl[q] + (e - b);
   }
+  // No stencil cell holds fluid: no fluid neighbour, S = H = 0 as a full scan
+  // would leave them.
+  int any = 0;
+  {
+    const int nrow = zhi - zlo + 1;
+    for (int t = lane; t < NCOL * nrow; t += G) {
+      const int q = t / nrow, z = zlo + t % nrow;
+      int ix, iy = 0;
+      bool ok;
+      if constexpr (DIM == 2) {
+        ix = c[0] - kStencil + q;
+        ok = ix >= 0 && ix < T.nc[0];
+      } else {
+        ix = c[0] - kStencil + q / (2 * kStencil + 1);
+        iy = c[1] - kStencil + q % (2 * kStencil + 1);
+        ok = ix >= 0 && ix < T.nc[0] && iy >= 0 && iy < T.nc[1];
+      }
+      if (ok) {
+        const uint32_t row = DIM == 2
+                                 ? T.cell_off + static_cast<uint32_t>(ix) * T.nc[1]
+                                 : T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] +
+                                                 static_cast<uint32_t>(iy)) * T.nc[2];
+        any |= A.cell_fluid[row + static_cast<uint32_t>(z)];
+      }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) any |= __shfl_xor(any, off, G);
+  }
+  const uint32_t n_cand = any ? rl[NCOL] : 0u;
   double S = 0.0, H = 0.0;
   const double xi_1 = P.xi - 1.0, inv_xi_1 = 1.0 / xi_1, cs2 = P.cs0 * P.cs0;
-  for (uint32_t t = lane; t < rl[NCOL]; t += G) {
+  for (uint32_t t = lane; t < n_cand; t += G) {
     int q = 0;
     while (t >= rl[q + 1]) ++q;
     const uint32_t j = rb[q] + (t - rl[q]);
@@ -590,6 +635,12 @@ void launch_restore_order(hipStream_t s, int dim, uint32_t n, const uint32_t* pe
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
                          const uint32_t* key, uint32_t* cell_off, const uint32_t* ctl) {
   k_cell_offsets<<<grid_for(n + 1), kBlock, 0, s>>>(n_cells, n, key, cell_off, ctl);
+}
+
+void launch_cell_fluid(hipStream_t s, uint32_t n_cells, const uint32_t* cell_off,
+                       const uint32_t* tag, uint8_t* fluid, const uint32_t* ctl) {
+  if (n_cells == 0) return;
+  k_cell_fluid<<<grid_for(n_cells), kBlock, 0, s>>>(n_cells, cell_off, tag, fluid, ctl);
 }
 
 void launch_boundary(hipStream_t s, const BoundaryArgs& a) {

@@ -84,6 +84,7 @@ struct BoundaryArgs {
   const uint32_t* cell_off;
   const uint32_t* fixed_list;
   const double* normal;  // reference order, dim per particle (fixed only)
+  const uint8_t* cell_fluid;  // per cell: holds a fluid particle (last rebuild)
   uint32_t n_fixed;
   // semi-analytic wall: fixed particle id0 + k sits at wall vertex k; its
   // mass fraction gamma_k (mass m gamma_k) and its density, written here for
@@ -128,6 +129,9 @@ void launch_restore_order(hipStream_t s, int dim, uint32_t n, const uint32_t* pe
                           const uint32_t* ctl);
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
                          const uint32_t* key, uint32_t* cell_off, const uint32_t* ctl);
+// Per cell: 1 when it holds a fluid particle (after launch_cell_offsets).
+void launch_cell_fluid(hipStream_t s, uint32_t n_cells, const uint32_t* cell_off,
+                       const uint32_t* tag, uint8_t* fluid, const uint32_t* ctl);
 void launch_boundary(hipStream_t s, const BoundaryArgs& a);
 
 // Semi-analytic wall (wall.hip): per fluid particle of the stage input state,
