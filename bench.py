@@ -203,6 +203,7 @@ def main():
                      "avg_launch_us": avg_stage_s * 1e6},
         "kernels": kernels,
     }
+    out["search_roofline"] = search_roofline(ctx, arrays)
     ctx.close()
 
     if not args.no_learned and args.workload == "dam2d":
@@ -216,6 +217,38 @@ def main():
 
 
 EAGER = False
+
+
+def search_roofline(ctx, arrays, reps=5):
+    """The neighbour search's index build (GridIndex, grid_search.hpp:73-113):
+    cell keys, the LSD radix sort of (key, slot), the gather of the state into
+    cell order and the cell offsets, timed with HIP events over `reps` full
+    rebuilds (each after a fresh upload, so every particle moves).
+
+    Algorithmic bytes per particle: keys read x (8 dim B) and the tag (4 B) and
+    write the key (4 B); each radix pass reads the key for its histogram (4 B)
+    and reads and writes (key, slot) (8 + 8 B); the gather reads the state
+    (x, v, rho: 16 dim + 8 B) and writes it with the two pair records (24 B);
+    the offsets read the keys (4 B)."""
+    n = sum(a.n for a in arrays)
+    dim = arrays[0].dim
+    ctx.set_profiling(True)
+    for _ in range(reps):
+        ctx.upload(arrays)
+        ctx.prepare()
+    st = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    names = ["cell_keys", "radix_hist", "radix_digit_scan", "radix_scatter", "radix_copy",
+             "permute", "cell_offsets"]
+    t = sum(st[k][1] for k in names if k in st) / reps * 1e-3
+    passes = st["radix_scatter"][0] / reps if "radix_scatter" in st else 0
+    state = 16 * dim + 8
+    per = (8 * dim + 4 + 4) + passes * (4 + 8 + 8) + (state + state + 24) + 4
+    achieved = per * n / t / 1e9
+    return {"bound": "hbm", "kernels": names, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "bytes_per_particle": per,
+            "particles": n, "radix_passes": passes, "rebuild_us": t * 1e6,
+            "per_kernel_us": {k: 1e3 * st[k][1] / reps for k in names if k in st}}
 
 
 def terms_of(eq, dr):

@@ -254,8 +254,8 @@ void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode) {
   }
   {
     ProfScope ps(c->prof, s, "cell_offsets");
-    launch_cell_offsets(s, c->n_cells, c->n, c->key, c->cell_off, rc.ctl);
-    launch_cell_fluid(s, c->n_cells, c->cell_off, c->tag, c->cell_fluid, rc.ctl);
+    launch_cell_offsets(s, c->n_cells, c->n, c->key, c->tag, c->cell_off, c->cell_fluid,
+                        c->cell_gaps, c->cell_n_gaps, rc.ctl);
   }
   {
     ProfScope ps(c->prof, s, "build_chunks");
@@ -787,6 +787,8 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->tag_alt, n));
   TIT_HIP_TRY(c, dalloc(c, &c->cell_off, static_cast<size_t>(c->n_cells) + 1));
   TIT_HIP_TRY(c, dalloc(c, &c->cell_fluid, std::max<uint32_t>(c->n_cells, 1)));
+  TIT_HIP_TRY(c, dalloc(c, &c->cell_gaps, static_cast<size_t>(n_total) + 1));
+  TIT_HIP_TRY(c, dalloc(c, &c->cell_n_gaps, 1));
   TIT_HIP_TRY(c, dalloc(c, &c->fixed_list, std::max<uint32_t>(c->n_fixed, 1)));
   TIT_HIP_TRY(c, dalloc(c, &c->counter, 4));
   TIT_HIP_TRY(c, dalloc(c, &c->rebuild_ctl, 4));

@@ -52,6 +52,8 @@ struct tit_hip_ctx {
   DevTank* tanks = nullptr;
   double* normal = nullptr;
   uint8_t* cell_fluid = nullptr;  // per cell: holds fluid (rebuilt with the index)
+  uint4* cell_gaps = nullptr;     // scratch of the cell offsets: long empty runs
+  uint32_t* cell_n_gaps = nullptr;
   double *dt = nullptr, *time = nullptr, *disp = nullptr;
   unsigned long long* dt_min = nullptr;
   uint32_t* vmax = nullptr;

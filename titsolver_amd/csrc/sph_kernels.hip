@@ -215,33 +215,67 @@ __global__ __launch_bounds__(kBlock) void k_restore_order(
 }
 
 // CSR offsets: off[c] = first sorted position with key >= c.  Thread i
-// (0..n) fills the cells c in (key[i-1], key[i]] with i (key[-1] = -1,
-// key[n] = n_cells), so every cell is written exactly once.
+// (0..n) owns the cells c in (key[i-1], key[i]] (key[-1] = -1, key[n] =
+// n_cells), so every cell is written exactly once: off[c] = i, and the cells
+// before key[i] are marked as holding no fluid.  A run of up to kGapInline
+// cells is filled by the thread; a longer run (empty space above or beside
+// the water) is listed and filled by a whole wave in k_cell_gaps.  The first
+// particle of cell key[i] marks whether the cell holds a fluid particle.  A
+// fixed particle with no marked cell in its stencil has no fluid neighbour
+// (fluid particles stay within skin/2 of their cell).
+constexpr int64_t kGapInline = 32;
+
+__global__ void k_cell_gaps_clear(uint32_t* n_gaps, const uint32_t* __restrict__ ctl) {
+  if (ctl[0]) n_gaps[0] = 0u;
+}
+
 __global__ __launch_bounds__(kBlock) void k_cell_offsets(
     uint32_t n_cells, uint32_t n, const uint32_t* __restrict__ key,
-    uint32_t* __restrict__ off, const uint32_t* __restrict__ ctl) {
+    const uint32_t* __restrict__ tag, uint32_t* __restrict__ off,
+    uint8_t* __restrict__ fluid, uint4* __restrict__ gaps, uint32_t* __restrict__ n_gaps,
+    const uint32_t* __restrict__ ctl) {
   if (!ctl[0]) return;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i > n) return;
   const int64_t prev = i > 0 ? static_cast<int64_t>(key[i - 1]) : -1;
   const int64_t cur = i < n ? static_cast<int64_t>(key[i]) : static_cast<int64_t>(n_cells);
-  for (int64_t c = prev + 1; c <= cur; ++c) off[c] = i;
+  if (cur - prev <= kGapInline) {
+    for (int64_t c = prev + 1; c <= cur; ++c) {
+      off[c] = i;
+      if (c < cur) fluid[c] = 0;
+    }
+  } else {
+    off[cur] = i;
+    const uint32_t g = atomicAdd(n_gaps, 1u);
+    gaps[g] = make_uint4(static_cast<uint32_t>(prev + 1), static_cast<uint32_t>(cur), i, 0u);
+  }
+  if (i < n && prev < cur) {
+    uint8_t any = 0;
+    for (uint32_t j = i; j < n && key[j] == static_cast<uint32_t>(cur); ++j) {
+      any |= static_cast<uint8_t>(!(tag[j] & 1u));
+    }
+    fluid[cur] = any;
+  }
 }
 
-// Marks the cells that hold a fluid particle (after the cell offsets).  A
-// fixed particle with no marked cell in its stencil has no fluid neighbour:
-// fluid particles stay within skin/2 of the cell they were sorted into.
-__global__ __launch_bounds__(kBlock) void k_cell_fluid(uint32_t n_cells,
-                                                       const uint32_t* __restrict__ off,
-                                                       const uint32_t* __restrict__ tag,
-                                                       uint8_t* __restrict__ fluid,
-                                                       const uint32_t* __restrict__ ctl) {
+// The listed runs [first, end) of empty cells: off = the owning thread's
+// index, no fluid.  Persistent waves over the list.
+__global__ __launch_bounds__(kBlock) void k_cell_gaps(uint32_t* __restrict__ off,
+                                                      uint8_t* __restrict__ fluid,
+                                                      const uint4* __restrict__ gaps,
+                                                      const uint32_t* __restrict__ n_gaps,
+                                                      const uint32_t* __restrict__ ctl) {
   if (!ctl[0]) return;
-  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= n_cells) return;
-  uint8_t any = 0;
-  for (uint32_t j = off[c]; j < off[c + 1]; ++j) any |= static_cast<uint8_t>(!(tag[j] & 1u));
-  fluid[c] = any;
+  const uint32_t ng = n_gaps[0];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t g = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); g < ng; g += waves) {
+    const uint4 r = gaps[g];
+    for (uint32_t c = r.x + lane; c < r.y; c += 64) {
+      off[c] = r.z;
+      fluid[c] = 0;
+    }
+  }
 }
 
 // ---- boundary extrapolation ---------------------------------------------------
@@ -633,14 +667,12 @@ void launch_restore_order(hipStream_t s, int dim, uint32_t n, const uint32_t* pe
 }
 
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
-                         const uint32_t* key, uint32_t* cell_off, const uint32_t* ctl) {
-  k_cell_offsets<<<grid_for(n + 1), kBlock, 0, s>>>(n_cells, n, key, cell_off, ctl);
-}
-
-void launch_cell_fluid(hipStream_t s, uint32_t n_cells, const uint32_t* cell_off,
-                       const uint32_t* tag, uint8_t* fluid, const uint32_t* ctl) {
-  if (n_cells == 0) return;
-  k_cell_fluid<<<grid_for(n_cells), kBlock, 0, s>>>(n_cells, cell_off, tag, fluid, ctl);
+                         const uint32_t* key, const uint32_t* tag, uint32_t* cell_off,
+                         uint8_t* fluid, uint4* gaps, uint32_t* n_gaps, const uint32_t* ctl) {
+  k_cell_gaps_clear<<<1, 1, 0, s>>>(n_gaps, ctl);
+  k_cell_offsets<<<grid_for(n + 1), kBlock, 0, s>>>(n_cells, n, key, tag, cell_off, fluid,
+                                                     gaps, n_gaps, ctl);
+  k_cell_gaps<<<1024, kBlock, 0, s>>>(cell_off, fluid, gaps, n_gaps, ctl);
 }
 
 void launch_boundary(hipStream_t s, const BoundaryArgs& a) {

@@ -127,11 +127,11 @@ void launch_restore_order(hipStream_t s, int dim, uint32_t n, const uint32_t* pe
                           DevState src, DevState dst, const uint32_t* id_src,
                           uint32_t* id_dst, const uint32_t* tag_src, uint32_t* tag_dst,
                           const uint32_t* ctl);
+// Cell offsets, and per cell whether it holds a fluid particle; gaps
+// (n + 1 entries) and n_gaps are scratch for the long runs of empty cells.
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
-                         const uint32_t* key, uint32_t* cell_off, const uint32_t* ctl);
-// Per cell: 1 when it holds a fluid particle (after launch_cell_offsets).
-void launch_cell_fluid(hipStream_t s, uint32_t n_cells, const uint32_t* cell_off,
-                       const uint32_t* tag, uint8_t* fluid, const uint32_t* ctl);
+                         const uint32_t* key, const uint32_t* tag, uint32_t* cell_off,
+                         uint8_t* fluid, uint4* gaps, uint32_t* n_gaps, const uint32_t* ctl);
 void launch_boundary(hipStream_t s, const BoundaryArgs& a);
 
 // Semi-analytic wall (wall.hip): per fluid particle of the stage input state,
