@@ -271,6 +271,35 @@ def test_stage_kernel_sums_exactly_the_neighbour_sets(dim, dr, schedule):
     ctx.close()
 
 
+@pytest.mark.parametrize("dim,n,radius", [(2, 3000, 0.06), (3, 3000, 0.15)])
+def test_escaped_particles_keep_exact_sets(dim, n, radius):
+    """Particles outside the tank box (here 3/8 of the cloud per axis) are
+    clamped into its border cells: the index's neighbour sets and the stage
+    kernel's accepted pairs stay the brute-force sets."""
+    rng = np.random.default_rng(17)
+    r = rng.uniform(-0.3, 1.3, size=(n, dim))
+    pa = ta.ParticleArray(dim, r, np.full(n, 1000.0), n, np.zeros((0, dim)),
+                          tuple([0.0] * dim), tuple([1.0] * dim))
+    eq = _eq(dim, radius / 2.0)
+    eq.h = radius / 2.0
+    ctx = _ctx(eq, [pa])
+    want = oracle.naive_search(pa.r, radius)
+    assert csr_equal(ctx.neighbors(), want)
+    counts = ctx.pair_counts().astype(np.int64)
+    assert np.array_equal(counts, np.diff(want[0].astype(np.int64)) - 1)
+    ctx.close()
+
+
+def test_empty_upload_rejected():
+    eq = _eq(2, 0.02)
+    ctx = ta.HipContext(eq)
+    empty = ta.ParticleArray(2, np.zeros((0, 2)), np.zeros(0), 0, np.zeros((0, 2)),
+                             (0.0, 0.0), (1.0, 1.0))
+    with pytest.raises(ta.TitError, match="no particles"):
+        ctx.upload([empty])
+    ctx.close()
+
+
 def test_cell_index_rebuilds_only_when_needed():
     """The cell index is rebuilt on the first step and then only when a tank's
     particles could leave the skin: far fewer rebuilds than steps, and the

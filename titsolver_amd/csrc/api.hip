@@ -642,6 +642,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
     n_total += v.n_fluid + v.n_fixed;
     n_fixed_total += v.n_fixed;
   }
+  if (n_total == 0) return fail(c, TIT_E_ARG, "no particles");
   if (n_total >= (1ull << 31)) return fail(c, TIT_E_ARG, "too many particles");
   const double csz = cell_size(c);
   std::vector<DevTank> ht(n_tanks);
