@@ -139,8 +139,9 @@ struct FluidEquations {
   double c_force = 0.25;
   double c_visc = 0.125;
   /// End every step with post_integrate (shifting and free-surface
-  /// correction, fluid_equations.hpp:313-512; wall-free form only).
-  bool post_integrate = false;
+  /// correction, fluid_equations.hpp:313-512), as the reference always does
+  /// (time_integrator.hpp:181); false leaves it out.
+  bool post_integrate = true;
 };
 
 /// ParticleArray (particle_array.hpp:138-293) in the reference layout:

@@ -155,8 +155,9 @@ tit_status tit_hip_tank_clock(tit_hip_ctx* ctx, double* time, double* dt_next);
 tit_status tit_hip_prepare(tit_hip_ctx* ctx);
 
 /* FluidEquations::post_integrate (fluid_equations.hpp:313-321) on the
- * current state, in the wall-free form: prepare, apply_shifts (:327-471)
- * and apply_free_surface_correction (:480-512).  Requires a context created
+ * current state: prepare, apply_shifts (:327-471) and
+ * apply_free_surface_correction (:480-512), with the wall's face terms and
+ * gamma when tit_hip_set_walls was called.  Requires a context created
  * with params.post_integrate = 1.  phi (may be NULL) receives the
  * free-surface indicator of every particle in the reference order (DBL_MIN
  * on the surface, 1 far from it).  The next step size is taken again from

@@ -1061,8 +1061,8 @@ static int prepare(const orc_params* prm, orc_state* s, step_ws* w) {
 }
 
 /* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
-/* Post-integration (fluid_equations.hpp:313-512), wall-free form: gamma = 1,
- * so the face loops are empty and the velocity correction always applies. */
+/* Post-integration (fluid_equations.hpp:313-512).  Without walls gamma = 1,
+ * the face loops are empty and the velocity correction always applies. */
 
 /* FluidEquations constants (fluid_equations.hpp:518-524). */
 static const double kShiftCfl = 0.4;  /* CFL_ */
@@ -1129,12 +1129,46 @@ int orc_post_integrate(const orc_params* prm, orc_state* s, const uint64_t* off,
   const double* v = s->v;
   const double* rho = s->rho;
 
-  /* apply_shifts, pair sums over unique pairs b < a (:340-353) */
+  /* apply_shifts, face terms (:342-349): the face's fixed particles are the
+   * ones at its vertices, their fields averaged (11 planted: dropped) */
+  if (prm->walls && g_mutation != 11) {
+    const orc_walls* w = prm->walls;
+    const face_grid* fg = faces_near(w, R);
+    for (size_t a = 0; a < nf; ++a) {
+      const double* x = r + 2 * a;
+      const double ga = s->gamma[a];
+      uint32_t jb, je;
+      face_range(fg, x, &jb, &je);
+      for (uint32_t j = jb; j < je; ++j) {
+        const size_t f = fg->idx[j];
+        if (!orc_face_intersects(wall_vert(w, f, 0), wall_vert(w, f, 1), x, R)) continue;
+        double gg[2];
+        orc_face_flux(prm->kernel, wall_vert(w, f, 0), wall_vert(w, f, 1), x, prm->h, gg);
+        const size_t s0 = nf + w->faces[2 * f], s1 = nf + w->faces[2 * f + 1];
+        double r_sa[2], v_sa[2];
+        for (int k = 0; k < 2; ++k) {
+          r_sa[k] = (r[2 * s0 + k] + r[2 * s1 + k]) / 2.0 - r[2 * a + k];
+          v_sa[k] = (v[2 * s0 + k] + v[2 * s1 + k]) / 2.0 - v[2 * a + k];
+        }
+        const double rho_sa = (rho[s0] + rho[s1]) / 2.0 - rho[a];
+        for (int i = 0; i < 2; ++i) {
+          N[a * 2 + i] -= gg[i] / ga;
+          for (int q = 0; q < 2; ++q) {
+            L[a * 4 + i * 2 + q] -= r_sa[i] * gg[q] / ga;
+            gv[a * 4 + i * 2 + q] -= v_sa[i] * gg[q] / ga;
+          }
+          gr[a * 2 + i] -= rho_sa * gg[i] / ga;
+        }
+      }
+    }
+  }
+  /* apply_shifts, pair sums over unique pairs b < a (:350-365); V / gamma */
   for (size_t a = 0; a < n; ++a) {
     for (uint64_t j = off[a]; j < off[a + 1]; ++j) {
       const size_t b = idx[j];
       if (b >= a) continue;
-      const double V_a = prm->m / rho[a], V_b = prm->m / rho[b];
+      const double V_a = mass_of(prm, s, a) / rho[a] / gamma_of(prm, s, b);
+      const double V_b = mass_of(prm, s, b) / rho[b] / gamma_of(prm, s, a);
       double x_ab[3], gW[3];
       for (int k = 0; k < dim; ++k) x_ab[k] = r[a * dim + k] - r[b * dim + k];
       orc_kernel_grad(prm->kernel, dim, x_ab, prm->h, gW);
@@ -1261,9 +1295,11 @@ int orc_post_integrate(const orc_params* prm, orc_state* s, const uint64_t* off,
       for (int q = 0; q < dim; ++q) x += gv[a * dd + i * dim + q] * da[q];
       dv[i] = x;
     }
+    /* the velocity correction only where gamma ~ 1 (:465-468) */
+    const int correct_v = is_tiny(gamma_of(prm, s, a) - 1.0);
     for (int i = 0; i < dim; ++i) {
       s->r[a * dim + i] += da[i];
-      s->v[a * dim + i] += dv[i];
+      if (correct_v) s->v[a * dim + i] += dv[i];
     }
     s->rho[a] += dot(dim, gr + a * dim, da);
   }
@@ -1281,14 +1317,15 @@ int orc_post_integrate(const orc_params* prm, orc_state* s, const uint64_t* off,
       double x_ab[3];
       for (int k = 0; k < dim; ++k) x_ab[k] = s->r[a * dim + k] - s->r[b * dim + k];
       const double W_ab = orc_kernel_value(prm->kernel, dim, x_ab, prm->h);
-      alpha += prm->m / rho_raw[b] * W_ab;
-      rho_t += prm->m * W_ab;
+      alpha += mass_of(prm, s, b) / rho_raw[b] * W_ab;
+      rho_t += mass_of(prm, s, b) * W_ab;
     }
-    const double ratio = fmin(1.0, alpha / 1.0);
+    const double ga = gamma_of(prm, s, a);
+    const double ratio = fmin(1.0, alpha / ga);
     if (ratio > 0.99) continue;
     const double e = g_mutation == 8 ? K * pow2(ratio - 1.0) : -K * pow2(ratio - 1.0);
     const double beta = exp(e);
-    const double corr = beta * 1.0 + (1.0 - beta) * alpha;
+    const double corr = beta * ga + (1.0 - beta) * alpha;
     if (!is_tiny(corr)) s->rho[a] = rho_t / corr;
   }
   if (out) {
@@ -1325,10 +1362,7 @@ static int substep(const orc_params* prm, orc_state* s, step_ws* w,
 int orc_ssprk_step(const orc_params* prm, orc_state* s, int order,
                    double* dt_out) {
   if (order != 2 && order != 3) return -1;
-  /* the post-integration passes are restated in the wall-free form only */
-  if (prm->walls && (prm->post_integrate || prm->dim != 2 || prm->flavour != ORC_FLAVOUR_REF)) {
-    return -1;
-  }
+  if (prm->walls && (prm->dim != 2 || prm->flavour != ORC_FLAVOUR_REF)) return -1;
   const int dim = prm->dim;
   const size_t n = s->n_fluid + s->n_fixed, nf = s->n_fluid;
   step_ws* w = &g_ws;

@@ -159,9 +159,9 @@ typedef struct orc_shift_out {
 } orc_shift_out;
 
 /* FluidEquations::apply_shifts + apply_free_surface_correction
- * (fluid_equations.hpp:327-512) in the wall-free form (gamma = 1), on the
- * adjacency of the current positions (as post_integrate's prepare builds
- * it).  Updates r, v, rho of the fluid particles.  Returns 0 on success. */
+ * (fluid_equations.hpp:327-512), on the adjacency of the current positions
+ * (as post_integrate's prepare builds it); with walls, the face terms and
+ * s->gamma (orc_compute_gamma first).  Updates r, v, rho of the fluid particles.  Returns 0 on success. */
 int orc_post_integrate(const orc_params* prm, orc_state* s, const uint64_t* off,
                        const uint32_t* idx, orc_shift_out* out);
 
@@ -185,7 +185,8 @@ int orc_ssprk_generic(int order, double dt, size_t n, double* u,
  * 6: free-surface visibility cone reversed, 7: shifts applied to every
  * fluid particle, 8: free-surface blend exponent with the wrong sign,
  * 9: wall face normals reversed in the kernel flux, 10: wall face terms of
- * the momentum equation dropped); 0
+ * the momentum equation dropped, 11: wall face terms of apply_shifts
+ * dropped); 0
  * restores the faithful oracle. */
 void orc_set_mutation(int m);
 

@@ -7,9 +7,10 @@
 //       host-only checks of the mirror (tessellation, argument errors)
 //   dam_break_mirror walls|layers <dr> <steps> <out.bin>
 //       walls:  the closed pool with the semi-analytic wall, reference terms,
-//               Wendland C4, h = 2 dr (titwcsph's own set-up);
+//               Wendland C4, h = 2 dr, every step ending with post_integrate
+//               (titwcsph's own set-up and step);
 //       layers: the open tank with three layers of wall particles, benchmark
-//               terms, Wendland C2, h = 2 dr.
+//               terms, Wendland C2, h = 2 dr, no post-integration.
 //       out.bin: int64 dim, n, n_fluid, n_fixed, steps; then float64 h, m,
 //                initial r, v, rho, normals (n_fixed x dim), final r, v, rho,
 //                and the step sizes.
@@ -188,6 +189,7 @@ int layers_case(double dr, int steps, const char* out) {
   FluidEquations<QuarticWendlandKernel> equations{kG, 0.0, std::nullopt, std::nullopt,
                                                   TaitEquationOfState{cs_0, rho_0}};
   equations.terms = Terms::benchmark;
+  equations.post_integrate = false;
   equations.alpha = 0.02;
   equations.delta = 0.1;
   equations.cfl = 0.25;

@@ -73,6 +73,7 @@ def test_mirror_dam_break_vs_oracle(case, tmp_path):
         # the C++ tessellation put the wall particles where the Python one does
         assert np.array_equal(walls.verts, d["r0"][nf:])
         prm = oracle.params(eq, walls)
+        prm.post_integrate = 1  # the mirror's default, as the reference's step
     else:
         eq = ta.cfg_equations(2, dr, 0.5)
         prm = oracle.params(eq)

@@ -1,11 +1,12 @@
 """HIP path with the semi-analytic wall (2D, reference terms) against the FP64
 oracle, device 0.
 
-Tolerances: gamma, grad gamma, the face terms and the boundary
-extrapolation run in FP64 on both sides (fixed-particle densities agree to
-1e-12); the pair sums run in FP32 with Kahan accumulation as in the
+Tolerances: gamma, grad gamma, the face terms, the boundary extrapolation
+and post-integration run in FP64 on both sides (fixed-particle densities and
+the post-integrated state agree to 1e-12, the free-surface classification
+exactly); the pair sums run in FP32 with Kahan accumulation as in the
 wall-free tests, so the right-hand sides are compared at 2e-5 of the field's
-largest value, and a 20-step run at the bars of the wall-free step test.
+largest value, and 20-step runs at the bars of the wall-free step test.
 """
 from __future__ import annotations
 
@@ -111,14 +112,64 @@ def test_walls_graph_equals_eager():
         assert np.array_equal(res[0][k], res[1][k]), k
 
 
+def test_post_integrate_with_walls_matches_oracle():
+    """FluidEquations::post_integrate with the semi-analytic wall: the
+    free-surface classification identical to the oracle's, r, v, rho to
+    1e-12 (FP64 on both sides)."""
+    dr = 0.02
+    pa, walls, eq = _case(dr, ta.KERNEL_WENDLAND_C4, seed=4)
+    ctx = ta.HipContext(eq, post_integrate=True, walls=walls)
+    ctx.upload([pa])
+    phi = ctx.post_integrate()
+    got = ctx.download()
+    prm, st = _oracle(pa, walls, eq)
+    off, idx = oracle.adjacency(st, prm)
+    oracle.compute_gamma(prm, st)
+    oracle.setup_boundary(prm, st, off, idx)
+    ref = oracle.post_integrate(prm, st, off, idx)
+    nf = pa.n_fluid
+    tiny = np.finfo(np.float64).tiny
+    near = np.abs(st.gamma[:nf] - 1.0) > 1e-12
+    shifted = np.any(ref["dr"][:nf] != 0.0, axis=1)
+    print(f"{near.sum()} near a wall, {(ref['phi'][:nf] == tiny).sum()} on the surface, "
+          f"{shifted.sum()} shifted, {(shifted & near).sum()} shifted near a wall")
+    assert (shifted & near).sum() > 10
+    assert np.array_equal(phi[:nf] == tiny, ref["phi"][:nf] == tiny)
+    assert np.array_equal(phi[:nf] == 1.0, ref["phi"][:nf] == 1.0)
+    for k, want in (("r", st.r), ("v", st.v)):
+        err = np.abs(got[k][:nf] - want[:nf]).max() / max(np.abs(want[:nf]).max(), 1e-30)
+        assert err < 1e-12, (k, err)
+    assert np.abs(got["rho"][:nf] / st.rho[:nf] - 1.0).max() < 1e-12
+
+
+def test_steps_with_walls_and_post_integrate_vs_oracle():
+    """20 SSPRK3 steps, each ending with post_integrate, in the closed pool:
+    the reference's full step on titwcsph's set-up."""
+    dr = 0.02
+    pa, walls, eq = _case(dr, ta.KERNEL_WENDLAND_C4)
+    ctx = ta.HipContext(eq, post_integrate=True, walls=walls)
+    ctx.upload([pa])
+    prm, st = _oracle(pa, walls, eq)
+    prm.post_integrate = 1
+    nf = pa.n_fluid
+    for s in range(20):
+        dt_g = ctx.step(1)
+        dt_o = oracle.ssprk_step(prm, st, 3)
+        assert abs(dt_g - dt_o) <= 1e-7 * dt_o, (s, dt_g, dt_o)
+    out = ctx.download()
+    rho_err = np.max(np.abs(out["rho"][:nf] - st.rho[:nf]) / st.rho[:nf])
+    r_err = np.max(np.abs(out["r"][:nf] - st.r[:nf])) / dr
+    print(f"20 steps with walls and post_integrate: rho rel {rho_err:.2e}, r/dr {r_err:.2e}")
+    assert rho_err < 1e-6
+    assert r_err < 1e-5
+
+
 def test_wall_arguments_rejected():
     dr = 0.05
     pa, walls, eq = _case(dr, ta.KERNEL_WENDLAND_C4)
     cfg = ta.cfg_equations(2, dr, 0.5)
     with pytest.raises(ta.TitError, match="reference flavour"):
         ta.HipContext(cfg, walls=walls)
-    with pytest.raises(ta.TitError, match="post_integrate"):
-        ta.HipContext(eq, post_integrate=True, walls=walls)
     ctx = ta.HipContext(eq, walls=walls)
     moved = pa.copy()
     moved.r[pa.n_fluid] += 1e-3

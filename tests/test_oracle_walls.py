@@ -310,3 +310,79 @@ def test_face_grid_sums_equal_a_scan_of_every_face():
             if lib.orc_face_intersects(oracle._dp(a), oracle._dp(b), oracle._dp(xx), R):
                 g = g + oracle.face_flux(ta.KERNEL_WENDLAND_C4, a, b, xx, h)
         assert np.array_equal(g, st.grad_gamma[i]), (x, g, st.grad_gamma[i])
+
+
+def _linear_fields_with_walls():
+    """Every particle, the wall particles included, carries v = A r + c and
+    rho = 1000 + g . r: the renormalised gradients of apply_shifts, face
+    terms included, reproduce A and g wherever L is invertible."""
+    dr = DR
+    walls = ta.box_walls(W_BOX, H_BOX, dr)
+    nx, ny = round(W_BOX / dr) - 1, round(H_BOX / dr) - 1
+    g = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    rng = np.random.default_rng(8)
+    fl = np.stack([dr * (g[0].ravel() + 1.0), dr * (g[1].ravel() + 1.0)], axis=1)
+    fl += rng.uniform(-0.1, 0.1, size=fl.shape) * dr
+    r = np.concatenate([fl, walls.verts])
+    A = np.array([[0.3, -1.2], [0.7, 0.25]])
+    c = np.array([0.1, -0.4])
+    gvec = np.array([35.0, -80.0])
+    v = r @ A.T + c
+    rho = 1000.0 + r @ gvec
+    eq = ta.cfg_equations(2, dr, 0.6, kernel=ta.KERNEL_WENDLAND_C4, flavour=ta.FLAVOUR_REF)
+    eq.h = 2.0 * dr
+    prm = oracle.params(eq, walls)
+    st = oracle.State(r, rho, len(fl), np.zeros((0, 2)), v)
+    oracle.initialize(prm, st)
+    off, idx = oracle.adjacency(st, prm)
+    oracle.compute_gamma(prm, st)
+    near = np.abs(st.gamma[: len(fl)] - 1.0) > 1e-9
+    out = oracle.post_integrate(prm, st, off, idx)
+    nf = len(fl)
+    err_v = np.abs(out["grad_v"][:nf] - A).max(axis=(1, 2)) / np.abs(A).max()
+    err_r = np.abs(out["grad_rho"][:nf] - gvec).max(axis=1) / np.abs(gvec).max()
+    return near, err_v, err_r
+
+
+def test_apply_shifts_with_walls_exact_for_linear_fields():
+    near, err_v, err_r = _linear_fields_with_walls()
+    assert near.sum() > 100
+    assert err_v.max() < 1e-9 and err_r.max() < 1e-9, (err_v.max(), err_r.max())
+
+
+def _shift_ratio():
+    """Mean shift of the wall-adjacent (gamma < 1) particles over the mean shift
+    of the others, on a box filled with a jittered lattice.  The face terms of
+    apply_shifts complete the truncated kernel sums at the walls, so both are
+    at the level of the jitter; without them the kernel deficit pushes every
+    wall-adjacent particle."""
+    walls = ta.box_walls(W_BOX, H_BOX, DR)
+    nx, ny = round(W_BOX / DR) - 1, round(H_BOX / DR) - 1
+    g = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    rng = np.random.default_rng(5)
+    fl = np.stack([DR * (g[0].ravel() + 1.0), DR * (g[1].ravel() + 1.0)], axis=1)
+    fl += rng.uniform(-0.05, 0.05, size=fl.shape) * DR
+    r = np.concatenate([fl, walls.verts])
+    eq = ta.cfg_equations(2, DR, 0.6, kernel=ta.KERNEL_WENDLAND_C4, flavour=ta.FLAVOUR_REF)
+    eq.h = 2.0 * DR
+    prm = oracle.params(eq, walls)
+    st = oracle.State(r, np.full(len(r), eq.rho0), len(fl), np.zeros((0, 2)))
+    oracle.initialize(prm, st)
+    off, idx = oracle.adjacency(st, prm)
+    oracle.compute_gamma(prm, st)
+    nf = len(fl)
+    near = st.gamma[:nf] < 1.0 - 1e-9
+    out = oracle.post_integrate(prm, st, off, idx)
+    d = np.linalg.norm(out["dr"][:nf], axis=1)
+    far = out["phi"][:nf] == 1.0
+    assert (far & near).sum() > 100 and (far & ~near).sum() > 100
+    return d[far & near].mean() / d[far & ~near].mean()
+
+
+def test_shifts_at_the_walls_stay_at_the_jitter_level():
+    assert _shift_ratio() < 2.0
+
+
+def test_planted_dropped_shift_face_terms_fail():
+    oracle.set_mutation(11)
+    assert _shift_ratio() > 5.0

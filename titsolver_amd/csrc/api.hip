@@ -265,7 +265,7 @@ void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode) {
   launch_reset_disp(s, c->n_tanks, c->disp, rc);
 }
 
-void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
+void enqueue_boundary(tit_hip_ctx* c, const DevState& st, bool for_post_integrate) {
   BoundaryArgs a{};
   a.P = c->dp;
   a.tanks = c->tanks;
@@ -308,6 +308,10 @@ void enqueue_boundary(tit_hip_ctx* c, const DevState& st) {
     w.out = c->wall_out;
     w.near = c->wall_near;
     w.near_count = c->wall_near_count;
+    if (for_post_integrate) {
+      w.near_idx = c->wall_near_idx;
+      w.pi_face = c->wall_pi_face;
+    }
     ProfScope ps(c->prof, c->stream, "wall");
     tit_amd::launch_wall(c->stream, w, c->n_cu);
   }
@@ -430,7 +434,7 @@ void enqueue_step(tit_hip_ctx* c) {
   launch_step_end(s, c->n_tanks, c->dt, c->vmax, c->time, c->disp,
                   c->dp.skin_half, c->counter + 1);
   if (c->prm.post_integrate) {
-    enqueue_boundary(c, c->A);  // prepare (fluid_equations.hpp:99-105)
+    enqueue_boundary(c, c->A, true);  // prepare (fluid_equations.hpp:99-105)
     enqueue_post_integrate(c);
   }
 }
@@ -467,6 +471,13 @@ void enqueue_post_integrate(tit_hip_ctx* c) {
   for (int k = 0; k < dim; ++k) a.dv_last[k] = c->rhs_dv[k];
   a.dt_min = c->dt_min;
   a.vmax = c->vmax;
+  if (c->walls) {
+    a.wall = c->wall_out;
+    a.wall_mu = c->wall_mu;
+    a.wall_id0 = c->wall_id0;
+    a.near_idx = c->wall_near_idx;
+    a.pi_face = c->wall_pi_face;
+  }
   ProfScope ps(c->prof, c->stream, "post_integrate");
   launch_post_integrate(c->stream, a, c->n_tanks, c->disp, c->dp.skin_half, c->counter + 1,
                         c->n_cu);
@@ -572,9 +583,6 @@ tit_status tit_hip_set_walls(tit_hip_ctx* c, const tit_hip_walls* w) {
   if (c->prm.dim != 2) return fail(c, TIT_E_ARG, "walls are 2D only");
   if (c->prm.flavour != TIT_FLAVOUR_REF) {
     return fail(c, TIT_E_ARG, "walls need the reference flavour");
-  }
-  if (c->prm.post_integrate) {
-    return fail(c, TIT_E_ARG, "post_integrate is restated in the wall-free form only");
   }
   if (!w->verts || w->n_verts < 2 || !w->faces || w->n_faces == 0 || !w->cverts ||
       w->n_cverts < 3 || !w->cfaces || w->n_cfaces < 3) {
@@ -838,6 +846,10 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
     TIT_HIP_TRY(c, dalloc(c, &c->wall_rho, std::max<size_t>(nv, 1)));
     TIT_HIP_TRY(c, dalloc(c, &c->wall_out, 4 * static_cast<size_t>(n)));
     TIT_HIP_TRY(c, dalloc(c, &c->wall_near, static_cast<size_t>(n)));
+    if (c->prm.post_integrate) {
+      TIT_HIP_TRY(c, dalloc(c, &c->wall_near_idx, static_cast<size_t>(n)));
+      TIT_HIP_TRY(c, dalloc(c, &c->wall_pi_face, 12 * static_cast<size_t>(n)));
+    }
     TIT_HIP_TRY(c, dalloc(c, &c->wall_near_count, 1));
     TIT_HIP_TRY(c, hipMemcpyAsync(c->wall_face_d, wh.faces.data(),
                                   wh.faces.size() * sizeof(tit_amd::WallFace),
@@ -997,7 +1009,7 @@ tit_status tit_hip_post_integrate(tit_hip_ctx* c, double* phi) {
   }
   TIT_HIP_TRY(c, hipSetDevice(c->device));
   enqueue_rebuild(c, kRebuildPending);
-  enqueue_boundary(c, c->A);
+  enqueue_boundary(c, c->A, true);
   enqueue_post_integrate(c);
   const size_t n = c->n, dim = static_cast<size_t>(c->prm.dim);
   if (phi) {  // phi_near follows N, dr0, grad v, grad rho, phi in pi_buf

@@ -82,9 +82,11 @@ struct tit_hip_ctx {
   int wall_nc[2] = {0, 0};
   double* wall_mu = nullptr;   // per vertex: gamma at set-up (mass fraction)
   double* wall_rho = nullptr;  // per vertex: density of this stage
-  double* wall_out = nullptr;  // per slot: face terms / gamma, 1 / gamma
+  double* wall_out = nullptr;  // per slot: face terms / gamma, gamma
   uint32_t* wall_near = nullptr;  // per slot: list of particles near a wall
   uint32_t* wall_near_count = nullptr;
+  uint32_t* wall_near_idx = nullptr;  // post-integration: per slot, near-list index
+  double* wall_pi_face = nullptr;     // post-integration: 12 face terms per near particle
   uint32_t wall_id0 = 0;       // id of the fixed particle at vertex 0
 
   // chunk schedule of the stage kernel
@@ -142,7 +144,9 @@ enum RebuildMode {
   kRebuildForce     // every tank
 };
 void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode);
-void enqueue_boundary(tit_hip_ctx* c, const DevState& st);
+// setup_boundary (and with walls compute_gamma and the face terms; before
+// post-integration also apply_shifts' face terms).
+void enqueue_boundary(tit_hip_ctx* c, const DevState& st, bool for_post_integrate = false);
 void enqueue_step(tit_hip_ctx* c);
 void enqueue_post_integrate(tit_hip_ctx* c);
 tit_status check_launch(tit_hip_ctx* c, const char* what);

@@ -1,7 +1,11 @@
-// Post-integration step of FluidEquations (fluid_equations.hpp:313-512) in
-// the wall-free form of this path (gamma = 1, no face terms): particle
-// shifting with renormalised gradients (apply_shifts, :327-471) and the
-// free-surface density correction (apply_free_surface_correction, :480-512).
+// Post-integration step of FluidEquations (fluid_equations.hpp:313-512):
+// particle shifting with renormalised gradients (apply_shifts, :327-471) and
+// the free-surface density correction (apply_free_surface_correction,
+// :480-512).  Without walls gamma = 1 and there are no face terms; with the
+// semi-analytic wall the sums start from the face terms (computed by the wall
+// kernel, 16 lanes per particle), the pair terms take per-particle masses
+// over gamma_a, the velocity correction applies where gamma = 1, and the
+// free-surface correction compares the Shepard sum with gamma.
 //
 // Everything runs in FP64, as the reference does, one particle per thread;
 // the neighbour sets are those of the post-step positions (the exact
@@ -210,9 +214,21 @@ __device__ __forceinline__ bool lu_inverse(const double* M, double* inv) {
   return true;
 }
 
+// Mass of particle j: m, or m gamma for a fixed particle with walls
+// (FluidEquations::initialize, fluid_equations.hpp:87-88).
+__device__ __forceinline__ double mass_of(const PiArgs& A, uint32_t j) {
+  if (A.wall_mu && (A.tag[j] & 1u)) return A.P.m * A.wall_mu[A.id[j] - A.wall_id0];
+  return A.P.m;
+}
+
+// gamma of fluid particle i (1 without walls; fixed particles' sums are not used).
+__device__ __forceinline__ double gamma_of(const PiArgs& A, uint32_t i) {
+  return A.wall && !(A.tag[i] & 1u) ? A.wall[4 * static_cast<size_t>(i) + 3] : 1.0;
+}
+
 __global__ __launch_bounds__(kPiBlock) void k_pi_volume(PiArgs A) {
   const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
-  if (i < A.n) A.vol[i] = A.P.m / A.a.rho[i];
+  if (i < A.n) A.vol[i] = mass_of(A, i) / A.a.rho[i];
 }
 
 // apply_shifts, sums and renormalisation (:335-365)
@@ -230,12 +246,30 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
   const double rho_a = A.a.rho[i];
   const double w = kernel_factor<DIM, KERNEL>(P), h_inv = 1.0 / P.h;
   double N[DIM] = {}, Lm[DIM * DIM] = {}, G[DIM * DIM] = {}, g[DIM] = {};
+  if constexpr (DIM == 2) {
+    // the face terms of a particle near a wall (fluid_equations.hpp:342-349)
+    const uint32_t q = A.near_idx ? A.near_idx[i] : ~0u;
+    if (q != ~0u) {
+      const double* t = A.pi_face + 12 * static_cast<size_t>(q);
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        N[p] = t[p];
+        g[p] = t[10 + p];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        Lm[k] = t[2 + k];
+        G[k] = t[6 + k];
+      }
+    }
+  }
+  const double ga = gamma_of(A, i);
   uint32_t cnt = 0;
   for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) {
     ++cnt;
     if (j == i) return;
     const double rho_b = A.a.rho[j];
-    const double V_b = A.vol[j];
+    const double V_b = A.vol[j] / ga;
     // unit direction of r_ab (zero below the tiny norm, as normalize does)
     const double r = sqrt(s2);
     const double inv_r = s2 >= kTiny * kTiny ? 1.0 / r : 0.0;
@@ -446,11 +480,13 @@ __device__ __forceinline__ void shift_one(const PiArgs& A, uint32_t i,
     double drho = A.grad_rho[0][i] * dr[0];
 #pragma unroll
     for (int k = 1; k < DIM; ++k) drho = drho + A.grad_rho[k][i] * dr[k];
+    // the velocity correction only where gamma = 1 (:465-468)
+    const bool correct_v = fabs(gamma_of(A, i) - 1.0) <= kTiny;
     double d2 = 0.0;
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
       xa[k] += dr[k];
-      va[k] += dv[k];
+      if (correct_v) va[k] += dv[k];
       d2 += dr[k] * dr[k];
     }
     rho += drho;
@@ -500,14 +536,16 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_fsc(PiArgs A) {
         x2 += x * x;
       }
       const double W = w * kernel_value_poly_f64<KERNEL>(h_inv * sqrt(x2));
-      alpha += P.m / A.b.rho[j] * W;
-      rho_t += P.m * W;
+      const double m_b = mass_of(A, j);
+      alpha += m_b / A.b.rho[j] * W;
+      rho_t += m_b * W;
     });
-    const double ratio = fmin(1.0, alpha);
+    const double ga = gamma_of(A, i);
+    const double ratio = fmin(1.0, alpha / ga);
     if (!(ratio > 0.99)) {
       const double K = -log(0.05) / (0.01 * 0.01);
       const double beta = exp(-K * ((ratio - 1.0) * (ratio - 1.0)));
-      const double corr = beta + (1.0 - beta) * alpha;
+      const double corr = beta * ga + (1.0 - beta) * alpha;
       if (fabs(corr) > kTiny) rho = rho_t / corr;
     }
   }

@@ -42,7 +42,7 @@ struct StageArgs {
   uint32_t grid_lds, grid_global;  // persistent grid sizes
   double* dv_last[3];          // final stage: its acceleration (post-integration only)
   // semi-analytic wall (kFlavourWall): per slot, the face terms of drho/dt
-  // and dv/dt already divided by gamma, then 1 / gamma
+  // and dv/dt already divided by gamma, then gamma
   const double* wall;
 };
 
@@ -72,6 +72,13 @@ struct PiArgs {
   double* dv_last[3];    // the last stage's acceleration
   unsigned long long* dt_min;
   uint32_t* vmax;
+  // semi-analytic wall (null without): gamma at [4 slot + 3], the fixed
+  // particles' mass fractions by id - wall_id0, the face terms of apply_shifts
+  const double* wall;
+  const double* wall_mu;
+  uint32_t wall_id0;
+  const uint32_t* near_idx;
+  const double* pi_face;
 };
 
 struct BoundaryArgs {
@@ -136,7 +143,10 @@ void launch_boundary(hipStream_t s, const BoundaryArgs& a);
 
 // Semi-analytic wall (wall.hip): per fluid particle of the stage input state,
 // gamma, grad gamma and the face terms, into out (4 per slot: face drho/dt and
-// dv/dt already divided by gamma, then 1 / gamma).
+// dv/dt already divided by gamma, then gamma).  With pi_face set (before
+// post-integration), also the face terms of apply_shifts of each particle
+// near a wall (2D: N, L, grad v, grad rho = 12 values, already divided by
+// gamma) at its index in the near list, near_idx[slot] (~0: none).
 struct WallFace;
 struct WallArgs {
   DevParams P;
@@ -155,6 +165,8 @@ struct WallArgs {
   double* out;
   uint32_t* near;                  // scratch: slots of the particles near a wall
   uint32_t* near_count;
+  uint32_t* near_idx;              // post-integration: per slot, index in near (~0: none)
+  double* pi_face;                 // post-integration: 12 per near-list index
 };
 void launch_wall(hipStream_t s, const WallArgs& a, int n_cu);
 // post_integrate.hip: the passes in order, the shift into disp[] (flags bit 0

@@ -726,7 +726,7 @@ __device__ __forceinline__ void chunk_targets(const StageArgs& A, const ChunkInf
     if constexpr (FLAVOUR == kFlavourWall) {
       // face terms (already divided by gamma_a) + pair sums / gamma_a
       const double* wt = A.wall + 4 * static_cast<size_t>(i);
-      const double ig = wt[3];
+      const double ig = 1.0 / wt[3];
       drho = wt[0] + drho * ig;
 #pragma unroll
       for (int k = 0; k < DIM; ++k) dv[k] = wt[1 + k] + dv[k] * ig;

@@ -68,25 +68,27 @@ __global__ __launch_bounds__(kWallBlock) void k_wall_scan(WallArgs A) {
     const uint32_t cell = wall_cell(A, A.s.x[0][i], A.s.x[1][i]);
     near = A.cell_faces_off[cell] != A.cell_faces_off[cell + 1];
     if (!near) {
-      const double gamma = A.cell_inside[cell] ? 1.0 : 0.0;
       double* o = A.out + 4 * static_cast<size_t>(i);
       o[0] = 0.0;
       o[1] = 0.0;
       o[2] = 0.0;
-      o[3] = 1.0 / gamma;
+      o[3] = A.cell_inside[cell] ? 1.0 : 0.0;
     }
   }
   const uint64_t m = __ballot(near);
-  if (m == 0) return;
+  if (m == 0) {
+    if (A.near_idx && i < A.n) A.near_idx[i] = ~0u;
+    return;
+  }
   const int lane = static_cast<int>(threadIdx.x % 64);
   const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
   uint32_t base = 0;
   if (lane == leader) base = atomicAdd(A.near_count, static_cast<uint32_t>(__popcll(m)));
   base = __shfl(base, leader);
-  if (near) {
-    const uint64_t below = lane ? (m & ((1ull << lane) - 1ull)) : 0ull;
-    A.near[base + static_cast<uint32_t>(__popcll(below))] = i;
-  }
+  const uint64_t below = lane ? (m & ((1ull << lane) - 1ull)) : 0ull;
+  const uint32_t g = base + static_cast<uint32_t>(__popcll(below));
+  if (near) A.near[g] = i;
+  if (A.near_idx && i < A.n) A.near_idx[i] = near ? g : ~0u;
 }
 
 constexpr int kWallLanes = 16;
@@ -182,7 +184,41 @@ __global__ __launch_bounds__(kWallBlock) void k_wall_near(WallArgs A) {
       o[0] = d[0];
       o[1] = d[1];
       o[2] = d[2];
-      o[3] = 1.0 / gamma;
+      o[3] = gamma;
+    }
+    if (A.pi_face) {
+      // apply_shifts face terms (fluid_equations.hpp:342-349), the wall
+      // particles at rest: r_sa = face midpoint - r_a, v_sa = -v_a,
+      // rho_sa = mean vertex density - rho_a
+      double t[12] = {};
+      const double rho_a = A.s.rho[i];
+      const double va[2] = {A.s.v[0][i], A.s.v[1][i]};
+      for (uint32_t j = fb + lane; j < fe; j += kWallLanes) {
+        const WallFace f = A.faces[A.cell_faces[j]];
+        if (!wall_face_intersects(f, x, y, R)) continue;
+        double gf[2];
+        wall_face_flux<KERNEL>(f, x, y, h, gf);
+        const double r_sa[2] = {(f.ax + f.bx) / 2.0 - x, (f.ay + f.by) / 2.0 - y};
+        const double v_sa[2] = {0.0 - va[0], 0.0 - va[1]};
+        const double rho_sa = (A.vert_rho[f.s0] + A.vert_rho[f.s1]) / 2.0 - rho_a;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          t[p] -= gf[p] / gamma;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            t[2 + 2 * p + q] -= r_sa[p] * gf[q] / gamma;
+            t[6 + 2 * p + q] -= v_sa[p] * gf[q] / gamma;
+          }
+          t[10 + p] -= rho_sa * gf[p] / gamma;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 12; ++k) t[k] = group_sum(t[k]);
+      if (lane == 0) {
+        double* o = A.pi_face + 12 * static_cast<size_t>(g);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) o[k] = t[k];
+      }
     }
   }
 }
