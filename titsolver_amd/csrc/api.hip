@@ -342,7 +342,12 @@ tit_status build_schedule(tit_hip_ctx* c) {
     }
   }
   if (cols.size() >= (1ull << 31)) return fail(c, TIT_E_ARG, "too many cell columns");
-  const size_t need = c->n / c->chunk_targets + cols.size() + 1;
+  // equal parts: n / chunk_targets + one per column at most; cuts by span:
+  // (nz - 1) / kChunkRows per column at most
+  int nz_max = 1;
+  for (const DevTank& T : c->host_tanks) nz_max = std::max(nz_max, T.nc[dim - 1]);
+  const size_t need = c->n / c->chunk_targets +
+                      cols.size() * (1 + static_cast<size_t>(nz_max - 1) / kChunkRows) + 1;
   if (!c->cols || cols.size() != c->n_cols) {
     forget(c, c->cols);
     forget(c, c->block_tot);
@@ -466,6 +471,7 @@ void enqueue_post_integrate(tit_hip_ctx* c) {
   a.vol = p + 2 * n;
   a.count = c->pi_count;
   a.cell_mark = c->pi_cells;
+  a.cell_near = c->pi_cells + c->n_cells + 1;
   a.n_cells = c->n_cells;
   a.shift_max = c->pi_shift;
   for (int k = 0; k < dim; ++k) a.dv_last[k] = c->rhs_dv[k];
@@ -811,8 +817,10 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->dt, n_tanks));
   TIT_HIP_TRY(c, dalloc(c, &c->time, n_tanks));
   TIT_HIP_TRY(c, dalloc(c, &c->disp, n_tanks));
-  TIT_HIP_TRY(c, dalloc(c, &c->dt_min, n_tanks));
-  TIT_HIP_TRY(c, dalloc(c, &c->vmax, n_tanks));
+  // per-tank reduction slots: kRedStripes copies per tank
+  const size_t n_red = static_cast<size_t>(n_tanks) * kRedStripes;
+  TIT_HIP_TRY(c, dalloc(c, &c->dt_min, n_red));
+  TIT_HIP_TRY(c, dalloc(c, &c->vmax, n_red));
   TIT_HIP_TRY(c, dalloc(c, &c->stage_r, static_cast<size_t>(n) * dim));
   TIT_HIP_TRY(c, dalloc(c, &c->stage_v, static_cast<size_t>(n) * dim));
   TIT_HIP_TRY(c, dalloc(c, &c->stage_rho, n));
@@ -824,9 +832,9 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
     const size_t per = static_cast<size_t>(3 * dim + dim * dim + 3);
     TIT_HIP_TRY(c, dalloc(c, &c->pi_buf, per * n));
     TIT_HIP_TRY(c, dalloc(c, &c->pi_count, n));
-    TIT_HIP_TRY(c, dalloc(c, &c->pi_cells, static_cast<size_t>(c->n_cells) + 1));
-    TIT_HIP_TRY(c, dalloc(c, &c->pi_shift, n_tanks));
-    TIT_HIP_TRY(c, hipMemsetAsync(c->pi_shift, 0, n_tanks * sizeof(unsigned long long),
+    TIT_HIP_TRY(c, dalloc(c, &c->pi_cells, 2 * (static_cast<size_t>(c->n_cells) + 1)));
+    TIT_HIP_TRY(c, dalloc(c, &c->pi_shift, n_red));
+    TIT_HIP_TRY(c, hipMemsetAsync(c->pi_shift, 0, n_red * sizeof(unsigned long long),
                                   c->stream));
     // the first step's time step selection reads the uploaded dv/dt only
     for (int k = 0; k < dim; ++k) {
@@ -886,7 +894,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
                                 hipMemcpyHostToDevice, s));
   launch_import(s, c->dp, n, c->stage_r, c->stage_v, c->stage_rho, c->A, c->id);
   std::vector<double> zeros(n_tanks, 0.0);
-  std::vector<unsigned long long> inf(n_tanks, ~0ull);
+  std::vector<unsigned long long> inf(static_cast<size_t>(n_tanks) * kRedStripes, ~0ull);
   std::vector<uint32_t> z32(4, 0u);
   TIT_HIP_TRY(c, hipMemcpyAsync(c->time, zeros.data(), n_tanks * sizeof(double),
                                 hipMemcpyHostToDevice, s));
@@ -894,8 +902,9 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
                                 hipMemcpyHostToDevice, s));
   TIT_HIP_TRY(c, hipMemcpyAsync(c->dt, zeros.data(), n_tanks * sizeof(double),
                                 hipMemcpyHostToDevice, s));
-  TIT_HIP_TRY(c, hipMemcpyAsync(c->dt_min, inf.data(), n_tanks * sizeof(unsigned long long),
+  TIT_HIP_TRY(c, hipMemcpyAsync(c->dt_min, inf.data(), inf.size() * sizeof(unsigned long long),
                                 hipMemcpyHostToDevice, s));
+  TIT_HIP_TRY(c, hipMemsetAsync(c->vmax, 0, inf.size() * sizeof(uint32_t), s));
   TIT_HIP_TRY(c, hipMemcpyAsync(c->counter, z32.data(), 4 * sizeof(uint32_t),
                                 hipMemcpyHostToDevice, s));
   // the first step rebuilds every tank
@@ -973,12 +982,15 @@ tit_status tit_hip_tank_clock(tit_hip_ctx* c, double* time, double* dt_next) {
     TIT_HIP_TRY(c, hipMemcpy(time, c->time, c->n_tanks * sizeof(double), hipMemcpyDeviceToHost));
   }
   if (dt_next) {
-    std::vector<unsigned long long> b(c->n_tanks);
+    std::vector<unsigned long long> b(static_cast<size_t>(c->n_tanks) * kRedStripes);
     TIT_HIP_TRY(c, hipMemcpy(b.data(), c->dt_min, b.size() * sizeof(unsigned long long),
                              hipMemcpyDeviceToHost));
     for (int t = 0; t < c->n_tanks; ++t) {
+      // the smallest of the tank's copies
+      unsigned long long m = ~0ull;
+      for (int k = 0; k < kRedStripes; ++k) m = std::min(m, b[static_cast<size_t>(t) * kRedStripes + k]);
       double x = 0.0;
-      if (b[t] != ~0ull) std::memcpy(&x, &b[t], sizeof(double));
+      if (m != ~0ull) std::memcpy(&x, &m, sizeof(double));
       dt_next[t] = x;
     }
   }

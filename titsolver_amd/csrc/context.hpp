@@ -66,7 +66,7 @@ struct tit_hip_ctx {
   // rhs_dv holds the last stage's acceleration between steps then
   double* pi_buf = nullptr;  // N, dr0, grad v, grad rho, phi, phi_near (SoA)
   uint32_t* pi_count = nullptr;
-  uint32_t* pi_cells = nullptr;  // per cell mark
+  uint32_t* pi_cells = nullptr;  // per cell: surface mark, then window mark
   unsigned long long* pi_shift = nullptr;  // per tank
   // semi-analytic wall (tit_hip_set_walls; one 2D tank, REF terms)
   bool walls = false;

@@ -202,4 +202,38 @@ __device__ __forceinline__ unsigned long long dbl_bits(double x) {
   return static_cast<unsigned long long>(__double_as_longlong(x));
 }
 
+// ---- per-tank reduction slots ------------------------------------------------
+// A per-tank minimum or maximum reduced over a whole grid (the next step
+// size, the largest speed, the largest shift) lives in kRedStripes copies
+// per tank, slot t * kRedStripes + s.  A wave updates the copy picked by its
+// global wave index, so the atomics of a large grid spread over many
+// addresses: atomics on one address serialise at about 11.5 ns each on
+// MI355X (8192 wave flushes onto one slot take ~95 us, onto 64 slots ~2 us).
+// The readers fold the copies with one wave per tank, lane s taking copy s.
+constexpr int kRedStripes = 64;
+
+__device__ __forceinline__ uint32_t red_stripe() {
+  return (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kRedStripes - 1);
+}
+
+template <class T>
+__device__ __forceinline__ T* red_slot(T* base, uint32_t t) {
+  return base + static_cast<size_t>(t) * kRedStripes + red_stripe();
+}
+
+// Folds tank t's copies (called by all 64 lanes of a wave; lane s reads copy
+// s) and, with `reset`, puts the identity back.  Every lane gets the result.
+template <bool MAX, class T>
+__device__ __forceinline__ T red_fold(T* base, uint32_t t, bool reset) {
+  T* p = base + static_cast<size_t>(t) * kRedStripes + (threadIdx.x & 63);
+  T v = *p;
+  if (reset) *p = MAX ? T(0) : ~T(0);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const T o = __shfl_xor(v, off);
+    v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+  }
+  return v;
+}
+
 }  // namespace tit_amd

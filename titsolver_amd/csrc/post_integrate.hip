@@ -62,27 +62,30 @@ __device__ __forceinline__ void tank_atomic(bool part, uint32_t t, T v, T* slot)
     }
     if ((threadIdx.x & 63) == static_cast<unsigned>(first)) {
       if constexpr (MAX) {
-        atomicMax(slot + t0, r);
+        atomicMax(red_slot(slot, t0), r);
       } else {
-        atomicMin(slot + t0, r);
+        atomicMin(red_slot(slot, t0), r);
       }
     }
   } else if (part) {
     if constexpr (MAX) {
-      atomicMax(slot + t, v);
+      atomicMax(red_slot(slot, t), v);
     } else {
-      atomicMin(slot + t, v);
+      atomicMin(red_slot(slot, t), v);
     }
   }
 }
 
-// Calls f(j, d, s2) for every particle j (self included) with
-// s2 = |r_j - r_a|^2 <= (2h)^2, d = r_j - r_a, positions from S; a callback
-// that returns bool stops the walk by returning true.  With ONLY_MARKED,
-// only the stencil columns with a marked cell are visited.
-template <int DIM, bool ONLY_MARKED = false, class F>
-__device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, const double* xa,
-                                               const DevState& S, F&& f) {
+// The 5^d cell window around particle i's cell, clamped to its tank: cells
+// (ix, iy, z) for ix in [xlo, xhi], iy in [ylo, yhi] (3D; 0 in 2D), z in
+// [zlo, zhi] along the last axis; row(ix, iy) is the global index of (ix, iy, 0).
+struct Window {
+  uint32_t cell_off, nc1, nc2;
+  int xlo, xhi, ylo, yhi, zlo, zhi;
+};
+
+template <int DIM>
+__device__ __forceinline__ Window window_of(const PiArgs& A, uint32_t i) {
   const DevTank T = A.tanks[A.tag[i] >> 1];
   const uint32_t local = A.key[i] - T.cell_off;
   int c[3];
@@ -96,15 +99,37 @@ __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, cons
     c[0] = static_cast<int>(q / T.nc[1]);
   }
   constexpr int L = DIM - 1;
-  const int zlo = max(c[L] - kStencil, 0), zhi = min(c[L] + kStencil, T.nc[L] - 1);
-  const int ylo = DIM == 3 ? max(c[1] - kStencil, 0) : 0;
-  const int yhi = DIM == 3 ? min(c[1] + kStencil, T.nc[1] - 1) : 0;
-  for (int ix = max(c[0] - kStencil, 0); ix <= min(c[0] + kStencil, T.nc[0] - 1); ++ix) {
-    for (int iy = ylo; iy <= yhi; ++iy) {
-      const uint32_t row =
-          DIM == 2 ? T.cell_off + static_cast<uint32_t>(ix) * T.nc[1]
-                   : T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] +
-                                   static_cast<uint32_t>(iy)) * T.nc[2];
+  Window w;
+  w.cell_off = T.cell_off;
+  w.nc1 = static_cast<uint32_t>(T.nc[1]);
+  w.nc2 = DIM == 3 ? static_cast<uint32_t>(T.nc[2]) : 1u;
+  w.xlo = max(c[0] - kStencil, 0);
+  w.xhi = min(c[0] + kStencil, T.nc[0] - 1);
+  w.ylo = DIM == 3 ? max(c[1] - kStencil, 0) : 0;
+  w.yhi = DIM == 3 ? min(c[1] + kStencil, T.nc[1] - 1) : 0;
+  w.zlo = max(c[L] - kStencil, 0);
+  w.zhi = min(c[L] + kStencil, T.nc[L] - 1);
+  return w;
+}
+
+template <int DIM>
+__device__ __forceinline__ uint32_t window_row(const Window& w, int ix, int iy) {
+  return DIM == 2 ? w.cell_off + static_cast<uint32_t>(ix) * w.nc1
+                  : w.cell_off + (static_cast<uint32_t>(ix) * w.nc1 + static_cast<uint32_t>(iy)) * w.nc2;
+}
+
+// Calls f(j, d, s2) for every particle j (self included) with
+// s2 = |r_j - r_a|^2 <= (2h)^2, d = r_j - r_a, positions from S; a callback
+// that returns bool stops the walk by returning true.  With ONLY_MARKED,
+// only the stencil columns with a marked cell are visited.
+template <int DIM, bool ONLY_MARKED = false, class F>
+__device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, const double* xa,
+                                               const DevState& S, F&& f) {
+  const Window w = window_of<DIM>(A, i);
+  const int zlo = w.zlo, zhi = w.zhi;
+  for (int ix = w.xlo; ix <= w.xhi; ++ix) {
+    for (int iy = w.ylo; iy <= w.yhi; ++iy) {
+      const uint32_t row = window_row<DIM>(w, ix, iy);
       if constexpr (ONLY_MARKED) {
         bool any = false;
         for (int z = zlo; z <= zhi; ++z) any = any || A.cell_mark[row + z] != 0u;
@@ -127,6 +152,71 @@ __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, cons
       }
     }
   }
+}
+
+// Accepted candidates buffered per lane: kPiWords windows of 32 consecutive
+// candidates (a mask and the window's first index each), in LDS.
+constexpr int kPiWords = 12;
+
+// The same neighbours in the same order as for_neighbours, for a callback
+// too heavy to run under the acceptance branch: a wave's lanes walk
+// different candidates, so under the branch the wave pays the callback on
+// nearly every candidate iteration.  Here the predicate only fills masks,
+// and the callback runs in one flat loop over the buffered masks: a wave
+// pays it about max-over-lanes(neighbour count) times.  s_m, s_j: this
+// workgroup's kPiWords x kPiBlock buffers.
+template <int DIM, class F>
+__device__ __forceinline__ void for_neighbours_packed(const PiArgs& A, uint32_t i,
+                                                      const double* xa, const DevState& S,
+                                                      uint32_t* s_m, uint32_t* s_j, F&& f) {
+  const uint32_t lane = threadIdx.x;
+  int nw = 0;
+  auto drain = [&]() {
+    int w = 0;
+    uint32_t m = nw > 0 ? s_m[lane] : 0u;
+    for (;;) {
+      while (m == 0u && ++w < nw) m = s_m[w * kPiBlock + lane];
+      if (m == 0u) break;
+      const uint32_t t = static_cast<uint32_t>(__builtin_ctz(m));
+      m &= m - 1u;
+      const uint32_t j = s_j[w * kPiBlock + lane] + t;
+      double d[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) d[k] = S.x[k][j] - xa[k];
+      double s2 = d[0] * d[0];
+#pragma unroll
+      for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
+      f(j, d, s2);
+    }
+    nw = 0;
+  };
+  const Window win = window_of<DIM>(A, i);
+  for (int ix = win.xlo; ix <= win.xhi; ++ix) {
+    for (int iy = win.ylo; iy <= win.yhi; ++iy) {
+      const uint32_t row = window_row<DIM>(win, ix, iy);
+      const uint32_t b = A.cell_off[row + win.zlo], e = A.cell_off[row + win.zhi + 1];
+      for (uint32_t c0 = b; c0 < e; c0 += 32u) {
+        const uint32_t len = min(32u, e - c0);
+        uint32_t mask = 0u;
+        for (uint32_t t = 0; t < len; ++t) {
+          const uint32_t j = c0 + t;
+          double d[DIM];
+#pragma unroll
+          for (int k = 0; k < DIM; ++k) d[k] = S.x[k][j] - xa[k];
+          double s2 = d[0] * d[0];
+#pragma unroll
+          for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
+          if (s2 <= A.P.R2) mask |= 1u << t;
+        }
+        if (mask != 0u) {
+          s_m[nw * kPiBlock + lane] = mask;
+          s_j[nw * kPiBlock + lane] = c0;
+          if (++nw == kPiWords) drain();
+        }
+      }
+    }
+  }
+  drain();
 }
 
 template <int KERNEL>
@@ -234,6 +324,7 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_volume(PiArgs A) {
 // apply_shifts, sums and renormalisation (:335-365)
 template <int DIM, int KERNEL>
 __global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
+  __shared__ uint32_t s_m[kPiWords * kPiBlock], s_j[kPiWords * kPiBlock];
   const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
   if (i >= A.n) return;
   const DevParams& P = A.P;
@@ -265,11 +356,12 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
   }
   const double ga = gamma_of(A, i);
   uint32_t cnt = 0;
-  for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) {
+  for_neighbours_packed<DIM>(A, i, xa, A.a, s_m, s_j, [&](uint32_t j, const double* d, double s2) {
     ++cnt;
     if (j == i) return;
     const double rho_b = A.a.rho[j];
-    const double V_b = A.vol[j] / ga;
+    // V_b / gamma_a (x / 1 is x: no division away from walls)
+    const double V_b = ga == 1.0 ? A.vol[j] : A.vol[j] / ga;
     // unit direction of r_ab (zero below the tiny norm, as normalize does)
     const double r = sqrt(s2);
     const double inv_r = s2 >= kTiny * kTiny ? 1.0 / r : 0.0;
@@ -379,7 +471,18 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_phi(PiArgs A) {
     });
   }
   A.phi[i] = phi;
-  if (same_bits(phi, DBL_MIN)) A.cell_mark[A.key[i]] = 1u;  // a surface particle's cell
+  if (same_bits(phi, DBL_MIN)) {
+    // a surface particle's cell, and every cell whose window holds it (the
+    // window relation is symmetric: the cells within kStencil of its cell)
+    A.cell_mark[A.key[i]] = 1u;
+    const Window w = window_of<DIM>(A, i);
+    for (int ix = w.xlo; ix <= w.xhi; ++ix) {
+      for (int iy = w.ylo; iy <= w.yhi; ++iy) {
+        const uint32_t row = window_row<DIM>(w, ix, iy);
+        for (int z = w.zlo; z <= w.zhi; ++z) A.cell_near[row + z] = 1u;
+      }
+    }
+  }
 }
 
 __global__ __launch_bounds__(kPiBlock) void k_pi_clear(uint32_t n, uint32_t* mark) {
@@ -400,9 +503,9 @@ struct TankRun {
     if (tank != t) {
       if (t != ~0u) {
         if constexpr (MAX) {
-          atomicMax(slot + t, v);
+          atomicMax(red_slot(slot, t), v);
         } else {
-          atomicMin(slot + t, v);
+          atomicMin(red_slot(slot, t), v);
         }
       }
       t = tank;
@@ -439,7 +542,9 @@ __device__ __forceinline__ void shift_one(const PiArgs& A, uint32_t i,
   double rho = A.a.rho[i];
   double phi = A.phi[i];
   const bool fluid = !(A.tag[i] & 1u);
-  if (fluid && same_bits(phi, kPhiMax)) {
+  // only a particle whose window holds a surface particle can have one as a
+  // neighbour
+  if (fluid && same_bits(phi, kPhiMax) && A.cell_near[A.key[i]] != 0u) {
     // the nearest surface neighbour; the reference's lists are in index
     // order, so equal distances go to the smaller index
     uint32_t best = ~0u, best_id = ~0u;
@@ -553,15 +658,16 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_fsc(PiArgs A) {
 }
 
 // Shift displacement into the index's bound; the next step size is taken
-// again after the shifts.
-__global__ void k_pi_bound(int n_tanks, unsigned long long* shift_max, double* disp,
-                           double skin_half, uint32_t* flags, unsigned long long* dt_min) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_tanks) return;
-  const double d = disp[t] + __longlong_as_double(static_cast<long long>(shift_max[t]));
+// again after the shifts.  One wave per tank.
+__global__ __launch_bounds__(64) void k_pi_bound(unsigned long long* shift_max, double* disp,
+                                                 double skin_half, uint32_t* flags,
+                                                 unsigned long long* dt_min) {
+  const uint32_t t = blockIdx.x;
+  const unsigned long long sm = red_fold<true>(shift_max, t, true);
+  dt_min[static_cast<size_t>(t) * kRedStripes + threadIdx.x] = ~0ull;
+  if (threadIdx.x != 0) return;
+  const double d = disp[t] + __longlong_as_double(static_cast<long long>(sm));
   disp[t] = d;
-  shift_max[t] = 0ull;
-  dt_min[t] = ~0ull;
   if (!(d <= skin_half)) atomicOr(flags, 1u);
 }
 
@@ -605,6 +711,20 @@ __global__ void k_export_f64(uint32_t n, const uint32_t* __restrict__ id,
   if (i < n) dst[id[i]] = src[i];
 }
 
+// Workgroups of K resident per CU (registers and LDS), asked once: the
+// persistent passes launch exactly that many per CU, so that no part of
+// their grid waits for a second round.
+template <auto K>
+uint32_t resident_per_cu() {
+  static const uint32_t n = [] {
+    int b = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(K),
+                                                       kPiBlock, 0);
+    return static_cast<uint32_t>(std::max(b, 1));
+  }();
+  return n;
+}
+
 }  // namespace
 
 void launch_export_f64(hipStream_t s, uint32_t n, const uint32_t* id, const double* src,
@@ -616,9 +736,15 @@ void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* 
                            double skin_half, uint32_t* flags, int n_cu) {
   if (a.n == 0) return;
   const uint32_t nb = (a.n + kPiBlock - 1) / kPiBlock;
-  // the reducing passes run persistent: 8 workgroups per CU at most
-  const uint32_t np = std::min<uint32_t>(nb, 8u * static_cast<uint32_t>(std::max(n_cu, 1)));
   const int dim = a.P.dim, kernel = a.P.kernel;
+  // the reducing passes run persistent, as many workgroups as stay resident
+  const uint32_t cus = static_cast<uint32_t>(std::max(n_cu, 1));
+  const uint32_t np_shift =
+      std::min(nb, cus * (dim == 2 ? resident_per_cu<k_pi_shift<2>>() : resident_per_cu<k_pi_shift<3>>()));
+  const uint32_t np_commit =
+      std::min(nb, cus * (dim == 2 ? resident_per_cu<k_pi_commit<2>>() : resident_per_cu<k_pi_commit<3>>()));
+  // cell_mark and cell_near lie back to back, n_cells + 1 entries each
+  const uint32_t n_marks = 2u * (a.n_cells + 1u);
   k_pi_volume<<<nb, kPiBlock, 0, s>>>(a);
   if (dim == 2) {
     if (kernel == kKernelC2) {
@@ -626,9 +752,9 @@ void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* 
     } else {
       k_pi_sums<2, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
     }
-    k_pi_clear<<<(a.n_cells + kPiBlock - 1) / kPiBlock, kPiBlock, 0, s>>>(a.n_cells, a.cell_mark);
+    k_pi_clear<<<(n_marks + kPiBlock - 1) / kPiBlock, kPiBlock, 0, s>>>(n_marks, a.cell_mark);
     k_pi_phi<2><<<nb, kPiBlock, 0, s>>>(a);
-    k_pi_shift<2><<<np, kPiBlock, 0, s>>>(a);
+    k_pi_shift<2><<<np_shift, kPiBlock, 0, s>>>(a);
     if (kernel == kKernelC2) {
       k_pi_fsc<2, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
     } else {
@@ -640,21 +766,20 @@ void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* 
     } else {
       k_pi_sums<3, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
     }
-    k_pi_clear<<<(a.n_cells + kPiBlock - 1) / kPiBlock, kPiBlock, 0, s>>>(a.n_cells, a.cell_mark);
+    k_pi_clear<<<(n_marks + kPiBlock - 1) / kPiBlock, kPiBlock, 0, s>>>(n_marks, a.cell_mark);
     k_pi_phi<3><<<nb, kPiBlock, 0, s>>>(a);
-    k_pi_shift<3><<<np, kPiBlock, 0, s>>>(a);
+    k_pi_shift<3><<<np_shift, kPiBlock, 0, s>>>(a);
     if (kernel == kKernelC2) {
       k_pi_fsc<3, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
     } else {
       k_pi_fsc<3, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
     }
   }
-  k_pi_bound<<<(n_tanks + 63) / 64, 64, 0, s>>>(n_tanks, a.shift_max, disp, skin_half, flags,
-                                                a.dt_min);
+  k_pi_bound<<<n_tanks, 64, 0, s>>>(a.shift_max, disp, skin_half, flags, a.dt_min);
   if (dim == 2) {
-    k_pi_commit<2><<<np, kPiBlock, 0, s>>>(a);
+    k_pi_commit<2><<<np_commit, kPiBlock, 0, s>>>(a);
   } else {
-    k_pi_commit<3><<<np, kPiBlock, 0, s>>>(a);
+    k_pi_commit<3><<<np_commit, kPiBlock, 0, s>>>(a);
   }
 }
 

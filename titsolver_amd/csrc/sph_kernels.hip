@@ -91,9 +91,9 @@ __device__ __forceinline__ void wave_atomic_min_u64(unsigned long long* base,
       const unsigned long long o = __shfl_xor(m, off);
       m = o < m ? o : m;
     }
-    if (lane == leader) atomicMin(base + t0, m);
+    if (lane == leader) atomicMin(red_slot(base, t0), m);
   } else if (active) {
-    atomicMin(base + t, v);
+    atomicMin(red_slot(base, t), v);
   }
 }
 
@@ -467,18 +467,19 @@ __global__ __launch_bounds__(kBlock) void k_time_step(
 
 __global__ void k_rebuild_clear(uint32_t* ctl) { ctl[0] = 0u; }
 
-__global__ void k_step_begin(int n_tanks, unsigned long long* dt_min, double* dt,
-                             uint32_t* vmax, const double* disp, double skin_half,
-                             double span, uint32_t* ctl, uint32_t* need) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_tanks) return;
-  const unsigned long long b = dt_min[t];
+// One wave per tank (the reduction slots' copies folded lane by lane).
+__global__ __launch_bounds__(64) void k_step_begin(unsigned long long* dt_min, double* dt,
+                                                   uint32_t* vmax, const double* disp,
+                                                   double skin_half, double span, uint32_t* ctl,
+                                                   uint32_t* need) {
+  const uint32_t t = blockIdx.x;
+  const unsigned long long b = red_fold<false>(dt_min, t, true);
+  const uint32_t vb = red_fold<true>(vmax, t, true);
+  if (threadIdx.x != 0) return;
   // A tank without fluid keeps dt = 0 (nothing moves).
   const double h = b == ~0ull ? 0.0 : __longlong_as_double(static_cast<long long>(b));
   dt[t] = h;
-  dt_min[t] = ~0ull;
-  const double vprev = static_cast<double>(__uint_as_float(vmax[t]));
-  vmax[t] = 0u;
+  const double vprev = static_cast<double>(__uint_as_float(vb));
   const bool nd = ctl[1] != 0u || !(disp[t] + 2.0 * span * h * vprev <= 0.9 * skin_half);
   need[t] = nd ? 1u : 0u;
   if (nd) atomicOr(ctl, 1u);
@@ -499,14 +500,16 @@ __global__ void k_pending_rebuild(int n_tanks, uint32_t* ctl, uint32_t* need) {
   if (t == 0) ctl[0] = f;
 }
 
-__global__ void k_step_end(int n_tanks, const double* dt, const uint32_t* vmax,
-                           double* time, double* disp, double skin_half,
-                           uint32_t* flags, double span) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_tanks) return;
+// One wave per tank.
+__global__ __launch_bounds__(64) void k_step_end(const double* dt, uint32_t* vmax, double* time,
+                                                 double* disp, double skin_half, uint32_t* flags,
+                                                 double span) {
+  const uint32_t t = blockIdx.x;
+  const uint32_t vb = red_fold<true>(vmax, t, false);
+  if (threadIdx.x != 0) return;
   const double T = span * dt[t];
   time[t] += T;
-  const double d = disp[t] + T * static_cast<double>(__uint_as_float(vmax[t]));
+  const double d = disp[t] + T * static_cast<double>(__uint_as_float(vb));
   disp[t] = d;
   if (!(d <= skin_half)) atomicOr(flags, 1u);
 }
@@ -708,8 +711,8 @@ void launch_step_begin(hipStream_t s, int n_tanks, unsigned long long* dt_min,
                        double* dt, uint32_t* vmax, const double* disp,
                        double skin_half, RebuildCtl rc, double span) {
   k_rebuild_clear<<<1, 1, 0, s>>>(rc.ctl);
-  k_step_begin<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, dt_min, dt, vmax, disp,
-                                                     skin_half, span, rc.ctl, rc.need);
+  k_step_begin<<<n_tanks, 64, 0, s>>>(dt_min, dt, vmax, disp, skin_half, span, rc.ctl,
+                                      rc.need);
 }
 
 void launch_force_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc) {
@@ -725,10 +728,9 @@ void launch_pending_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc) {
 }
 
 void launch_step_end(hipStream_t s, int n_tanks, const double* dt,
-                     const uint32_t* vmax, double* time, double* disp,
+                     uint32_t* vmax, double* time, double* disp,
                      double skin_half, uint32_t* flags, double span) {
-  k_step_end<<<grid_for(n_tanks, 64), 64, 0, s>>>(n_tanks, dt, vmax, time, disp,
-                                                   skin_half, flags, span);
+  k_step_end<<<n_tanks, 64, 0, s>>>(dt, vmax, time, disp, skin_half, flags, span);
 }
 
 void launch_reset_disp(hipStream_t s, int n_tanks, double* disp, RebuildCtl rc) {

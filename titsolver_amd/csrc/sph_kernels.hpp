@@ -9,6 +9,11 @@ namespace tit_amd {
 // 2D stage kernel.
 constexpr int kChunkMax = 128;
 constexpr int kStageLdsCap = 1024;
+// Cells a chunk's targets span along a column at most: a column's equal
+// parts are cut further where they span more (a column whose particles are
+// split by empty cells, such as fluid below and a wall's vertices at the
+// top), so that every chunk's window fits the LDS kernel's cell rows.
+constexpr int kChunkRows = 92;
 
 struct StageArgs {
   DevParams P;
@@ -26,12 +31,13 @@ struct StageArgs {
   double* rhs_drho;
   double* rhs_dv[3];
   uint32_t* rhs_count;       // accepted neighbours per target (may be null)
-  // reductions
+  // reductions (kRedStripes copies per tank, device_common.hpp)
   unsigned long long* dt_min;  // per tank, FP64 bits
   uint32_t* vmax;              // per tank, FP32 bits
   // chunk schedule: cell columns (tank, ix, iy, first cell) and, rebuilt on
   // the device after every cell rebuild, chunks (column, first particle,
   // end particle) of at most chunk_targets consecutive particles of a column
+  // spanning at most kChunkRows cells
   const uint4* cols;
   uint32_t n_cols;
   uint32_t chunk_targets;
@@ -66,9 +72,11 @@ struct PiArgs {
   double* phi;           // free-surface indicator after the cone test
   double* phi_near;      // after the near-surface classification
   uint32_t* cell_mark;   // per cell: holds a surface particle
+  uint32_t* cell_near;   // per cell: its 5^d window holds a surface particle
+                         // (cell_mark + n_cells + 1: the two are cleared together)
   double* vol;           // m / rho per particle
   uint32_t n_cells;
-  unsigned long long* shift_max;  // per tank, FP64 bits
+  unsigned long long* shift_max;  // per tank, FP64 bits (kRedStripes copies per tank)
   double* dv_last[3];    // the last stage's acceleration
   unsigned long long* dt_min;
   uint32_t* vmax;
@@ -186,6 +194,8 @@ void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot,
 void launch_time_step(hipStream_t s, const DevParams& P, uint32_t n,
                       DevState st, const uint32_t* tag, const uint32_t* id,
                       const double* dv_ref, unsigned long long* dt_min);
+// The per-tank dt_min / vmax / shift_max buffers hold kRedStripes copies per
+// tank (n_tanks x kRedStripes values); the launchers below fold them.
 // Takes each tank's pending step size dt and decides its rebuild for a step
 // of span x dt: needed when forced or when disp + 2 span dt vmax (vmax of the
 // previous step) could pass 0.9 of half the skin by the end of the step.
@@ -201,7 +211,7 @@ void launch_mark_pending(hipStream_t s, RebuildCtl rc);
 void launch_pending_rebuild(hipStream_t s, int n_tanks, RebuildCtl rc);
 // time += span dt; disp += span dt vmax, flagging a skin overrun.
 void launch_step_end(hipStream_t s, int n_tanks, const double* dt,
-                     const uint32_t* vmax, double* time, double* disp,
+                     uint32_t* vmax, double* time, double* disp,
                      double skin_half, uint32_t* flags, double span = 1.0);
 // Ends a rebuild sequence: zero displacement of the re-sorted tanks, count.
 void launch_reset_disp(hipStream_t s, int n_tanks, double* disp, RebuildCtl rc);

@@ -624,8 +624,8 @@ struct WaveReduce {
   __device__ __forceinline__ void flush(const StageArgs& A) {
     if (t == ~0u) return;
     if ((threadIdx.x & 63) == 0) {
-      if (vmax > 0.0f) atomicMax(A.vmax + t, __float_as_uint(vmax));
-      if (dtmin != ~0ull) atomicMin(A.dt_min + t, dtmin);
+      if (vmax > 0.0f) atomicMax(red_slot(A.vmax, t), __float_as_uint(vmax));
+      if (dtmin != ~0ull) atomicMin(red_slot(A.dt_min, t), dtmin);
     }
     vmax = 0.0f;
     dtmin = ~0ull;
@@ -885,14 +885,50 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 
 constexpr int kScanBlock = 1024;
 
-__device__ __forceinline__ uint32_t column_chunks(const StageArgs& A, uint32_t col_i,
-                                                  uint32_t* beg, uint32_t* n) {
+static_assert(kChunkRows + 2 * kStencil == kRowsMax, "a chunk's window rows fill the LDS rows");
+
+// The chunks of column col_i, in order: the column's particles in
+// ceil(n / chunk_targets) equal parts, each cut further into pieces whose
+// targets span at most kChunkRows cells (a piece starting at particle p ends
+// before the first particle with key >= key[p] + kChunkRows).  Calls
+// emit(k, p0, p1) for the k-th chunk and returns the count.  A column gains
+// at most (nz - 1) / kChunkRows cuts: each cut starts a piece kChunkRows
+// cells or more above the last, and the parts' spans do not overlap.
+template <class F>
+__device__ __forceinline__ uint32_t column_chunks(const StageArgs& A, uint32_t col_i, F&& emit) {
   const uint4 col = A.cols[col_i];
   const DevTank& T = A.tanks[col.x];
   const int nz = T.nc[A.P.dim - 1];
-  *beg = A.cell_off[col.w];
-  *n = A.cell_off[col.w + nz] - *beg;
-  return (*n + A.chunk_targets - 1) / A.chunk_targets;
+  const uint32_t beg = A.cell_off[col.w];
+  const uint32_t n = A.cell_off[col.w + nz] - beg;
+  const uint32_t nch = (n + A.chunk_targets - 1) / A.chunk_targets;
+  uint32_t count = 0;
+  for (uint32_t k = 0; k < nch; ++k) {
+    const uint32_t p0 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * k) / nch);
+    const uint32_t p1 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * (k + 1)) / nch);
+    uint32_t p = p0;
+    while (p < p1) {
+      const uint32_t lim = A.key[p] + static_cast<uint32_t>(kChunkRows);
+      uint32_t q = p1;
+      if (A.key[p1 - 1] >= lim) {
+        // the first particle in (p, p1) with key >= lim (keys ascend)
+        uint32_t lo = p + 1, hi = p1 - 1;
+        while (lo < hi) {
+          const uint32_t mid = lo + (hi - lo) / 2;
+          if (A.key[mid] >= lim) {
+            hi = mid;
+          } else {
+            lo = mid + 1;
+          }
+        }
+        q = lo;
+      }
+      emit(count, p, q);
+      ++count;
+      p = q;
+    }
+  }
+  return count;
 }
 
 // Exclusive scan over a 1024-thread block: wave shuffles, then the 16 wave
@@ -932,8 +968,8 @@ __global__ __launch_bounds__(kScanBlock) void k_chunk_count(StageArgs A, uint32_
   if (!ctl[0]) return;
   __shared__ uint32_t wsum[kScanBlock / 64 + 1];
   const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
-  uint32_t beg = 0, n = 0, nch = 0;
-  if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
+  uint32_t nch = 0;
+  if (c < A.n_cols) nch = column_chunks(A, c, [](uint32_t, uint32_t, uint32_t) {});
   uint32_t tot;
   (void)block_excl_scan(nch, wsum, &tot);
   if (threadIdx.x == 0) block_tot[blockIdx.x] = tot;
@@ -964,14 +1000,15 @@ __global__ __launch_bounds__(kScanBlock) void k_chunk_emit(StageArgs A,
   if (!ctl[0]) return;
   __shared__ uint32_t wsum[kScanBlock / 64 + 1];
   const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
-  uint32_t beg = 0, n = 0, nch = 0;
-  if (c < A.n_cols) nch = column_chunks(A, c, &beg, &n);
+  // the count again (the scan needs every column's before the emission)
+  uint32_t nch = 0;
+  if (c < A.n_cols) nch = column_chunks(A, c, [](uint32_t, uint32_t, uint32_t) {});
   uint32_t tot;
   const uint32_t off = block_tot[blockIdx.x] + block_excl_scan(nch, wsum, &tot);
-  for (uint32_t k = 0; k < nch; ++k) {
-    const uint32_t p0 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * k) / nch);
-    const uint32_t p1 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * (k + 1)) / nch);
-    A.chunks[off + k] = make_uint4(c, p0, p1, 0u);
+  if (c < A.n_cols) {
+    (void)column_chunks(A, c, [&](uint32_t k, uint32_t p0, uint32_t p1) {
+      A.chunks[off + k] = make_uint4(c, p0, p1, 0u);
+    });
   }
 }
 

@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
     }
   }
   // wave minimum when the wave's fluid lanes share a tank, then one atomic
-  // per workgroup and tank
+  // per workgroup and tank (on the workgroup's copy of the tank's slots)
   __shared__ unsigned long long s_min[kBlk / 64];
   __shared__ float s_max[kBlk / 64];
   __shared__ uint32_t s_t[kBlk / 64];
@@ -336,8 +336,8 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
   const uint32_t t0 = __shfl(t, leader);
   const bool uniform = act != 0 && __ballot(fluid && t != t0) == 0;
   if (act != 0 && !uniform && fluid) {
-    atomicMin(dt_min + t, cand);
-    atomicMax(vmax + t, __float_as_uint(vm));
+    atomicMin(red_slot(dt_min, t), cand);
+    atomicMax(red_slot(vmax, t), __float_as_uint(vm));
   }
   unsigned long long m = uniform ? cand : ~0ull;
   float mx = uniform ? vm : 0.0f;
@@ -364,8 +364,10 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
           s_t[b] = ~0u;
         }
       }
-      atomicMin(dt_min + s_t[a], mm);
-      atomicMax(vmax + s_t[a], __float_as_uint(xx));
+      // the workgroup's copy of the tank's slots
+      const size_t slot = static_cast<size_t>(s_t[a]) * kRedStripes + (blockIdx.x & (kRedStripes - 1));
+      atomicMin(dt_min + slot, mm);
+      atomicMax(vmax + slot, __float_as_uint(xx));
     }
   }
 }
@@ -395,10 +397,11 @@ __device__ __forceinline__ unsigned long long ordered_bits(double x) {
 }
 
 // Judge of one learned step L against the plain reference P (the current
-// state), both from the start state S, over fluid particles:
-//   err_bits: max |rho_L - rho_P| / rho_P (FP32 bits)
-//   front[0], front[1]: largest x of P and of L (ordered bits)
-//   dv[0] += |v_L - v_P|^2, dv[1] += |v_P - v_S|^2
+// state), both from the start state S, over fluid particles, each result in
+// kRedStripes copies (one atomic per wave on the wave's copy; the host folds):
+//   err_bits[s]: max |rho_L - rho_P| / rho_P (FP32 bits)
+//   front[s], front[kRedStripes + s]: largest x of P and of L (ordered bits)
+//   dv[s] += |v_L - v_P|^2, dv[kRedStripes + s] += |v_P - v_S|^2
 template <int DIM>
 __global__ __launch_bounds__(kBlk) void k_judge(uint32_t n, DevState st,
                                                 const uint32_t* __restrict__ id,
@@ -412,13 +415,14 @@ __global__ __launch_bounds__(kBlk) void k_judge(uint32_t n, DevState st,
                                                 double* __restrict__ dv) {
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   double num = 0.0, den = 0.0;
+  uint32_t eb = 0u;                   // below every non-negative error
+  unsigned long long f0 = 0, f1 = 0;  // below every ordered double
   if (i < n && !(tag[i] & 1u)) {
     const uint32_t j = id[i];
     const double rp = st.rho[i];
-    const float e = static_cast<float>(fabs(rho_l[j] - rp) / rp);
-    atomicMax(err_bits, __float_as_uint(e));
-    atomicMax(front, ordered_bits(st.x[0][i]));
-    atomicMax(front + 1, ordered_bits(x_l[j]));
+    eb = __float_as_uint(static_cast<float>(fabs(rho_l[j] - rp) / rp));
+    f0 = ordered_bits(st.x[0][i]);
+    f1 = ordered_bits(x_l[j]);
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
       const double vp = st.v[k][i];
@@ -431,21 +435,19 @@ __global__ __launch_bounds__(kBlk) void k_judge(uint32_t n, DevState st,
   for (int o = 32; o > 0; o >>= 1) {
     num += __shfl_xor(num, o);
     den += __shfl_xor(den, o);
+    const uint32_t e2 = __shfl_xor(eb, o);
+    const unsigned long long g0 = __shfl_xor(f0, o), g1 = __shfl_xor(f1, o);
+    eb = e2 > eb ? e2 : eb;
+    f0 = g0 > f0 ? g0 : f0;
+    f1 = g1 > f1 ? g1 : f1;
   }
-  __shared__ double red[2][kBlk / 64];
   if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = num;
-    red[1][threadIdx.x >> 6] = den;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0.0, b = 0.0;
-    for (int w = 0; w < kBlk / 64; ++w) {
-      a += red[0][w];
-      b += red[1][w];
-    }
-    atomicAdd(dv, a);
-    atomicAdd(dv + 1, b);
+    const uint32_t s = red_stripe();
+    atomicMax(err_bits + s, eb);
+    atomicMax(front + s, f0);
+    atomicMax(front + kRedStripes + s, f1);
+    atomicAdd(dv + s, num);
+    atomicAdd(dv + kRedStripes + s, den);
   }
 }
 
@@ -1093,11 +1095,11 @@ hipError_t alloc_snapshot(tit_hip_ctx* c, Snapshot& S) {
     TIT_TRY_E(lalloc(m, &S.counter, 4));
     TIT_TRY_E(lalloc(m, &S.ctl, 4));
     TIT_TRY_E(lalloc(m, &S.need, c->n_tanks));
-    TIT_TRY_E(lalloc(m, &S.vmax, c->n_tanks));
+    TIT_TRY_E(lalloc(m, &S.vmax, static_cast<size_t>(c->n_tanks) * kRedStripes));
     TIT_TRY_E(lalloc(m, &S.disp, c->n_tanks));
     TIT_TRY_E(lalloc(m, &S.time, c->n_tanks));
     TIT_TRY_E(lalloc(m, &S.dt, c->n_tanks));
-    TIT_TRY_E(lalloc(m, &S.dt_min, c->n_tanks));
+    TIT_TRY_E(lalloc(m, &S.dt_min, static_cast<size_t>(c->n_tanks) * kRedStripes));
   }
   // the chunk buffer follows the context's schedule
   TIT_TRY_E(lalloc(m, &S.chunks, c->chunk_cap));
@@ -1135,11 +1137,11 @@ hipError_t copy_state(tit_hip_ctx* c, Snapshot& S, bool to_snapshot) {
   TIT_TRY_E(cp(c->counter, S.counter, 4));
   TIT_TRY_E(cp(c->rebuild_ctl, S.ctl, 4));
   TIT_TRY_E(cp(c->tank_need, S.need, nt));
-  TIT_TRY_E(cp(c->vmax, S.vmax, nt));
+  TIT_TRY_E(cp(c->vmax, S.vmax, nt * kRedStripes));
   TIT_TRY_E(cp(c->disp, S.disp, nt));
   TIT_TRY_E(cp(c->time, S.time, nt));
   TIT_TRY_E(cp(c->dt, S.dt, nt));
-  TIT_TRY_E(cp(c->dt_min, S.dt_min, nt));
+  TIT_TRY_E(cp(c->dt_min, S.dt_min, nt * kRedStripes));
   return hipSuccess;
 }
 
@@ -1181,9 +1183,9 @@ tit_status tit_hip_learned_init(tit_hip_ctx* c, uint64_t seed) {
   TIT_HIP_TRY(c, lalloc(L->mem, &L->adam_m, kParams));
   TIT_HIP_TRY(c, lalloc(L->mem, &L->adam_v, kParams));
   TIT_HIP_TRY(c, lalloc(L->mem, &L->loss, 1));
-  TIT_HIP_TRY(c, lalloc(L->mem, &L->err_bits, 1));
-  TIT_HIP_TRY(c, lalloc(L->mem, &L->front, 2));
-  TIT_HIP_TRY(c, lalloc(L->mem, &L->dv, 2));
+  TIT_HIP_TRY(c, lalloc(L->mem, &L->err_bits, kRedStripes));
+  TIT_HIP_TRY(c, lalloc(L->mem, &L->front, 2 * kRedStripes));
+  TIT_HIP_TRY(c, lalloc(L->mem, &L->dv, 2 * kRedStripes));
   L->sc = default_scales(c);
   std::vector<float> w;
   init_weights(w, seed);
@@ -1538,9 +1540,9 @@ tit_status tit_hip_learned_run(tit_hip_ctx* c, int n_steps, int judge_every, dou
     TIT_HIP_TRY(c, restore(c, L->before));
     st = run_plain(c, kPlainPerLearned);
     if (st != TIT_OK) return st;
-    TIT_HIP_TRY(c, hipMemsetAsync(L->err_bits, 0, sizeof(uint32_t), s));
-    TIT_HIP_TRY(c, hipMemsetAsync(L->front, 0, 2 * sizeof(unsigned long long), s));
-    TIT_HIP_TRY(c, hipMemsetAsync(L->dv, 0, 2 * sizeof(double), s));
+    TIT_HIP_TRY(c, hipMemsetAsync(L->err_bits, 0, kRedStripes * sizeof(uint32_t), s));
+    TIT_HIP_TRY(c, hipMemsetAsync(L->front, 0, 2 * kRedStripes * sizeof(unsigned long long), s));
+    TIT_HIP_TRY(c, hipMemsetAsync(L->dv, 0, 2 * kRedStripes * sizeof(double), s));
     if (c->dp.dim == 2) {
       k_judge<2><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, c->tag, L->rho_l, L->x_l,
                                              L->v_l, L->v_s, L->err_bits, L->front, L->dv);
@@ -1548,15 +1550,26 @@ tit_status tit_hip_learned_run(tit_hip_ctx* c, int n_steps, int judge_every, dou
       k_judge<3><<<nblk(c->n), kBlk, 0, s>>>(c->n, c->A, c->id, c->tag, L->rho_l, L->x_l,
                                              L->v_l, L->v_s, L->err_bits, L->front, L->dv);
     }
-    uint32_t eb = 0;
-    unsigned long long fr[2] = {0, 0};
-    double dvs[2] = {0.0, 0.0};
-    TIT_HIP_TRY(c, hipMemcpyAsync(&eb, L->err_bits, 4, hipMemcpyDeviceToHost, s));
-    TIT_HIP_TRY(c, hipMemcpyAsync(fr, L->front, 16, hipMemcpyDeviceToHost, s));
-    TIT_HIP_TRY(c, hipMemcpyAsync(dvs, L->dv, 16, hipMemcpyDeviceToHost, s));
+    uint32_t ebs[kRedStripes];
+    unsigned long long frs[2 * kRedStripes];
+    double dvss[2 * kRedStripes];
+    TIT_HIP_TRY(c, hipMemcpyAsync(ebs, L->err_bits, sizeof(ebs), hipMemcpyDeviceToHost, s));
+    TIT_HIP_TRY(c, hipMemcpyAsync(frs, L->front, sizeof(frs), hipMemcpyDeviceToHost, s));
+    TIT_HIP_TRY(c, hipMemcpyAsync(dvss, L->dv, sizeof(dvss), hipMemcpyDeviceToHost, s));
     TIT_HIP_TRY(c, hipStreamSynchronize(s));
     st = check_launch(c, "judge");
     if (st != TIT_OK) return st;
+    // fold the copies
+    uint32_t eb = 0;
+    unsigned long long fr[2] = {0, 0};
+    double dvs[2] = {0.0, 0.0};
+    for (int k = 0; k < kRedStripes; ++k) {
+      eb = std::max(eb, ebs[k]);
+      fr[0] = std::max(fr[0], frs[k]);
+      fr[1] = std::max(fr[1], frs[kRedStripes + k]);
+      dvs[0] += dvss[k];
+      dvs[1] += dvss[kRedStripes + k];
+    }
     float err;
     std::memcpy(&err, &eb, 4);
     auto unorder = [](unsigned long long b) {
