@@ -158,14 +158,22 @@ __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, cons
 // candidates (a mask and the window's first index each), in LDS.
 constexpr int kPiWords = 12;
 
+// A neighbour's fields, loaded ahead of its turn.
+template <int DIM>
+struct NbFields {
+  double x[DIM], v[DIM], rho, vol;
+};
+
 // The same neighbours in the same order as for_neighbours, for a callback
 // too heavy to run under the acceptance branch: a wave's lanes walk
 // different candidates, so under the branch the wave pays the callback on
-// nearly every candidate iteration.  Here the predicate only fills masks,
-// and the callback runs in one flat loop over the buffered masks: a wave
-// pays it about max-over-lanes(neighbour count) times.  s_m, s_j: this
-// workgroup's kPiWords x kPiBlock buffers.
-template <int DIM, class F>
+// nearly every candidate iteration.  Here the predicate only fills masks
+// (positions loaded a batch at a time), and the callback runs in one flat
+// loop over the buffered masks, a wave paying it about max-over-lanes
+// (neighbour count) times, with the next neighbour's fields (S's x, v, rho;
+// vol) loaded while the current one is summed.  f(j, d, s2, fields).
+// s_m, s_j: this workgroup's kPiWords x NT buffers (NT threads).
+template <int DIM, int NT, class F>
 __device__ __forceinline__ void for_neighbours_packed(const PiArgs& A, uint32_t i,
                                                       const double* xa, const DevState& S,
                                                       uint32_t* s_m, uint32_t* s_j, F&& f) {
@@ -174,19 +182,43 @@ __device__ __forceinline__ void for_neighbours_packed(const PiArgs& A, uint32_t 
   auto drain = [&]() {
     int w = 0;
     uint32_t m = nw > 0 ? s_m[lane] : 0u;
-    for (;;) {
-      while (m == 0u && ++w < nw) m = s_m[w * kPiBlock + lane];
-      if (m == 0u) break;
+    // the next buffered neighbour, if any
+    auto next = [&](uint32_t& j) -> bool {
+      while (m == 0u && ++w < nw) m = s_m[w * NT + lane];
+      if (m == 0u) return false;
       const uint32_t t = static_cast<uint32_t>(__builtin_ctz(m));
       m &= m - 1u;
-      const uint32_t j = s_j[w * kPiBlock + lane] + t;
+      j = s_j[w * NT + lane] + t;
+      return true;
+    };
+    auto load = [&](uint32_t j, NbFields<DIM>& nb) {
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        nb.x[k] = S.x[k][j];
+        nb.v[k] = S.v[k][j];
+      }
+      nb.rho = S.rho[j];
+      nb.vol = A.vol[j];
+    };
+    uint32_t j = 0;
+    bool have = next(j);
+    NbFields<DIM> cur;
+    if (have) load(j, cur);
+    while (have) {
+      uint32_t j2 = 0;
+      const bool have2 = next(j2);
+      NbFields<DIM> nxt;
+      if (have2) load(j2, nxt);
       double d[DIM];
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) d[k] = S.x[k][j] - xa[k];
+      for (int k = 0; k < DIM; ++k) d[k] = cur.x[k] - xa[k];
       double s2 = d[0] * d[0];
 #pragma unroll
       for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
-      f(j, d, s2);
+      f(j, d, s2, cur);
+      j = j2;
+      cur = nxt;
+      have = have2;
     }
     nw = 0;
   };
@@ -198,19 +230,32 @@ __device__ __forceinline__ void for_neighbours_packed(const PiArgs& A, uint32_t 
       for (uint32_t c0 = b; c0 < e; c0 += 32u) {
         const uint32_t len = min(32u, e - c0);
         uint32_t mask = 0u;
-        for (uint32_t t = 0; t < len; ++t) {
-          const uint32_t j = c0 + t;
-          double d[DIM];
+        // kBatch candidates' positions loaded together, then tested (the
+        // loads of one batch overlap; indices past the window's end repeat
+        // its last candidate and are masked out)
+        constexpr uint32_t kBatch = DIM == 2 ? 8u : 4u;
+        for (uint32_t t0 = 0; t0 < len; t0 += kBatch) {
+          double px[kBatch][DIM];
 #pragma unroll
-          for (int k = 0; k < DIM; ++k) d[k] = S.x[k][j] - xa[k];
-          double s2 = d[0] * d[0];
+          for (uint32_t u = 0; u < kBatch; ++u) {
+            const uint32_t j = c0 + min(t0 + u, len - 1u);
 #pragma unroll
-          for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
-          if (s2 <= A.P.R2) mask |= 1u << t;
+            for (int k = 0; k < DIM; ++k) px[u][k] = S.x[k][j];
+          }
+#pragma unroll
+          for (uint32_t u = 0; u < kBatch; ++u) {
+            double d[DIM];
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) d[k] = px[u][k] - xa[k];
+            double s2 = d[0] * d[0];
+#pragma unroll
+            for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
+            if (t0 + u < len && s2 <= A.P.R2) mask |= 1u << (t0 + u);
+          }
         }
         if (mask != 0u) {
-          s_m[nw * kPiBlock + lane] = mask;
-          s_j[nw * kPiBlock + lane] = c0;
+          s_m[nw * NT + lane] = mask;
+          s_j[nw * NT + lane] = c0;
           if (++nw == kPiWords) drain();
         }
       }
@@ -219,15 +264,19 @@ __device__ __forceinline__ void for_neighbours_packed(const PiArgs& A, uint32_t 
   drain();
 }
 
+// f'(q) / q of the unit kernel (f' = -5 q t^3 for Wendland C2,
+// -14/3 q (1 + 5q/2) t^5 for C4, t = 1 - q/2; sph/kernel.gen.cpp:539-546):
+// grad W = w f'(q) / h * r_ab / |r_ab| = (w / h^2) (f'(q) / q) r_ab, with no
+// division by |r_ab|.
 template <int KERNEL>
-__device__ __forceinline__ double unit_deriv(double q) {
+__device__ __forceinline__ double unit_deriv_over_q(double q) {
   if (!(q < 2.0)) return 0.0;
   const double t = 1.0 - q / 2.0;
   if constexpr (KERNEL == kKernelC2) {
-    return -5.0 * q * (t * t * t);
+    return -5.0 * (t * t * t);
   } else {
     const double t5 = t * t * t * t * t;
-    return -14.0 / 3.0 * q * (1.0 + 2.5 * q) * t5;
+    return -14.0 / 3.0 * (1.0 + 2.5 * q) * t5;
   }
 }
 
@@ -321,21 +370,17 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_volume(PiArgs A) {
   if (i < A.n) A.vol[i] = mass_of(A, i) / A.a.rho[i];
 }
 
-// apply_shifts, sums and renormalisation (:335-365)
-template <int DIM, int KERNEL>
-__global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
-  __shared__ uint32_t s_m[kPiWords * kPiBlock], s_j[kPiWords * kPiBlock];
-  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
-  if (i >= A.n) return;
+// apply_shifts, sums and renormalisation (:335-365) of particle i at xa:
+// walk(f) calls f(j, d, s2, fields) for each neighbour j (self included).
+template <int DIM, int KERNEL, class Walk>
+__device__ __forceinline__ void pi_sums_one(const PiArgs& A, uint32_t i, const double* xa,
+                                            Walk&& walk) {
   const DevParams& P = A.P;
-  double xa[DIM], va[DIM];
+  double va[DIM];
 #pragma unroll
-  for (int k = 0; k < DIM; ++k) {
-    xa[k] = A.a.x[k][i];
-    va[k] = A.a.v[k][i];
-  }
+  for (int k = 0; k < DIM; ++k) va[k] = A.a.v[k][i];
   const double rho_a = A.a.rho[i];
-  const double w = kernel_factor<DIM, KERNEL>(P), h_inv = 1.0 / P.h;
+  const double h_inv = 1.0 / P.h, w2 = kernel_factor<DIM, KERNEL>(P) * (h_inv * h_inv);
   double N[DIM] = {}, Lm[DIM * DIM] = {}, G[DIM * DIM] = {}, g[DIM] = {};
   if constexpr (DIM == 2) {
     // the face terms of a particle near a wall (fluid_equations.hpp:342-349)
@@ -356,26 +401,22 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
   }
   const double ga = gamma_of(A, i);
   uint32_t cnt = 0;
-  for_neighbours_packed<DIM>(A, i, xa, A.a, s_m, s_j, [&](uint32_t j, const double* d, double s2) {
+  walk([&](uint32_t j, const double* d, double s2, const NbFields<DIM>& nb) {
     ++cnt;
     if (j == i) return;
-    const double rho_b = A.a.rho[j];
+    const double rho_b = nb.rho;
     // V_b / gamma_a (x / 1 is x: no division away from walls)
-    const double V_b = ga == 1.0 ? A.vol[j] : A.vol[j] / ga;
-    // unit direction of r_ab (zero below the tiny norm, as normalize does)
-    const double r = sqrt(s2);
-    const double inv_r = s2 >= kTiny * kTiny ? 1.0 / r : 0.0;
-    double nx[DIM];
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) nx[k] = -d[k] * inv_r;
-    const double s = w * unit_deriv<KERNEL>(h_inv * r);
+    const double V_b = ga == 1.0 ? nb.vol : nb.vol / ga;
+    // grad W_ab, r_ab = -d (zero below the tiny separation, where the
+    // reference's normalize gives a zero direction)
+    const double s = s2 >= kTiny * kTiny ? w2 * unit_deriv_over_q<KERNEL>(h_inv * sqrt(s2)) : 0.0;
     double gW[DIM];
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) gW[k] = s * (nx[k] * h_inv);
+    for (int k = 0; k < DIM; ++k) gW[k] = s * -d[k];
 #pragma unroll
     for (int p = 0; p < DIM; ++p) {
       N[p] += V_b * gW[p];
-      const double v_ba = A.a.v[p][j] - va[p];
+      const double v_ba = nb.v[p] - va[p];
 #pragma unroll
       for (int q = 0; q < DIM; ++q) {
         Lm[p * DIM + q] += V_b * (d[p] * gW[q]);
@@ -437,6 +478,76 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
   A.count[i] = cnt;
 }
 
+
+// One thread per particle, neighbours from global memory.
+template <int DIM, int KERNEL>
+__global__ __launch_bounds__(kPiBlock) void k_pi_sums(PiArgs A) {
+  __shared__ uint32_t s_m[kPiWords * kPiBlock], s_j[kPiWords * kPiBlock];
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  if (i >= A.n) return;
+  double xa[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) xa[k] = A.a.x[k][i];
+  pi_sums_one<DIM, KERNEL>(A, i, xa, [&](auto&& f) {
+    for_neighbours_packed<DIM, kPiBlock>(A, i, xa, A.a, s_m, s_j, f);
+  });
+}
+
+// Whether a neighbour j != i lies inside the 45-degree cone behind particle
+// i along its normal N (:392-417).  Any one decides, so the order of the
+// walk does not change the answer: each axis of the window is walked from
+// the side the cone opens to (descending where N_k < 0, since the cone
+// opens along -N), so that an interior particle meets one within its
+// first candidates instead of after most of its window.
+template <int DIM>
+__device__ __forceinline__ bool any_in_cone(const PiArgs& A, uint32_t i, const double* xa,
+                                            const double* Na) {
+  const double cos_fov = 0.7071067811865476;  // cos(pi / 4) in FP64
+  const Window w = window_of<DIM>(A, i);
+  const bool rx = Na[0] < 0.0, ry = DIM == 3 && Na[1] < 0.0, rz = Na[DIM - 1] < 0.0;
+  const int nx = w.xhi - w.xlo + 1, ny = w.yhi - w.ylo + 1;
+  for (int a = 0; a < nx; ++a) {
+    const int ix = rx ? w.xhi - a : w.xlo + a;
+    for (int c = 0; c < ny; ++c) {
+      const int iy = ry ? w.yhi - c : w.ylo + c;
+      const uint32_t row = window_row<DIM>(w, ix, iy);
+      const uint32_t b = A.cell_off[row + w.zlo], e = A.cell_off[row + w.zhi + 1];
+      // kBatch candidates loaded together, then tested (indices past the
+      // column's end repeat its last candidate: a repeat decides nothing new)
+      constexpr uint32_t kBatch = 4u;
+      const uint32_t len = e - b;
+      for (uint32_t k0 = 0; k0 < len; k0 += kBatch) {
+        uint32_t jj[kBatch];
+        double px[kBatch][DIM];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+          const uint32_t k = min(k0 + u, len - 1u);
+          jj[u] = rz ? e - 1u - k : b + k;
+#pragma unroll
+          for (int q = 0; q < DIM; ++q) px[u][q] = A.a.x[q][jj[u]];
+        }
+        bool hit = false;
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+          double d[DIM];
+#pragma unroll
+          for (int q = 0; q < DIM; ++q) d[q] = px[u][q] - xa[q];
+          double s2 = d[0] * d[0];
+#pragma unroll
+          for (int q = 1; q < DIM; ++q) s2 = s2 + d[q] * d[q];
+          const double fov = (cos_fov * cos_fov) * s2;
+          double n_a = Na[0] * -d[0];
+#pragma unroll
+          for (int q = 1; q < DIM; ++q) n_a = n_a + Na[q] * -d[q];
+          hit = hit || (jj[u] != i && s2 <= A.P.R2 && n_a > 0.0 && n_a * n_a >= fov);
+        }
+        if (hit) return true;
+      }
+    }
+  }
+  return false;
+}
+
 // Free-surface indicator (:375-426): far when a neighbour lies inside the
 // 45-degree cone behind the particle along N; splashes are on the surface.
 template <int DIM>
@@ -456,19 +567,7 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_phi(PiArgs A) {
       xa[k] = A.a.x[k][i];
       Na[k] = A.N[k][i];
     }
-    const double cos_fov = 0.7071067811865476;  // cos(pi / 4) in FP64
-    for_neighbours<DIM>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) -> bool {
-      if (j == i) return false;
-      const double fov = (cos_fov * cos_fov) * s2;
-      double n_a = Na[0] * -d[0];
-#pragma unroll
-      for (int k = 1; k < DIM; ++k) n_a = n_a + Na[k] * -d[k];
-      if (n_a > 0.0 && n_a * n_a >= fov) {
-        phi = kPhiMax;
-        return true;  // any one neighbour in the cone decides
-      }
-      return false;
-    });
+    if (any_in_cone<DIM>(A, i, xa, Na)) phi = kPhiMax;
   }
   A.phi[i] = phi;
   if (same_bits(phi, DBL_MIN)) {
@@ -530,8 +629,8 @@ __device__ __forceinline__ Share share(uint32_t n) {
 
 // Near-surface classification and the shifts (:439-470), into B.
 template <int DIM>
-__device__ __forceinline__ void shift_one(const PiArgs& A, uint32_t i,
-                                          TankRun<true, unsigned long long>& run) {
+__device__ __forceinline__ void shift_one(const PiArgs& A, uint32_t i, bool& shifted,
+                                          unsigned long long& shift_bits) {
   const DevParams& P = A.P;
   double xa[DIM], va[DIM];
 #pragma unroll
@@ -595,7 +694,8 @@ __device__ __forceinline__ void shift_one(const PiArgs& A, uint32_t i,
       d2 += dr[k] * dr[k];
     }
     rho += drho;
-    run.add(A.tag[i] >> 1, dbl_bits(sqrt(d2)), A.shift_max);
+    shifted = true;
+    shift_bits = dbl_bits(sqrt(d2));
   }
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
@@ -605,15 +705,15 @@ __device__ __forceinline__ void shift_one(const PiArgs& A, uint32_t i,
   A.b.rho[i] = rho;
 }
 
+// One particle per thread; the largest shift goes to the tank's slots with
+// one atomic per wave.
 template <int DIM>
 __global__ __launch_bounds__(kPiBlock) void k_pi_shift(PiArgs A) {
-  const Share sh = share(A.n);
-  TankRun<true, unsigned long long> run;
-  for (uint32_t base = sh.beg; base < sh.end; base += kPiBlock) {
-    const uint32_t i = base + threadIdx.x;
-    if (i < sh.end) shift_one<DIM>(A, i, run);
-  }
-  run.flush(A.shift_max);
+  const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
+  bool shifted = false;
+  unsigned long long bits = 0ull;
+  if (i < A.n) shift_one<DIM>(A, i, shifted, bits);
+  tank_atomic<true>(shifted, shifted ? A.tag[i] >> 1 : 0u, bits, A.shift_max);
 }
 
 // apply_free_surface_correction (:483-512): on the neighbour sets of the
@@ -671,6 +771,9 @@ __global__ __launch_bounds__(64) void k_pi_bound(unsigned long long* shift_max, 
   if (!(d <= skin_half)) atomicOr(flags, 1u);
 }
 
+// Persistent: a lane walks a contiguous run of particles and keeps running
+// per-tank values, so the grid's flushes stay few (a one-particle-per-thread
+// form, with two atomics per wave over 31k waves, measured 2.6x slower).
 template <int DIM>
 __global__ __launch_bounds__(kPiBlock) void k_pi_commit(PiArgs A) {
   const Share sh = share(A.n);
@@ -737,10 +840,8 @@ void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* 
   if (a.n == 0) return;
   const uint32_t nb = (a.n + kPiBlock - 1) / kPiBlock;
   const int dim = a.P.dim, kernel = a.P.kernel;
-  // the reducing passes run persistent, as many workgroups as stay resident
+  // the commit pass runs persistent, as many workgroups as stay resident
   const uint32_t cus = static_cast<uint32_t>(std::max(n_cu, 1));
-  const uint32_t np_shift =
-      std::min(nb, cus * (dim == 2 ? resident_per_cu<k_pi_shift<2>>() : resident_per_cu<k_pi_shift<3>>()));
   const uint32_t np_commit =
       std::min(nb, cus * (dim == 2 ? resident_per_cu<k_pi_commit<2>>() : resident_per_cu<k_pi_commit<3>>()));
   // cell_mark and cell_near lie back to back, n_cells + 1 entries each
@@ -754,7 +855,7 @@ void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* 
     }
     k_pi_clear<<<(n_marks + kPiBlock - 1) / kPiBlock, kPiBlock, 0, s>>>(n_marks, a.cell_mark);
     k_pi_phi<2><<<nb, kPiBlock, 0, s>>>(a);
-    k_pi_shift<2><<<np_shift, kPiBlock, 0, s>>>(a);
+    k_pi_shift<2><<<nb, kPiBlock, 0, s>>>(a);
     if (kernel == kKernelC2) {
       k_pi_fsc<2, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
     } else {
@@ -768,7 +869,7 @@ void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* 
     }
     k_pi_clear<<<(n_marks + kPiBlock - 1) / kPiBlock, kPiBlock, 0, s>>>(n_marks, a.cell_mark);
     k_pi_phi<3><<<nb, kPiBlock, 0, s>>>(a);
-    k_pi_shift<3><<<np_shift, kPiBlock, 0, s>>>(a);
+    k_pi_shift<3><<<nb, kPiBlock, 0, s>>>(a);
     if (kernel == kKernelC2) {
       k_pi_fsc<3, kKernelC2><<<nb, kPiBlock, 0, s>>>(a);
     } else {

@@ -14,6 +14,9 @@ constexpr int kStageLdsCap = 1024;
 // split by empty cells, such as fluid below and a wall's vertices at the
 // top), so that every chunk's window fits the LDS kernel's cell rows.
 constexpr int kChunkRows = 92;
+// Window columns of a 2D chunk and LDS cell-offset rows of its window.
+constexpr int kCols2 = 2 * kStencil + 1;
+constexpr int kRowsMax = kChunkRows + 2 * kStencil;
 
 struct StageArgs {
   DevParams P;

@@ -28,8 +28,6 @@ namespace tit_amd {
 
 namespace {
 
-constexpr int kCols2 = 2 * kStencil + 1;
-constexpr int kRowsMax = 96;   // LDS cell-offset rows of one chunk
 constexpr int kWords = 5;      // acceptance masks held per lane (LDS path)
 constexpr int kWordsG = 24;    // 32-candidate masks held per lane (global path)
 constexpr uint32_t kWordSpan = 1u << 22;  // candidate global-index span of an LDS chunk
@@ -884,8 +882,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 // ---- chunk list (after every cell rebuild) ----------------------------------
 
 constexpr int kScanBlock = 1024;
-
-static_assert(kChunkRows + 2 * kStencil == kRowsMax, "a chunk's window rows fill the LDS rows");
 
 // The chunks of column col_i, in order: the column's particles in
 // ceil(n / chunk_targets) equal parts, each cut further into pieces whose
