@@ -290,6 +290,30 @@ def test_escaped_particles_keep_exact_sets(dim, n, radius):
     ctx.close()
 
 
+def test_gapped_columns_are_cut_and_sum_exact_sets():
+    """A tall tank (300 cell rows) with a fluid block at the bottom and a thin
+    layer 2.6 m above it: each column's 140 particles fall into two equal
+    parts of the chunk schedule, the second spanning the gap, which the
+    builder cuts where it passes 92 cells.  Every particle is still summed
+    over exactly its brute-force neighbour set (a lost or doubled piece would
+    leave targets with no pairs or twice their pairs)."""
+    dr = 0.005
+    xs = (np.arange(40) + 0.5) * dr
+    low = np.stack(np.meshgrid(xs, (np.arange(60) + 1.0) * dr, indexing="ij"), -1).reshape(-1, 2)
+    high = np.stack(np.meshgrid(xs, 2.9 + np.arange(10) * dr, indexing="ij"), -1).reshape(-1, 2)
+    r = np.concatenate([low, high])
+    n = len(r)
+    pa = ta.ParticleArray(2, r, np.full(n, 1000.0), n, np.zeros((0, 2)), (0.0, 0.0), (0.2, 3.0))
+    eq = _eq(2, dr)
+    ctx = _ctx(eq, [pa])
+    assert ctx.get_schedule()["chunk_targets"] == 128
+    want = oracle.naive_search(pa.r, 2.0 * eq.h)
+    assert csr_equal(ctx.neighbors(), want)
+    counts = ctx.pair_counts().astype(np.int64)
+    assert np.array_equal(counts, np.diff(want[0].astype(np.int64)) - 1)
+    ctx.close()
+
+
 def test_empty_upload_rejected():
     eq = _eq(2, 0.02)
     ctx = ta.HipContext(eq)
