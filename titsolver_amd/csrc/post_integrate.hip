@@ -26,7 +26,6 @@
 //            displacement bound of the cell index.
 #include <algorithm>
 #include <cfloat>
-#include <type_traits>
 
 #include "pair_math.hpp"
 #include "sph_kernels.hpp"
@@ -119,9 +118,9 @@ __device__ __forceinline__ uint32_t window_row(const Window& w, int ix, int iy) 
 }
 
 // Calls f(j, d, s2) for every particle j (self included) with
-// s2 = |r_j - r_a|^2 <= (2h)^2, d = r_j - r_a, positions from S; a callback
-// that returns bool stops the walk by returning true.  With ONLY_MARKED,
-// only the stencil columns with a marked cell are visited.
+// s2 = |r_j - r_a|^2 <= (2h)^2, d = r_j - r_a, positions from S, in column
+// order and ascending j.  With ONLY_MARKED, only the stencil columns with a
+// marked cell are visited.
 template <int DIM, bool ONLY_MARKED = false, class F>
 __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, const double* xa,
                                                const DevState& S, F&& f) {
@@ -143,12 +142,7 @@ __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, cons
         double s2 = d[0] * d[0];
 #pragma unroll
         for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
-        if (!(s2 <= A.P.R2)) continue;
-        if constexpr (std::is_same_v<decltype(f(j, d, s2)), bool>) {
-          if (f(j, d, s2)) return;
-        } else {
-          f(j, d, s2);
-        }
+        if (s2 <= A.P.R2) f(j, d, s2);
       }
     }
   }
