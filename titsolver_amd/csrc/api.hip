@@ -469,6 +469,11 @@ void enqueue_post_integrate(tit_hip_ctx* c) {
   a.phi = p;
   a.phi_near = p + n;
   a.vol = p + 2 * n;
+  if (c->pi_pack) {
+    a.pk_x = c->pi_pack;
+    a.pk_v = c->pi_pack + n;
+    a.pk_rv = c->pi_pack + 2 * n;
+  }
   a.count = c->pi_count;
   a.cell_mark = c->pi_cells;
   a.cell_near = c->pi_cells + c->n_cells + 1;
@@ -831,6 +836,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   if (c->prm.post_integrate) {
     const size_t per = static_cast<size_t>(3 * dim + dim * dim + 3);
     TIT_HIP_TRY(c, dalloc(c, &c->pi_buf, per * n));
+    if (dim == 2) TIT_HIP_TRY(c, dalloc(c, &c->pi_pack, 3 * static_cast<size_t>(n)));
     TIT_HIP_TRY(c, dalloc(c, &c->pi_count, n));
     TIT_HIP_TRY(c, dalloc(c, &c->pi_cells, 2 * (static_cast<size_t>(c->n_cells) + 1)));
     TIT_HIP_TRY(c, dalloc(c, &c->pi_shift, n_red));

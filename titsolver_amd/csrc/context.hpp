@@ -65,6 +65,7 @@ struct tit_hip_ctx {
   // post-integration scratch (allocated when params.post_integrate is set);
   // rhs_dv holds the last stage's acceleration between steps then
   double* pi_buf = nullptr;  // N, dr0, grad v, grad rho, phi, phi_near (SoA)
+  double2* pi_pack = nullptr;  // 2D: packed x, v, (rho, vol), n each
   uint32_t* pi_count = nullptr;
   uint32_t* pi_cells = nullptr;  // per cell: surface mark, then window mark
   unsigned long long* pi_shift = nullptr;  // per tank

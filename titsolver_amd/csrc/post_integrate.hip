@@ -117,6 +117,21 @@ __device__ __forceinline__ uint32_t window_row(const Window& w, int ix, int iy) 
                   : w.cell_off + (static_cast<uint32_t>(ix) * w.nc1 + static_cast<uint32_t>(iy)) * w.nc2;
 }
 
+// Position of particle j in state S (= A.a): one 16-byte load of the packed
+// copy in 2D.
+template <int DIM>
+__device__ __forceinline__ void pos_of(const PiArgs& A, const DevState& S, uint32_t j,
+                                       double* x) {
+  if constexpr (DIM == 2) {
+    const double2 q = A.pk_x[j];
+    x[0] = q.x;
+    x[1] = q.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) x[k] = S.x[k][j];
+  }
+}
+
 // Calls f(j, d, s2) for every particle j (self included) with
 // s2 = |r_j - r_a|^2 <= (2h)^2, d = r_j - r_a, positions from S, in column
 // order and ascending j.  With ONLY_MARKED, only the stencil columns with a
@@ -137,8 +152,9 @@ __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, cons
       const uint32_t b = A.cell_off[row + zlo], e = A.cell_off[row + zhi + 1];
       for (uint32_t j = b; j < e; ++j) {
         double d[DIM];
+        pos_of<DIM>(A, S, j, d);
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) d[k] = S.x[k][j] - xa[k];
+        for (int k = 0; k < DIM; ++k) d[k] = d[k] - xa[k];
         double s2 = d[0] * d[0];
 #pragma unroll
         for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
@@ -165,7 +181,8 @@ struct NbFields {
 // (positions loaded a batch at a time), and the callback runs in one flat
 // loop over the buffered masks, a wave paying it about max-over-lanes
 // (neighbour count) times, with the next neighbour's fields (S's x, v, rho;
-// vol) loaded while the current one is summed.  f(j, d, s2, fields).
+// vol; in 2D from the packed copies of A, which hold S = A.a) loaded while
+// the current one is summed.  f(j, d, s2, fields).
 // s_m, s_j: this workgroup's kPiWords x NT buffers (NT threads).
 template <int DIM, int NT, class F>
 __device__ __forceinline__ void for_neighbours_packed(const PiArgs& A, uint32_t i,
@@ -186,13 +203,23 @@ __device__ __forceinline__ void for_neighbours_packed(const PiArgs& A, uint32_t 
       return true;
     };
     auto load = [&](uint32_t j, NbFields<DIM>& nb) {
+      if constexpr (DIM == 2) {
+        const double2 x = A.pk_x[j], v = A.pk_v[j], rv = A.pk_rv[j];
+        nb.x[0] = x.x;
+        nb.x[1] = x.y;
+        nb.v[0] = v.x;
+        nb.v[1] = v.y;
+        nb.rho = rv.x;
+        nb.vol = rv.y;
+      } else {
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        nb.x[k] = S.x[k][j];
-        nb.v[k] = S.v[k][j];
+        for (int k = 0; k < DIM; ++k) {
+          nb.x[k] = S.x[k][j];
+          nb.v[k] = S.v[k][j];
+        }
+        nb.rho = S.rho[j];
+        nb.vol = A.vol[j];
       }
-      nb.rho = S.rho[j];
-      nb.vol = A.vol[j];
     };
     uint32_t j = 0;
     bool have = next(j);
@@ -231,11 +258,7 @@ __device__ __forceinline__ void for_neighbours_packed(const PiArgs& A, uint32_t 
         for (uint32_t t0 = 0; t0 < len; t0 += kBatch) {
           double px[kBatch][DIM];
 #pragma unroll
-          for (uint32_t u = 0; u < kBatch; ++u) {
-            const uint32_t j = c0 + min(t0 + u, len - 1u);
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) px[u][k] = S.x[k][j];
-          }
+          for (uint32_t u = 0; u < kBatch; ++u) pos_of<DIM>(A, S, c0 + min(t0 + u, len - 1u), px[u]);
 #pragma unroll
           for (uint32_t u = 0; u < kBatch; ++u) {
             double d[DIM];
@@ -361,7 +384,14 @@ __device__ __forceinline__ double gamma_of(const PiArgs& A, uint32_t i) {
 
 __global__ __launch_bounds__(kPiBlock) void k_pi_volume(PiArgs A) {
   const uint32_t i = blockIdx.x * kPiBlock + threadIdx.x;
-  if (i < A.n) A.vol[i] = mass_of(A, i) / A.a.rho[i];
+  if (i >= A.n) return;
+  const double rho = A.a.rho[i], vol = mass_of(A, i) / rho;
+  A.vol[i] = vol;
+  if (A.pk_x) {  // 2D
+    A.pk_x[i] = make_double2(A.a.x[0][i], A.a.x[1][i]);
+    A.pk_v[i] = make_double2(A.a.v[0][i], A.a.v[1][i]);
+    A.pk_rv[i] = make_double2(rho, vol);
+  }
 }
 
 // apply_shifts, sums and renormalisation (:335-365) of particle i at xa:
@@ -517,8 +547,7 @@ __device__ __forceinline__ bool any_in_cone(const PiArgs& A, uint32_t i, const d
         for (uint32_t u = 0; u < kBatch; ++u) {
           const uint32_t k = min(k0 + u, len - 1u);
           jj[u] = rz ? e - 1u - k : b + k;
-#pragma unroll
-          for (int q = 0; q < DIM; ++q) px[u][q] = A.a.x[q][jj[u]];
+          pos_of<DIM>(A, A.a, jj[u], px[u]);
         }
         bool hit = false;
 #pragma unroll

@@ -78,6 +78,11 @@ struct PiArgs {
   uint32_t* cell_near;   // per cell: its 5^d window holds a surface particle
                          // (cell_mark + n_cells + 1: the two are cleared together)
   double* vol;           // m / rho per particle
+  // 2D: (x, y), (v_x, v_y), (rho, m / rho) per particle, packed by the volume
+  // pass so that a neighbour's fields come in three 16-byte loads
+  double2* pk_x;
+  double2* pk_v;
+  double2* pk_rv;
   uint32_t n_cells;
   unsigned long long* shift_max;  // per tank, FP64 bits (kRedStripes copies per tank)
   double* dv_last[3];    // the last stage's acceleration
