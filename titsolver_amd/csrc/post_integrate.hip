@@ -11,6 +11,8 @@
 // the neighbour sets are those of the post-step positions (the exact
 // predicate over the 5^d cell stencil, the index covering them by the skin
 // bound of the step).  Passes:
+//   volume   V = m / rho per particle; in 2D also packed double2 copies of
+//            (x, y), (v_x, v_y) and (rho, V), which the walks below read
 //   sums     N = sum V_b grad W_ab, L = sum V_b r_ba (x) grad W_ab, grad v,
 //            grad rho (every particle); renormalise by (L^T)^-1; N normalised
 //   phi      free-surface indicator: a fluid particle is "far" (1) when a
