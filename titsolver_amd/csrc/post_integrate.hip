@@ -136,31 +136,33 @@ __device__ __forceinline__ void pos_of(const PiArgs& A, const DevState& S, uint3
 
 // Calls f(j, d, s2) for every particle j (self included) with
 // s2 = |r_j - r_a|^2 <= (2h)^2, d = r_j - r_a, positions from S, in column
-// order and ascending j.  With ONLY_MARKED, only the stencil columns with a
-// marked cell are visited.
+// order and ascending j.  With ONLY_MARKED, only the particles of marked
+// cells are visited (still in ascending j).
 template <int DIM, bool ONLY_MARKED = false, class F>
 __device__ __forceinline__ void for_neighbours(const PiArgs& A, uint32_t i, const double* xa,
                                                const DevState& S, F&& f) {
   const Window w = window_of<DIM>(A, i);
-  const int zlo = w.zlo, zhi = w.zhi;
+  auto visit = [&](uint32_t b, uint32_t e) {
+    for (uint32_t j = b; j < e; ++j) {
+      double d[DIM];
+      pos_of<DIM>(A, S, j, d);
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) d[k] = d[k] - xa[k];
+      double s2 = d[0] * d[0];
+#pragma unroll
+      for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
+      if (s2 <= A.P.R2) f(j, d, s2);
+    }
+  };
   for (int ix = w.xlo; ix <= w.xhi; ++ix) {
     for (int iy = w.ylo; iy <= w.yhi; ++iy) {
       const uint32_t row = window_row<DIM>(w, ix, iy);
       if constexpr (ONLY_MARKED) {
-        bool any = false;
-        for (int z = zlo; z <= zhi; ++z) any = any || A.cell_mark[row + z] != 0u;
-        if (!any) continue;
-      }
-      const uint32_t b = A.cell_off[row + zlo], e = A.cell_off[row + zhi + 1];
-      for (uint32_t j = b; j < e; ++j) {
-        double d[DIM];
-        pos_of<DIM>(A, S, j, d);
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) d[k] = d[k] - xa[k];
-        double s2 = d[0] * d[0];
-#pragma unroll
-        for (int k = 1; k < DIM; ++k) s2 = s2 + d[k] * d[k];
-        if (s2 <= A.P.R2) f(j, d, s2);
+        for (int z = w.zlo; z <= w.zhi; ++z) {
+          if (A.cell_mark[row + z] != 0u) visit(A.cell_off[row + z], A.cell_off[row + z + 1]);
+        }
+      } else {
+        visit(A.cell_off[row + w.zlo], A.cell_off[row + w.zhi + 1]);
       }
     }
   }
@@ -673,7 +675,7 @@ __device__ __forceinline__ void shift_one(const PiArgs& A, uint32_t i, bool& shi
     // order, so equal distances go to the smaller index
     uint32_t best = ~0u, best_id = ~0u;
     double best_d2 = 0.0, bx[DIM] = {};
-    // only columns with a surface particle's cell can hold one
+    // only the cells holding a surface particle can hold one
     for_neighbours<DIM, true>(A, i, xa, A.a, [&](uint32_t j, const double* d, double s2) {
       if (!same_bits(A.phi[j], DBL_MIN)) return;
       const uint32_t jid = A.id[j];
