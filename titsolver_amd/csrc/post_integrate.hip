@@ -441,16 +441,22 @@ __device__ __forceinline__ void pi_sums_one(const PiArgs& A, uint32_t i, const d
     double gW[DIM];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) gW[k] = s * -d[k];
+    // V_b grad W once: the sums below are V_b (x grad W) re-associated as
+    // x (V_b grad W) (rounding differs from the reference's order by an ulp)
+    double Vg[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) Vg[k] = V_b * gW[k];
+    const double drho = rho_b - rho_a;
 #pragma unroll
     for (int p = 0; p < DIM; ++p) {
-      N[p] += V_b * gW[p];
+      N[p] += Vg[p];
       const double v_ba = nb.v[p] - va[p];
 #pragma unroll
       for (int q = 0; q < DIM; ++q) {
-        Lm[p * DIM + q] += V_b * (d[p] * gW[q]);
-        G[p * DIM + q] += V_b * (v_ba * gW[q]);
+        Lm[p * DIM + q] += d[p] * Vg[q];
+        G[p * DIM + q] += v_ba * Vg[q];
       }
-      g[p] += V_b * (rho_b - rho_a) * gW[p];
+      g[p] += drho * Vg[p];
     }
   });
   // renormalisation (:354-365)
