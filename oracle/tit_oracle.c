@@ -869,7 +869,9 @@ int orc_compute_continuity(const orc_params* prm, orc_state* s,
         const double cs_ab = s->cs[a] > s->cs[b] ? s->cs[a] : s->cs[b];
         const double rn = norm(dim, r_ab);
         double Psi[3] = {0.0, 0.0, 0.0}, ta[3] = {0.0, 0.0, 0.0}, tb[3] = {0.0, 0.0, 0.0};
-        for (int k = 0; k < dim; ++k) Psi[k] = cs_ab * rho_ab * r_ab[k] / rn;
+        double psi_sign = 1.0;
+        if (g_mutation == 12) psi_sign = -1.0; /* planted: anti-diffusive Psi */
+        for (int k = 0; k < dim; ++k) Psi[k] = psi_sign * cs_ab * rho_ab * r_ab[k] / rn;
         for (int k = 0; k < dim; ++k) {
           ta[k] = v_ab[k] + Psi[k] / s->rho[b];
           tb[k] = -v_ab[k] + Psi[k] / s->rho[a];
@@ -926,7 +928,8 @@ int orc_compute_momentum(const orc_params* prm, orc_state* s,
           s->p[a] / pow2(s->rho[a]) + s->p[b] / pow2(s->rho[b]);
       double Pi_ab;
       if (prm->flavour == ORC_FLAVOUR_REF) {
-        Pi_ab = 2.0 * prm->mu * dot(dim, v_ab, r_ab) /
+        const double two = g_mutation == 13 ? 1.0 : 2.0; /* 13 planted: factor 2 lost */
+        Pi_ab = two * prm->mu * dot(dim, v_ab, r_ab) /
                 (s->rho[a] * s->rho[b] * norm2(dim, r_ab));
       } else {
         /* Monaghan (1992): Pi = -alpha c_ab h mu_ab / rho_ab for approaching

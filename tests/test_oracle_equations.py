@@ -193,6 +193,95 @@ def test_artificial_viscosity_only_on_approaching_pairs(mutate):
         assert diff == 0.0
 
 
+def _first_radial_moment(kernel, dim, h):
+    """M1 = integral of |y| W(y) over R^dim, by quadrature of the unit kernel
+    (itself pinned by the special values and unit integral of the kernel
+    tests)."""
+    from scipy import integrate
+    lib = oracle.load()
+    w = lambda q: lib.orc_kernel_unit_value(kernel, q)
+    area = 2.0 * math.pi if dim == 2 else 4.0 * math.pi
+    omega = lib.orc_kernel_weight(kernel, dim)
+    val = integrate.quad(lambda q: q ** dim * w(q), 0.0, 2.0, epsabs=1e-15, epsrel=1e-13)[0]
+    return area * omega * h * val
+
+
+@pytest.mark.parametrize("dim,kernel", [(2, ta.KERNEL_WENDLAND_C2), (2, ta.KERNEL_WENDLAND_C4),
+                                        (3, ta.KERNEL_WENDLAND_C2)])
+@pytest.mark.parametrize("mutate", [0, 12])
+def test_ferrari_diffusion_of_a_quadratic_density(dim, kernel, mutate):
+    """REF continuity at rest: drho_a/dt = sum_b V_b cs_ab (rho_a - rho_b) F(r_ab)
+    with grad W = F(r) r_hat (fluid_equations.hpp:253-257).  For rho = rho0 +
+    c |x - x0|^2 the even part rho_a - rho_b = -c |y|^2 survives the lattice's
+    symmetry, and integral |y|^2 F(|y|) dy = -(d + 1) M1 by parts, so the term
+    tends to (d + 1) c cs_a M1, M1 the kernel's first radial moment.  The
+    residue comes from cs_ab = max(cs_a, cs_b) and V_b = m / rho_b varying at
+    relative order (rho - rho0)/rho0 <= 1e-4, and from lattice quadrature."""
+    dr = 0.01
+    n = 40 if dim == 2 else 18
+    r = _lattice(dim, n, dr)
+    eq = _eq(dim, dr, ta.FLAVOUR_REF, kernel)
+    x0 = r.mean(0)
+    d2 = ((r - x0) ** 2).sum(1)
+    c = 0.1 / d2.max()  # rho varies by 1e-4 rho0 at most
+    rho = 1000.0 + c * d2
+    oracle.set_mutation(mutate)
+    st = _rhs(eq, r, rho, len(r))
+    inner = _interior(r, dr, n)
+    prm = oracle.params(eq)
+    lib = oracle.load()
+    import ctypes as C
+    cs = np.array([lib.orc_eos_sound_speed(C.byref(prm), x) for x in rho[inner]])
+    want = (dim + 1) * c * cs * _first_radial_moment(kernel, dim, eq.h)
+    got = st.drho_dt[inner]
+    if mutate:
+        assert np.all(got < 0)  # anti-diffusive Psi is caught
+    else:
+        np.testing.assert_allclose(got, want, rtol=5e-3)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("mutate", [0, 13])
+def test_physical_viscosity_of_a_quadratic_velocity(dim, mutate):
+    """REF momentum with uniform rho0 (p = 0): the viscous pair force
+    sum_b m_b 2 mu (v_ab.r_ab) / (rho_a rho_b |r_ab|^2) grad W_ab
+    (fluid_equations.hpp:299-300).  For a quadratic field only the odd part
+    -1/2 y.H[y, y] of v_ab.y survives, the isotropic fourth moment of
+    F(|y|)/|y|^3 gives -1/(d + 2) after integration by parts, and the force
+    tends to mu/rho (lap v + 2 grad div v)/(d + 2), whatever the kernel.
+    The lattice sum approaches that limit as (dr/h)^4 (h = 2 dr: 2-5 %), so
+    the supports here are wider: h = 4 dr in 2D, 3 dr in 3D."""
+    dr = 0.01
+    hf, n, margin = (4.0, 40, 9) if dim == 2 else (3.0, 20, 7)
+    r = _lattice(dim, n, dr)
+    eq = ta.cfg_equations(dim, dr, 0.6, kernel=ta.KERNEL_WENDLAND_C4, h_factor=hf,
+                          flavour=ta.FLAVOUR_REF)
+    eq.mu = 1.0
+    x = r - r.mean(0)
+    A, B, Cc = 40.0, -25.0, 30.0
+    v = np.zeros_like(r)
+    v[:, 0] = A * x[:, 0] ** 2 + B * x[:, 0] * x[:, 1]
+    v[:, 1] = Cc * x[:, 1] ** 2
+    # lap v = (2A, 2C); div v = 2A x + B y + 2C y, grad div = (2A, B + 2C)
+    lap = np.array([2 * A, 2 * Cc] + [0.0] * (dim - 2))
+    grad_div = np.array([2 * A, B + 2 * Cc] + [0.0] * (dim - 2))
+    if dim == 3:
+        v[:, 2] = A * x[:, 2] ** 2 - B * x[:, 0] * x[:, 2]
+        lap[2] = 2 * A
+        grad_div += np.array([-B, 0.0, 2 * A])
+    oracle.set_mutation(mutate)
+    st = _rhs(eq, r, np.full(len(r), 1000.0), len(r), v)
+    inner = _interior(r, dr, n, margin)
+    got = st.dv_dt[inner].copy()
+    got[:, 1] += eq.g
+    want = eq.mu / 1000.0 * (lap + 2.0 * grad_div) / (dim + 2)
+    tol = 1e-2 * np.abs(want).max()  # measured 0.18 % (2D), 0.25 % (3D)
+    if mutate:
+        assert np.abs(got - want).max() > 0.3 * np.abs(want).max()
+    else:
+        np.testing.assert_allclose(got, np.broadcast_to(want, got.shape), rtol=0, atol=tol)
+
+
 # ---- boundary extrapolation ---------------------------------------------------------
 
 @pytest.mark.parametrize("kernel", [ta.KERNEL_WENDLAND_C2, ta.KERNEL_WENDLAND_C4])
