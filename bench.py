@@ -77,7 +77,10 @@ def parse():
                     help="eager steps with per-launch HIP events after the "
                          "timed region (kernel durations)")
     ap.add_argument("--cpu-steps", type=int, default=2,
-                    help="oracle steps timed for the CPU baseline (0: skip)")
+                    help="steps of the one-thread CPU baseline leg (0: skip)")
+    ap.add_argument("--parity-steps", type=int, default=20,
+                    help="steps of the threaded CPU baseline timed on every host "
+                         "thread and compared with the GPU (wall-free workloads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0,
                     help="stage-kernel targets per workgroup chunk (0: default)")
@@ -211,8 +214,8 @@ def main():
     if args.copy_probe_mb > 0:
         out["copy_probe"] = copy_probe(args.copy_probe_mb)
     if not args.no_cpu and args.cpu_steps > 0 and len(arrays) == 1:
-        out["cpu_baseline"], out["parity"] = cpu_baseline(pa, eq, args.cpu_steps,
-                                                          args.post_integrate, walls)
+        out["cpu_baseline"], out["parity"] = cpu_baseline(
+            pa, eq, args.cpu_steps, args.post_integrate, walls, args.parity_steps)
     print(json.dumps(out), flush=True)
 
 
@@ -301,17 +304,8 @@ def build_workload(args):
 
 
 def seeded_tanks(seed: int, dr: float, n_tanks: int = 0, min_fluid: int = 0):
-    """Dam breaks in the 3.22 x 2.4 m tank with seeded columns
-    U[0.3, 1.2] x U[0.5, 2.0] m (the batched configuration); n_tanks of them,
-    or as many as needed to reach min_fluid fluid particles."""
     import titsolver_amd as ta
-    rng = np.random.default_rng(seed)
-    arrays, nf = [], 0
-    while (n_tanks and len(arrays) < n_tanks) or (not n_tanks and nf < min_fluid):
-        col = [float(rng.uniform(0.3, 1.2)), float(rng.uniform(0.5, 2.0))]
-        arrays.append(ta.dam_break(2, dr, TANK, col))
-        nf += arrays[-1].n_fluid
-    return arrays
+    return ta.seeded_tanks(seed, dr, n_tanks=n_tanks, min_fluid=min_fluid, tank=TANK)
 
 
 def learned_leg(args):
@@ -395,39 +389,101 @@ def learned_leg(args):
     return res
 
 
-def cpu_baseline(pa, eq, steps, post_integrate=False, walls=None):
-    """FP64 oracle (single thread) on the same workload for a few steps, and
-    the GPU's error against it after the same steps from the same state."""
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: the job's CPU share (OMP_NUM_THREADS
+    when the host sets it, else every core), at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pa, eq, steps, post_integrate=False, walls=None, parity_steps=20):
+    """The reference's CPU path as restated here, timed on this host in the
+    same job, and the GPU's error against it over the same steps from the
+    same state.
+
+    Wall-free workloads: the threaded baseline (oracle/cpu_step.cpp, the
+    oracle's step on a std::thread pool, -O3 AVX2) runs `parity_steps` steps
+    on every host thread of the job (the reported value), then `steps` steps
+    on one thread from the same start; the GPU runs `parity_steps` steps from
+    that start and is compared with the threaded result.  With walls or
+    post-integration only the single-threaded oracle covers the step, and
+    `steps` of it are timed and compared."""
     import oracle
     import titsolver_amd as ta
 
     nf = pa.n_fluid
-    print(f"bench: CPU oracle {steps} steps on {pa.n} particles", file=sys.stderr,
-          flush=True)
     prm = oracle.params(eq, walls)
     prm.post_integrate = 1 if post_integrate else 0
-    st = oracle.State.from_particles(pa)
-    oracle.initialize(prm, st)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        oracle.ssprk_step(prm, st, 3)
-    wall = time.perf_counter() - t0
-    cpu = {"value": nf * steps / wall, "unit": "fluid particle-steps/s", "cores": 1,
-           "kind": "port",
-           "sample": f"{steps} SSPRK3 steps of the same {nf}-fluid-particle dam break "
-                     f"(FP64 oracle, grid search per stage), {wall:.1f} s"}
+    threaded = walls is None and not post_integrate
+    legs = {}
+    if threaded:
+        threads = cpu_threads()
+        st = oracle.State.from_particles(pa)
+        oracle.cpu_set_threads(threads)
+        print(f"bench: CPU baseline, {threads} threads, {parity_steps} steps on {pa.n} "
+              f"particles", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for _ in range(parity_steps):
+            oracle.cpu_ssprk_step(prm, st, 3)
+        wall = time.perf_counter() - t0
+        legs["threads"] = (threads, parity_steps, wall)
+        one = oracle.State.from_particles(pa)
+        oracle.cpu_set_threads(1)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            oracle.cpu_ssprk_step(prm, one, 3)
+        legs["one"] = (1, steps, time.perf_counter() - t0)
+        ref_state, n_cmp = st, parity_steps
+    else:
+        st = oracle.State.from_particles(pa)
+        oracle.initialize(prm, st)
+        print(f"bench: CPU oracle {steps} steps on {pa.n} particles", file=sys.stderr,
+              flush=True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            oracle.ssprk_step(prm, st, 3)
+        legs["one"] = (1, steps, time.perf_counter() - t0)
+        ref_state, n_cmp = st, steps
+    cores, k, wall = legs.get("threads", legs["one"])
+    kind = "threaded oracle step (oracle/cpu_step.cpp)" if threaded else "FP64 oracle"
+    cpu = {"value": nf * k / wall, "unit": "fluid particle-steps/s", "cores": cores,
+           "kind": "port", "cpu_model": cpu_model(),
+           "sample": f"{k} SSPRK3 steps of the same {nf}-fluid-particle workload, "
+                     f"{kind}, grid search per stage, {cores} thread(s), {wall:.1f} s"}
+    if threaded:
+        c1, k1, w1 = legs["one"]
+        cpu["one_thread"] = {"value": nf * k1 / w1, "cores": 1,
+                             "sample": f"{k1} steps, same step on one thread, {w1:.1f} s"}
     ctx = ta.HipContext(eq, order=3, post_integrate=post_integrate, walls=walls)
     ctx.upload([pa])
-    ctx.step(steps)
+    ctx.step(n_cmp)
     g = ctx.download()
+    off = ctx.neighbor_offsets()
     ctx.close()
-    rho_err = float(np.max(np.abs(g["rho"][:nf] - st.rho[:nf]) / st.rho[:nf]))
-    p_o = oracle.pressure(prm, st.rho[:nf])
+    rho_rel = np.abs(g["rho"][:nf] - ref_state.rho[:nf]) / ref_state.rho[:nf]
+    p_o = oracle.pressure(prm, ref_state.rho[:nf])
     p_err = float(np.max(np.abs(g["p"][:nf] - p_o)) / np.abs(p_o).max())
-    front_g = float(g["r"][:nf, 0].max())
-    front_o = float(st.r[:nf, 0].max())
-    parity = {"steps": steps, "rho_max_rel_err": rho_err,
+    # surge front: largest x of a fluid particle with more than 8 (2D) or
+    # 26 (3D) neighbours, each side's own neighbour counts
+    front_g = ta.surge_front(g["r"], nf, off)
+    off_o, _ = oracle.grid_search(ref_state.r, 2.0 * eq.h, eq.h)
+    front_o = ta.surge_front(ref_state.r, nf, off_o)
+    parity = {"steps": n_cmp, "against": kind,
+              "rho_max_rel_err": float(rho_rel.max()),
+              "rho_rms_rel_err": float(np.sqrt(np.mean(rho_rel ** 2))),
               "p_max_err_over_pmax": p_err,
+              "front_x_m": front_o,
               "front_x_rel_err": abs(front_g - front_o) / front_o}
     return cpu, parity
 

@@ -314,3 +314,39 @@ def ssprk_generic(order, dt, u, rhs):
     if rc != 0:
         raise RuntimeError("ssprk_generic failed")
     return u
+
+
+# ---------------------------------------------------------------------------
+# Threaded CPU baseline (cpu_step.cpp): the same step on a std::thread pool.
+
+CPU_LIB_PATH = HERE / "libtit_cpu.so"
+_cpu = None
+
+
+def _load_cpu():
+    global _cpu
+    if _cpu is None:
+        if not CPU_LIB_PATH.exists():
+            raise RuntimeError(f"{CPU_LIB_PATH} missing; run `make -C oracle`")
+        lib = C.CDLL(os.fspath(CPU_LIB_PATH))
+        lib.cpu_set_threads.restype = C.c_int
+        lib.cpu_set_threads.argtypes = [C.c_int]
+        lib.cpu_ssprk_step.restype = C.c_int
+        lib.cpu_ssprk_step.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcState), C.c_int, _DP]
+        _cpu = lib
+    return _cpu
+
+
+def cpu_set_threads(n: int) -> int:
+    """Thread count of the CPU baseline's pool (1: the caller only)."""
+    return _load_cpu().cpu_set_threads(int(n))
+
+
+def cpu_ssprk_step(prm, st, order=3) -> float:
+    """One step of the threaded CPU baseline (wall-free form, no
+    post-integration); the same contract as ssprk_step."""
+    dt = C.c_double()
+    rc = _load_cpu().cpu_ssprk_step(C.byref(prm), C.byref(st.c), order, C.byref(dt))
+    if rc != 0:
+        raise RuntimeError("CPU baseline step rejected its arguments")
+    return dt.value

@@ -1,0 +1,261 @@
+"""The benchmark configurations against the FP64 oracle on the device.
+
+- 3D dam break with h = 1.3 dr (support 2.6 dr, 81 lattice neighbours):
+  neighbour sets bit-exact, right-hand sides and a 20-step trajectory at
+  the pair-sum tolerance.
+- Batched seeded tanks (columns U[0.3, 1.2] x U[0.5, 2.0] m): 16 tanks at a
+  coarse spacing, each checked against its own oracle run (density,
+  pressure, surge front, per-tank clock); and the full 256-tank upload at
+  dr = 5 mm stepped under size-independent properties.
+- The 2D dam break at dr = 0.02 (5,000 fluid particles) over 1,000 steps,
+  while the surge front runs out: density, pressure and front errors
+  recorded every 100 steps.
+
+Density errors are max |rho_gpu - rho_oracle| / rho_oracle over fluid
+particles; pressure errors are max |p_gpu - p_oracle| over fluid particles
+divided by the largest oracle pressure; the surge front is the largest x of a
+fluid particle with more than 8 (2D) or 26 (3D) neighbours.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import titsolver_amd as ta
+from helpers import csr_equal, perturbed_dam_break
+
+pytestmark = pytest.mark.gpu
+
+
+def _errors(g, st, prm, nf):
+    rho_rel = np.abs(g["rho"][:nf] - st.rho[:nf]) / st.rho[:nf]
+    p_o = oracle.pressure(prm, st.rho[:nf])
+    p_err = float(np.max(np.abs(g["p"][:nf] - p_o)) / np.abs(p_o).max())
+    return float(rho_rel.max()), float(np.sqrt(np.mean(rho_rel ** 2))), p_err
+
+
+def _front(r, nf, eq):
+    off, _ = oracle.grid_search(r, 2.0 * eq.h, eq.h)
+    return ta.surge_front(r, nf, off)
+
+
+# ---------------------------------------------------------------------------
+# 3D, h = 1.3 dr
+
+
+def _eq3(dr, kernel=ta.KERNEL_WENDLAND_C2, flavour=ta.FLAVOUR_CFG):
+    eq = ta.cfg_equations(3, dr, 0.6, kernel=kernel, h_factor=1.3, flavour=flavour)
+    if flavour == ta.FLAVOUR_REF:
+        eq.mu, eq.cfl = 1e-3, 0.4
+    return eq
+
+
+def test_3d_h13_neighbors_bit_exact():
+    dr = 0.025
+    pa = perturbed_dam_break(3, dr, seed=21)
+    eq = _eq3(dr)
+    ctx = ta.HipContext(eq)
+    ctx.upload([pa])
+    got = ctx.neighbors()
+    assert csr_equal(got, oracle.grid_search(pa.r, 2.0 * eq.h, eq.h))
+    assert csr_equal(got, oracle.naive_search(pa.r, 2.0 * eq.h))
+    # the stage kernel sums exactly these sets
+    counts = ctx.pair_counts().astype(np.int64)
+    sizes = np.diff(got[0].astype(np.int64)) - 1
+    nf = pa.n_fluid
+    assert np.array_equal(counts[:nf], sizes[:nf])
+    ctx.close()
+
+
+def test_3d_h13_lattice_has_81_neighbours():
+    """Support 2.6 dr on a cubic lattice: 81 points of the lattice lie within
+    it (|i|^2 + |j|^2 + |k|^2 <= 6.76), self included."""
+    dr = 0.02
+    n_axis = 12
+    g = np.meshgrid(*[np.arange(n_axis)] * 3, indexing="ij")
+    r = np.stack([dr * (gi.ravel() + 1.0) for gi in g], axis=1)
+    n = len(r)
+    pa = ta.ParticleArray(3, r, np.full(n, 1000.0), n, np.zeros((0, 3)),
+                          (0.0,) * 3, (dr * (n_axis + 1),) * 3)
+    eq = _eq3(dr)
+    ctx = ta.HipContext(eq)
+    ctx.upload([pa])
+    off, idx = ctx.neighbors()
+    assert csr_equal((off, idx), oracle.naive_search(r, 2.0 * eq.h))
+    assert np.diff(off.astype(np.int64)).max() == 81
+    ctx.close()
+
+
+@pytest.mark.parametrize("kernel", [ta.KERNEL_WENDLAND_C2, ta.KERNEL_WENDLAND_C4])
+@pytest.mark.parametrize("flavour", [ta.FLAVOUR_CFG, ta.FLAVOUR_REF])
+def test_3d_h13_rhs_vs_oracle(kernel, flavour):
+    dr = 0.025
+    pa = perturbed_dam_break(3, dr, seed=23)
+    eq = _eq3(dr, kernel, flavour)
+    ctx = ta.HipContext(eq)
+    ctx.upload([pa])
+    ctx.prepare()
+    st_g = ctx.download()
+    drho_g, dv_g = ctx.compute_rhs()
+    prm = oracle.params(eq)
+    st = oracle.State.from_particles(pa)
+    off, idx = oracle.adjacency(st, prm)
+    oracle.setup_boundary(prm, st, off, idx)
+    nf = pa.n_fluid
+    np.testing.assert_allclose(st_g["rho"][nf:], st.rho[nf:], rtol=1e-12, atol=0)
+    oracle.compute_continuity(prm, st, off, idx)
+    oracle.compute_momentum(prm, st, off, idx)
+    err_rho = np.abs(drho_g[:nf] - st.drho_dt[:nf]).max() / np.abs(st.drho_dt[:nf]).max()
+    err_v = np.abs(dv_g[:nf] - st.dv_dt[:nf]).max() / np.abs(st.dv_dt[:nf]).max()
+    print(f"3D h=1.3dr kernel={kernel} flavour={flavour}: drho {err_rho:.2e} dv {err_v:.2e}")
+    # FP32 pair terms, Kahan-compensated: 2e-5 of the field's largest value
+    assert err_rho < 2e-5 and err_v < 2e-5
+    ctx.close()
+
+
+def test_3d_h13_dam_break_steps_vs_oracle():
+    """20 SSPRK3 steps of a 3D dam break at h = 1.3 dr from the hydrostatic
+    start: rho within 1e-6 relative, p within 1e-4 of p_max, r within
+    1e-5 dr, dt within 1e-7 (the force criterion reads FP32 accelerations)."""
+    dr = 0.03
+    pa = ta.dam_break(3, dr, [1.2, 1.0, 0.3 + dr], [0.45, 0.6, 0.3])
+    eq = ta.cfg_equations(3, dr, 0.6, h_factor=1.3)
+    ctx = ta.HipContext(eq)
+    ctx.upload([pa])
+    st = oracle.State.from_particles(pa)
+    prm = oracle.params(eq)
+    nf = pa.n_fluid
+    for s in range(20):
+        dt_g = ctx.step(1)
+        dt_o = oracle.ssprk_step(prm, st, 3)
+        assert abs(dt_g - dt_o) <= 1e-7 * dt_o, (s, dt_g, dt_o)
+    g = ctx.download()
+    rho_max, rho_rms, p_err = _errors(g, st, prm, nf)
+    r_err = np.max(np.abs(g["r"][:nf] - st.r[:nf])) / dr
+    print(f"3D h=1.3dr, {nf} fluid: rho max {rho_max:.2e} rms {rho_rms:.2e}, "
+          f"p/pmax {p_err:.2e}, r/dr {r_err:.2e}")
+    assert rho_max < 1e-6 and p_err < 1e-4 and r_err < 1e-5
+    ctx.close()
+
+
+# ---------------------------------------------------------------------------
+# Batched seeded tanks
+
+
+def test_batched_seeded_tanks_vs_oracle():
+    """16 seeded tanks (dr = 2.5 cm, 1,251-4,770 particles each) in one
+    context, 40 steps; every tank is checked against its own oracle run:
+    its clock (each tank has its own dt), rho, p and surge front."""
+    dr = 0.025
+    arrays = ta.seeded_tanks(0, dr, n_tanks=16)
+    eq = ta.cfg_equations(2, dr, 2.0)
+    ctx = ta.HipContext(eq)
+    ctx.upload(arrays)
+    steps = 40
+    ctx.step(steps)
+    g = ctx.download()
+    t_g, _ = ctx.tank_clock()
+    prm = oracle.params(eq)
+    o = 0
+    worst = [0.0, 0.0, 0.0, 0.0]
+    for k, pa in enumerate(arrays):
+        st = oracle.State.from_particles(pa)
+        t_o = sum(oracle.ssprk_step(prm, st, 3) for _ in range(steps))
+        sl = slice(o, o + pa.n)
+        gk = {f: g[f][sl] for f in ("r", "rho", "p")}
+        nf = pa.n_fluid
+        rho_max, _, p_err = _errors(gk, st, prm, nf)
+        fr_err = abs(_front(gk["r"], nf, eq) - _front(st.r, nf, eq)) / dr
+        assert abs(t_g[k] - t_o) <= 1e-7 * t_o, (k, t_g[k], t_o)
+        assert rho_max < 1e-6 and p_err < 1e-4 and fr_err < 1e-4, \
+            (k, rho_max, p_err, fr_err)
+        worst = [max(worst[0], rho_max), max(worst[1], p_err), max(worst[2], fr_err),
+                 max(worst[3], abs(t_g[k] - t_o) / t_o)]
+        o += pa.n
+    print(f"16 tanks, {steps} steps: rho {worst[0]:.2e}, p/pmax {worst[1]:.2e}, "
+          f"front/dr {worst[2]:.2e}, clock {worst[3]:.2e}")
+
+
+def test_batched_256_tanks_full_size_properties():
+    """The full batched workload: 256 seeded tanks at dr = 5 mm (10.2 M fluid
+    + 1.24 M fixed particles) uploaded once and stepped 3 times.  Properties
+    that hold at any size: every fluid particle's pair count equals its
+    neighbour-list size minus itself, fixed particles have none; the state
+    stays finite; each tank's clock equals its own three steps; fixed
+    particles never move."""
+    dr = 0.005
+    arrays = ta.seeded_tanks(0, dr, n_tanks=256)
+    assert len(arrays) == 256
+    eq = ta.cfg_equations(2, dr, 2.0)
+    ctx = ta.HipContext(eq)
+    ctx.upload(arrays)
+    dts = []
+    t_prev = np.zeros(256)
+    for _ in range(3):
+        ctx.step(1)
+        t, dt_next = ctx.tank_clock()
+        dts.append(t - t_prev)
+        t_prev = t
+    assert np.all(t > 0) and np.all(np.isfinite(t))
+    # the per-tank steps differ (different column heights reach different
+    # speeds), and each tank's clock is the sum of its own steps
+    np.testing.assert_allclose(np.sum(dts, axis=0), t, rtol=1e-15)
+    counts = ctx.pair_counts().astype(np.int64)
+    off, _ = ctx.neighbors()
+    sizes = np.diff(off.astype(np.int64)) - 1
+    fluid = np.concatenate([np.r_[np.ones(a.n_fluid, bool), np.zeros(a.n_fixed, bool)]
+                            for a in arrays])
+    assert np.array_equal(counts[fluid], sizes[fluid])
+    assert np.all(counts[~fluid] == 0)
+    g = ctx.download()
+    assert np.all(np.isfinite(g["r"])) and np.all(np.isfinite(g["rho"]))
+    o = 0
+    for pa in arrays:
+        sl = slice(o + pa.n_fluid, o + pa.n)
+        assert np.array_equal(g["r"][sl], pa.r[pa.n_fluid:])
+        o += pa.n
+    print(f"256 tanks: {ctx.n} particles, mean pairs per fluid particle "
+          f"{counts[fluid].mean():.1f}, dt range {dts[-1].min():.3e}..{dts[-1].max():.3e}")
+    ctx.close()
+
+
+# ---------------------------------------------------------------------------
+# A run where the front moves
+
+
+def test_dam_break_1000_steps_front_runs_out():
+    """The 2D dam break at dr = 0.02 (5,000 fluid particles, tank 3.22 x 2.4 m,
+    column 1 x 2 m), 1,000 SSPRK3 steps (t = 0.16 s, the front runs out from
+    x = 1.0 m), GPU against the oracle from the same start.  Every 100 steps:
+    rho max and RMS relative error, p error over p_max, front error over dr.
+    Bounds: rho max 1e-5, rho RMS 1e-6, p 1e-3 of p_max, front 1e-3 dr, and
+    the front must have moved by more than 5 dr."""
+    dr = 0.02
+    pa = ta.dam_break(2, dr, [3.22, 2.4], [1.0, 2.0])
+    eq = ta.cfg_equations(2, dr, 2.0)
+    ctx = ta.HipContext(eq)
+    ctx.upload([pa])
+    st = oracle.State.from_particles(pa)
+    prm = oracle.params(eq)
+    nf = pa.n_fluid
+    front0 = _front(pa.r, nf, eq)
+    rows = []
+    for block in range(10):
+        ctx.step(100)
+        for _ in range(100):
+            oracle.ssprk_step(prm, st, 3)
+        g = ctx.download()
+        rho_max, rho_rms, p_err = _errors(g, st, prm, nf)
+        f_g, f_o = _front(g["r"], nf, eq), _front(st.r, nf, eq)
+        t_g, _ = ctx.tank_clock()
+        rows.append((100 * (block + 1), float(t_g[0]), rho_max, rho_rms, p_err,
+                     f_o, abs(f_g - f_o) / dr))
+    print("step  t [s]    rho max   rho rms   p/pmax    front [m]  front err/dr")
+    for r in rows:
+        print(f"{r[0]:5d} {r[1]:.5f} {r[2]:.2e} {r[3]:.2e} {r[4]:.2e} {r[5]:.4f}     {r[6]:.2e}")
+    last = rows[-1]
+    assert last[5] - front0 > 5 * dr, "the front has not run out"
+    for r in rows:
+        assert r[2] < 1e-5 and r[3] < 1e-6 and r[4] < 1e-3 and r[6] < 1e-3, r
+    ctx.close()

@@ -4,4 +4,5 @@ from ._abi import (FLAVOUR_CFG, FLAVOUR_REF, KERNEL_WENDLAND_C2,  # noqa: F401
 from .sph import (FluidEquations, HipContext, LearnedStep,  # noqa: F401
                   ParticleArray, SSPRKIntegrator, TitError, Walls, box_walls,
                   cfg_equations, dam_break, dam_break_walls, nn_gemm,
-                  ref_equations, tessellate)
+                  ref_equations, seeded_tanks, surge_front,
+                  tessellate)

@@ -132,6 +132,34 @@ def dam_break(dim: int, dr: float, tank, column, layers: int = 3,
                          tuple(hi))
 
 
+def seeded_tanks(seed: int, dr: float, n_tanks: int = 0, min_fluid: int = 0,
+                 tank=(3.22, 2.4)) -> list[ParticleArray]:
+    """Dam breaks of the batched configuration: columns of width U[0.3, 1.2] m
+    and height U[0.5, 2.0] m drawn from numpy's default generator with `seed`,
+    in the open-top tank; n_tanks of them, or as many as reach min_fluid
+    fluid particles."""
+    rng = np.random.default_rng(seed)
+    arrays, nf = [], 0
+    while (n_tanks and len(arrays) < n_tanks) or (not n_tanks and nf < min_fluid):
+        col = [float(rng.uniform(0.3, 1.2)), float(rng.uniform(0.5, 2.0))]
+        arrays.append(dam_break(2, dr, list(tank), col))
+        nf += arrays[-1].n_fluid
+    return arrays
+
+
+def surge_front(r: np.ndarray, n_fluid: int, off: np.ndarray) -> float:
+    """Surge-front position: the largest x over fluid particles whose
+    neighbour list (self included) holds more than 8 entries in 2D, 26 in 3D,
+    i.e. that post_integrate would not mark as a splash
+    (fluid_equations.hpp:418-425).  `off` is the CSR offset array of the
+    particles' neighbour lists."""
+    dim = r.shape[1]
+    cutoff = 8 if dim == 2 else 26
+    sizes = np.diff(np.asarray(off, dtype=np.int64))[:n_fluid]
+    keep = sizes > cutoff
+    return float(r[:n_fluid][keep, 0].max()) if keep.any() else float("nan")
+
+
 @dataclass
 class Walls:
     """Semi-analytic wall boundary, 2D: the wall surface (vertices, faces as
@@ -312,6 +340,18 @@ class HipContext:
             self.h, off.ctypes.data_as(C.POINTER(C.c_uint64)),
             idx.ctypes.data_as(C.POINTER(C.c_uint32)), idx.size, C.byref(total)))
         return off, idx[: total.value]
+
+    def neighbor_offsets(self) -> np.ndarray:
+        """CSR offsets of the sorted neighbour lists (self included), without
+        materialising the lists."""
+        off = np.zeros(self.n + 1, dtype=np.uint64)
+        total = C.c_size_t()
+        st = self.lib.tit_hip_neighbors(
+            self.h, off.ctypes.data_as(C.POINTER(C.c_uint64)),
+            C.cast(None, C.POINTER(C.c_uint32)), 0, C.byref(total))
+        if st not in (0, _abi.TIT_E_ARG):
+            self._check(st)
+        return off
 
     def pair_counts(self) -> np.ndarray:
         """Neighbours summed per particle by the stage kernel (test hook)."""
