@@ -229,8 +229,9 @@ def test_dam_break_1000_steps_front_runs_out():
     column 1 x 2 m), 1,000 SSPRK3 steps (t = 0.16 s, the front runs out from
     x = 1.0 m), GPU against the oracle from the same start.  Every 100 steps:
     rho max and RMS relative error, p error over p_max, front error over dr.
-    Bounds: rho max 1e-5, rho RMS 1e-6, p 1e-3 of p_max, front 1e-3 dr, and
-    the front must have moved by more than 5 dr."""
+    Bounds: rho max 1e-7, rho RMS 1e-8, p 1e-5 of p_max, front 1e-5 dr (the
+    measured series stays below 2e-9, 5e-10, 1e-6 and 1e-7), and the front
+    must have moved by more than 5 dr."""
     dr = 0.02
     pa = ta.dam_break(2, dr, [3.22, 2.4], [1.0, 2.0])
     eq = ta.cfg_equations(2, dr, 2.0)
@@ -257,5 +258,5 @@ def test_dam_break_1000_steps_front_runs_out():
     last = rows[-1]
     assert last[5] - front0 > 5 * dr, "the front has not run out"
     for r in rows:
-        assert r[2] < 1e-5 and r[3] < 1e-6 and r[4] < 1e-3 and r[6] < 1e-3, r
+        assert r[2] < 1e-7 and r[3] < 1e-8 and r[4] < 1e-5 and r[6] < 1e-5, r
     ctx.close()
