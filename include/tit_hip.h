@@ -200,29 +200,43 @@ tit_status tit_hip_set_profiling(tit_hip_ctx* ctx, int enable);
 tit_status tit_hip_kernel_stats(tit_hip_ctx* ctx, tit_hip_kernel_stat* out,
                                 int cap, int* n_out);
 
-/* Schedule of the fused stage kernel: each workgroup takes a chunk of at
- * most chunk_targets consecutive particles of one cell column (one per lane);
- * in 2D up to lds_capacity candidate positions of a chunk are staged in LDS
- * (0 reads them from global memory; results are bitwise identical either
- * way).  occupancy is the mean number of particles per occupied cell at
- * upload.  No reference counterpart (the reference partitions pairs for TBB
- * threads instead, particle_mesh.hpp:164-241). */
+/* Schedule of the fused stage kernel.  Two families:
+ *  - column chunks: each workgroup takes a chunk of at most chunk_targets
+ *    consecutive particles of one cell column (one per lane); in 2D up to
+ *    lds_capacity candidate positions of a chunk are staged in LDS;
+ *  - 3D block chunks (lds_capacity > 0 in 3D): a workgroup of 256 lanes takes
+ *    the particles of a 4 x 4 block of cell columns over a few rows, with up
+ *    to lds_capacity candidate positions of the block's window in LDS.
+ * lds_capacity 0 reads positions from global memory (column chunks).  Results
+ * are bitwise identical on every schedule.  occupancy is the mean number of
+ * particles per occupied cell at upload.  cached (tit_hip_autotune only): 1
+ * when the schedule came from the process's tuning cache.  No reference
+ * counterpart (the reference partitions pairs for TBB threads instead,
+ * particle_mesh.hpp:164-241). */
 typedef struct tit_hip_schedule {
   int chunk_targets;
   int lds_capacity;
   double occupancy;
+  int cached;
+  int reserved;
 } tit_hip_schedule;
 tit_status tit_hip_get_schedule(tit_hip_ctx* ctx, tit_hip_schedule* out);
 /* chunk_targets = 0 keeps the current chunk length. */
 tit_status tit_hip_set_schedule(tit_hip_ctx* ctx, const tit_hip_schedule* s);
 
 /* Autotuner of the stage schedule: times every candidate (chunk_targets in
- * {32, 48, 64, 96, 128} x, in 2D, LDS capacity {max, max/2, 0}) over
+ * {32, 48, 64, 96, 128} x LDS capacity {max, max/2, 0}) over
  * steps_per_candidate real steps of the current state (all schedules give the
  * same bits, so the steps are ordinary progress), keeps the fastest and
- * returns it with its time per step.  No reference counterpart. */
+ * returns it with its time per step.  The result is kept in a per-process
+ * cache keyed by (dimension, kernel, term set, walls, one or many tanks, the
+ * particle count's power of two): a later call on a context of the same key
+ * applies the cached schedule without timing (best->cached = 1, no steps
+ * taken).  No reference counterpart. */
 tit_status tit_hip_autotune(tit_hip_ctx* ctx, int steps_per_candidate,
                             tit_hip_schedule* best, double* best_ms_per_step);
+/* Empties the autotuner's cache. */
+void tit_hip_autotune_clear_cache(void);
 
 /* Number of particles (all tanks), of cell-index rebuilds run so far (the
  * index is rebuilt only when a tank's particles could otherwise leave the

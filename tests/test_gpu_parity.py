@@ -204,15 +204,17 @@ def test_rebuild_interleaved_with_hooks():
 @pytest.mark.parametrize("dim", [2, 3])
 def test_schedules_are_bitwise_identical(dim):
     """Any chunk length, and LDS staging on or off, gives the same bits: each
-    target walks its own stencil in the same order on every path."""
+    target walks its own stencil in the same order on every path.  With a
+    small LDS capacity most chunks overflow to the global-memory kernel (2D:
+    column chunks; 3D: block chunks); capacity 0 is the column-chunk
+    global-memory path."""
     dr = 0.02 if dim == 2 else 0.03
     pa = perturbed_dam_break(dim, dr, seed=9)
     eq = _eq(dim, dr)
     outs = []
     base = _ctx(eq, [pa]).get_schedule()
     variants = [(0, None), (1, None), (37, None), (0, 0), (64, 0)]
-    if dim == 2:
-        variants.append((0, 60))  # LDS staging exists in 2D only
+    variants.append((0, 60) if dim == 2 else (0, 300))
     for chunk, lds in variants:
         ctx = _ctx(eq, [pa])
         ctx.set_schedule(chunk, lds)
@@ -231,9 +233,11 @@ def test_autotune_keeps_the_trajectory():
     dr = 0.02
     pa = perturbed_dam_break(2, dr, seed=3)
     eq = _eq(2, dr)
+    ta.HipContext.clear_autotune_cache()
     a = _ctx(eq, [pa])
     best = a.autotune(2)
     n_a, _, steps = a.counts()
+    assert not best["cached"] and steps > 0
     assert best["chunk_targets"] in (32, 48, 64, 96, 128)
     assert best["ms_per_step"] > 0
     b = _ctx(eq, [pa])
@@ -247,6 +251,35 @@ def test_autotune_keeps_the_trajectory():
     assert np.array_equal(a.download()["rho"], b.download()["rho"])
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_autotune_cache_reuses_the_schedule(dim):
+    """A second context of the same kind and size takes the first one's
+    tuned schedule from the cache without timing (no steps taken); a context
+    four times larger is tuned afresh.  In 3D the candidates include both
+    schedule families (block chunks with LDS staging, column chunks from
+    global memory)."""
+    dr = 0.02 if dim == 2 else 0.03
+    ta.HipContext.clear_autotune_cache()
+    pa = perturbed_dam_break(dim, dr, seed=4)
+    eq = _eq(dim, dr)
+    a = _ctx(eq, [pa])
+    first = a.autotune(2)
+    assert not first["cached"]
+    b = _ctx(eq, [pa])
+    second = b.autotune(2)
+    assert second["cached"] and b.counts()[2] == 0
+    assert (second["chunk_targets"], second["lds_capacity"]) == \
+        (first["chunk_targets"], first["lds_capacity"])
+    assert b.get_schedule()["lds_capacity"] == first["lds_capacity"]
+    big = perturbed_dam_break(dim, dr / (2.0 if dim == 2 else 1.6), seed=4)
+    eq_big = _eq(dim, dr / (2.0 if dim == 2 else 1.6))
+    c = _ctx(eq_big, [big])
+    assert big.n >= 2 * pa.n
+    assert not c.autotune(2)["cached"]
+    for x in (a, b, c):
+        x.close()
 
 
 @pytest.mark.parametrize("dim,dr,schedule", [(2, 0.02, (0, None)), (2, 0.02, (0, 0)),

@@ -57,7 +57,7 @@ class Walls(C.Structure):
 
 class Schedule(C.Structure):
     _fields_ = [("chunk_targets", C.c_int), ("lds_capacity", C.c_int),
-                ("occupancy", C.c_double)]
+                ("occupancy", C.c_double), ("cached", C.c_int), ("reserved", C.c_int)]
 
 
 class LearnedReport(C.Structure):
@@ -99,6 +99,7 @@ SIGNATURES = {
     "tit_hip_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
     "tit_hip_set_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
     "tit_hip_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Schedule), _DP]),
+    "tit_hip_autotune_clear_cache": (None, []),
     "tit_hip_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t),
                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "tit_hip_last_error": (C.c_char_p, [C.c_void_p]),

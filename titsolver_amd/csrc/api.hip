@@ -15,7 +15,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/tit_hip.h"
@@ -369,7 +372,35 @@ tit_status build_schedule(tit_hip_ctx* c) {
     c->chunk_cap = need;
   }
   if (!c->chunk_counts) TIT_HIP_TRY(c, dalloc(c, &c->chunk_counts, 4));
-  stage_grids(c->dp, c->n_cu, &c->grid_lds, &c->grid_global);
+  if (dim == 3 && !c->blocks) {
+    // blocks of kBlk3 x kBlk3 columns; a block chunk holds at least one
+    // particle and one non-empty row of its block
+    std::vector<uint4> blocks;
+    for (size_t t = 0; t < c->host_tanks.size(); ++t) {
+      const DevTank& T = c->host_tanks[t];
+      for (int bx = 0; bx < T.nc[0]; bx += kBlk3) {
+        for (int by = 0; by < T.nc[1]; by += kBlk3) {
+          blocks.push_back(make_uint4(static_cast<uint32_t>(t), bx, by, 0u));
+        }
+      }
+    }
+    const size_t cap =
+        std::min(static_cast<size_t>(c->n), blocks.size() * static_cast<size_t>(nz_max)) + 1;
+    TIT_HIP_TRY(c, dalloc(c, &c->blocks, blocks.size()));
+    TIT_HIP_TRY(c, dalloc(c, &c->bcount, blocks.size() + 1));
+    TIT_HIP_TRY(c, dalloc(c, &c->bchunks, cap));
+    TIT_HIP_TRY(c, dalloc(c, &c->boverflow, cap));
+    TIT_HIP_TRY(c, hipMemcpy(c->blocks, blocks.data(), blocks.size() * sizeof(uint4),
+                             hipMemcpyHostToDevice));
+    c->n_blocks = static_cast<uint32_t>(blocks.size());
+    c->bchunk_cap = cap;
+  }
+  StageGrids g;
+  stage_grids(c->dp, c->n_cu, &g);
+  c->grid_lds = g.lds;
+  c->grid_global = g.global;
+  c->grid_lds3 = g.lds3;
+  c->grid_block_global = g.block_global;
   drop_graph(c);
   if (c->rebuild_ctl) {
     // the chunk list follows the schedule: recut it from the current cell
@@ -392,6 +423,13 @@ StageArgs stage_args(tit_hip_ctx* c) {
   a.counts = c->chunk_counts;
   a.grid_lds = c->grid_lds;
   a.grid_global = c->grid_global;
+  a.blocks = c->blocks;
+  a.n_blocks = c->n_blocks;
+  a.bchunks = c->bchunks;
+  a.boverflow = c->boverflow;
+  a.bcount = c->bcount;
+  a.grid_lds3 = c->grid_lds3;
+  a.grid_block_global = c->grid_block_global;
   a.P = c->dp;
   a.tanks = c->tanks;
   a.key = c->key;
@@ -516,6 +554,30 @@ tit_status read_flags(tit_hip_ctx* c) {
                 "one step; raise tit_hip_params.skin");
   }
   return TIT_OK;
+}
+
+// The autotuner's per-process cache: the schedule found for a kind of
+// context (dimension, kernel, term set, walls, one or many tanks) and size
+// (the particle count's power of two).
+struct TuneKey {
+  int dim, kernel, flavour, walls, batched, log2n;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(dim, kernel, flavour, walls, batched, log2n) <
+           std::tie(o.dim, o.kernel, o.flavour, o.walls, o.batched, o.log2n);
+  }
+};
+struct TuneVal {
+  uint32_t chunk_targets, lds_cap;
+  double ms_per_step;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+TuneKey tune_key(const tit_hip_ctx* c) {
+  int log2n = 0;
+  while ((static_cast<size_t>(1) << (log2n + 1)) <= c->n) ++log2n;
+  return TuneKey{c->prm.dim, c->prm.kernel, c->prm.flavour, c->walls ? 1 : 0,
+                 c->n_tanks > 1 ? 1 : 0, log2n};
 }
 
 }  // namespace
@@ -937,6 +999,12 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   c->overflow = nullptr;
   c->block_tot = nullptr;
   c->chunk_counts = nullptr;
+  c->blocks = nullptr;
+  c->bchunks = nullptr;
+  c->boverflow = nullptr;
+  c->bcount = nullptr;
+  c->n_blocks = 0;
+  c->bchunk_cap = 0;
   c->n_cols = 0;
   c->chunk_cap = 0;
   c->lds_cap = stage_lds_capacity(dim);
@@ -1181,6 +1249,32 @@ tit_status tit_hip_autotune(tit_hip_ctx* c, int steps_per_candidate,
   if (steps_per_candidate < 2) return fail(c, TIT_E_ARG, "need at least 2 steps per candidate");
   if (c->prof.enabled) return fail(c, TIT_E_STATE, "autotune with profiling enabled");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
+  const TuneKey key = tune_key(c);
+  TuneVal hit{};
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      hit = it->second;
+      cached = true;
+    }
+  }
+  if (cached) {
+    TIT_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->chunk_targets = hit.chunk_targets;
+    c->lds_cap = hit.lds_cap;
+    const tit_status st = build_schedule(c);
+    if (st != TIT_OK) return st;
+    if (best) {
+      best->chunk_targets = static_cast<int>(hit.chunk_targets);
+      best->lds_capacity = static_cast<int>(hit.lds_cap);
+      best->occupancy = c->occupancy;
+      best->cached = 1;
+    }
+    if (best_ms_per_step) *best_ms_per_step = hit.ms_per_step;
+    return TIT_OK;
+  }
   const int timed = steps_per_candidate / 2 * 2;
   const uint32_t lds_max = stage_lds_capacity(c->prm.dim);
   std::vector<uint32_t> chunks = {32, 48, 64, 96, 128};
@@ -1226,13 +1320,23 @@ tit_status tit_hip_autotune(tit_hip_ctx* c, int steps_per_candidate,
   c->lds_cap = best_cap;
   st = build_schedule(c);
   if (st != TIT_OK) return st;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = TuneVal{best_chunk, best_cap, best_ms};
+  }
   if (best) {
     best->chunk_targets = static_cast<int>(best_chunk);
     best->lds_capacity = static_cast<int>(best_cap);
     best->occupancy = c->occupancy;
+    best->cached = 0;
   }
   if (best_ms_per_step) *best_ms_per_step = best_ms;
   return TIT_OK;
+}
+
+void tit_hip_autotune_clear_cache(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
 }
 
 tit_status tit_hip_counts(tit_hip_ctx* c, size_t* n_particles, uint64_t* n_rebuilds,

@@ -102,6 +102,14 @@ struct tit_hip_ctx {
   uint32_t chunk_targets = kChunkMax;
   uint32_t lds_cap = 0;
   uint32_t grid_lds = 0, grid_global = 0;
+  // 3D block chunks of the LDS stage kernel
+  uint4* blocks = nullptr;
+  uint32_t n_blocks = 0;
+  uint4* bchunks = nullptr;
+  uint32_t* boverflow = nullptr;
+  uint32_t* bcount = nullptr;
+  size_t bchunk_cap = 0;
+  uint32_t grid_lds3 = 0, grid_block_global = 0;
   int n_cu = 0;
   double occupancy = 1.0;  // mean particles per occupied cell at upload
 

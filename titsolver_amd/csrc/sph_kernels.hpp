@@ -17,6 +17,19 @@ constexpr int kChunkRows = 92;
 // Window columns of a 2D chunk and LDS cell-offset rows of its window.
 constexpr int kCols2 = 2 * kStencil + 1;
 constexpr int kRowsMax = kChunkRows + 2 * kStencil;
+// 3D block chunks: the cell columns of a tank in blocks of kBlk3 x kBlk3;
+// a chunk holds the particles of one block in a range of at most
+// kChunkRows3 rows, at most kStage3Threads of them (a row with more goes to
+// the global-memory kernel whole).  Its window is the block grown by the
+// stencil (kWin3 x kWin3 columns, kRows3 rows), FP64 positions of at most
+// kStage3LdsCap candidates staged in LDS.
+constexpr int kBlk3 = 4;
+constexpr int kWin3 = kBlk3 + 2 * kStencil;
+constexpr int kChunkRows3 = 8;
+constexpr int kRows3 = kChunkRows3 + 2 * kStencil;
+constexpr int kWords3 = 8;
+constexpr int kStage3Threads = 256;
+constexpr int kStage3LdsCap = 1536;
 
 struct StageArgs {
   DevParams P;
@@ -47,8 +60,20 @@ struct StageArgs {
   uint32_t lds_cap;            // candidate positions staged in LDS at most
   uint4* chunks;
   uint32_t* overflow;          // 2D chunks whose candidates exceed lds_cap
-  uint32_t* counts;            // [0] chunks, [1] overflow chunks
+  // [0] chunks, [1] overflow chunks, [2] 3D block chunks, [3] block chunks
+  // that take the global-memory kernel
+  uint32_t* counts;
   uint32_t grid_lds, grid_global;  // persistent grid sizes
+  // 3D with LDS staging (lds_cap > 0): cell-column blocks (tank, first ix,
+  // first iy, 0), rebuilt on the device after every cell rebuild: block
+  // chunks (block, first row, end row, targets), per-block chunk counts
+  // (scanned in place), and the block chunks that do not fit
+  const uint4* blocks;
+  uint32_t n_blocks;
+  uint4* bchunks;
+  uint32_t* boverflow;
+  uint32_t* bcount;
+  uint32_t grid_lds3, grid_block_global;
   double* dv_last[3];          // final stage: its acceleration (post-integration only)
   // semi-analytic wall (kFlavourWall): per slot, the face terms of drho/dt
   // and dv/dt already divided by gamma, then gamma
@@ -194,8 +219,11 @@ void launch_export_f64(hipStream_t s, uint32_t n, const uint32_t* id, const doub
                        double* dst);
 void launch_stage(hipStream_t s, const StageArgs& a, int kind, bool final);
 uint32_t stage_lds_capacity(int dim);
-void stage_grids(const DevParams& P, int n_cu, uint32_t* grid_lds,
-                 uint32_t* grid_global);
+// Persistent grid sizes of the stage kernels (multiples of 8 workgroups).
+struct StageGrids {
+  uint32_t lds, global, lds3, block_global;
+};
+void stage_grids(const DevParams& P, int n_cu, StageGrids* g);
 // Rebuilds the chunk list from the cell offsets (after a cell rebuild).
 void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot,
                          const uint32_t* ctl);

@@ -383,13 +383,20 @@ class HipContext:
         self._check(self.lib.tit_hip_set_schedule(self.h, C.byref(sch)))
 
     def autotune(self, steps_per_candidate: int = 4) -> dict:
-        """Times every stage schedule over real steps, keeps the fastest."""
+        """Times every stage schedule over real steps, keeps the fastest (or
+        takes the process cache's schedule for this kind and size of
+        context: "cached" True, no steps taken)."""
         sch = _abi.Schedule()
         ms = C.c_double()
         self._check(self.lib.tit_hip_autotune(self.h, steps_per_candidate, C.byref(sch),
                                               C.byref(ms)))
         return {"chunk_targets": sch.chunk_targets, "lds_capacity": sch.lds_capacity,
-                "occupancy": sch.occupancy, "ms_per_step": ms.value}
+                "occupancy": sch.occupancy, "ms_per_step": ms.value,
+                "cached": bool(sch.cached)}
+
+    @staticmethod
+    def clear_autotune_cache():
+        _abi.load().tit_hip_autotune_clear_cache()
 
     def counts(self):
         """(particles, cell-index rebuilds run, plain steps taken)."""
