@@ -182,6 +182,19 @@ tit_status tit_hip_compute_rhs(tit_hip_ctx* ctx, double* drho_dt,
 tit_status tit_hip_neighbors(tit_hip_ctx* ctx, uint64_t* off, uint32_t* idx,
                              size_t cap, size_t* total);
 
+/* tit_hip_neighbors without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-particle counts and the device copy of the lists live in a static
+ * device array of the library, one per GPU, held until the process ends
+ * (72 MiB), so at most 1,048,576 particles and 16,777,216 list entries
+ * (TIT_E_ARG beyond; *total is set when the entries overflow).  Same
+ * results as tit_hip_neighbors.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (several contexts on several host
+ * threads) race. */
+tit_status tit_hip_neighbors_noalloc(tit_hip_ctx* ctx, uint64_t* off, uint32_t* idx,
+                                     size_t cap, size_t* total);
+
 /* Number of neighbours (self excluded) whose pair terms the fused stage
  * kernel summed for each particle in the last right-hand side of the current
  * state, reference order; 0 for fixed particles (test hook: must equal the

@@ -92,6 +92,9 @@ SIGNATURES = {
     "tit_hip_neighbors": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint32), C.c_size_t,
                                     C.POINTER(C.c_size_t)]),
+    "tit_hip_neighbors_noalloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_uint32), C.c_size_t,
+                                            C.POINTER(C.c_size_t)]),
     "tit_hip_pair_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "tit_hip_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "tit_hip_kernel_stats": (C.c_int, [C.c_void_p, C.POINTER(KernelStat), C.c_int,

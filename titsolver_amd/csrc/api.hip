@@ -556,6 +556,40 @@ tit_status read_flags(tit_hip_ctx* c) {
   return TIT_OK;
 }
 
+// Scratch of tit_hip_neighbors_noalloc, one copy per GPU for the life of the
+// process: per-particle neighbour counts, then list offsets, and the device
+// copy of the lists.  Host code never passes the array's name to a kernel:
+// scratch_ptr() gives each device's own address (hipGetSymbolAddress at
+// tit_hip_create, with the device current).
+constexpr size_t kNoAllocParticles = size_t(1) << 20;
+constexpr size_t kNoAllocEntries = size_t(1) << 24;
+struct NeighborScratch {
+  uint64_t off[kNoAllocParticles + 1];
+  uint32_t idx[kNoAllocEntries];
+};
+__device__ __align__(256) NeighborScratch scratch;
+
+constexpr int kMaxDevices = 64;
+struct ScratchSlot {
+  std::once_flag once;
+  NeighborScratch* ptr = nullptr;
+  hipError_t err = hipSuccess;
+};
+ScratchSlot g_scratch[kMaxDevices];
+
+// Address of `scratch` on `device` (the current device on the first call
+// for it); null when the device has none.
+NeighborScratch* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  ScratchSlot& s = g_scratch[device];
+  std::call_once(s.once, [&s] {
+    void* p = nullptr;
+    s.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (s.err == hipSuccess) s.ptr = static_cast<NeighborScratch*>(p);
+  });
+  return s.ptr;
+}
+
 // The autotuner's per-process cache: the schedule found for a kind of
 // context (dimension, kernel, term set, walls, one or many tanks) and size
 // (the particle count's power of two).
@@ -646,6 +680,7 @@ tit_status tit_hip_create(int device, const tit_hip_params* params,
     delete c;
     return TIT_E_HIP;
   }
+  (void)scratch_ptr(device);  // this device's scratch address, once per device
   *out = c;
   return TIT_OK;
 }
@@ -1191,6 +1226,49 @@ tit_status tit_hip_neighbors(tit_hip_ctx* c, uint64_t* off, uint32_t* idx,
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(d_cnt);
   if (d_idx) (void)hipFree(d_idx);
+  if (e != hipSuccess) return fail(c, TIT_E_HIP, hipGetErrorString(e));
+  return read_flags(c);
+}
+
+tit_status tit_hip_neighbors_noalloc(tit_hip_ctx* c, uint64_t* off, uint32_t* idx,
+                                     size_t cap, size_t* total) {
+  if (!c || !off) return TIT_E_ARG;
+  if (!c->uploaded) return fail(c, TIT_E_STATE, "neighbors before upload");
+  const size_t n = c->n;
+  if (n > kNoAllocParticles) {
+    return fail(c, TIT_E_ARG, "tit_hip_neighbors_noalloc holds at most 1048576 particles");
+  }
+  NeighborScratch* sp = scratch_ptr(c->device);
+  if (!sp) return fail(c, TIT_E_HIP, "no scratch address on this device");
+  TIT_HIP_TRY(c, hipSetDevice(c->device));
+  enqueue_rebuild(c, kRebuildPending);
+  // counts by reference index into sp->off[0, n) (every entry written), then
+  // the offsets on the host, in off itself
+  launch_neighbor_count(c->stream, c->dp, c->tanks, c->n, c->A, c->key, c->tag, c->id,
+                        c->cell_off, sp->off);
+  hipError_t e = hipMemcpyAsync(off + 1, sp->off, n * sizeof(uint64_t),
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, TIT_E_HIP, hipGetErrorString(e));
+  off[0] = 0;
+  for (size_t a = 0; a < n; ++a) off[a + 1] += off[a];
+  if (total) *total = off[n];
+  if (off[n] > kNoAllocEntries) {
+    return fail(c, TIT_E_ARG, "tit_hip_neighbors_noalloc holds at most 16777216 list entries");
+  }
+  if (off[n] > cap || !idx) return fail(c, TIT_E_ARG, "neighbour capacity too small");
+  e = hipMemcpyAsync(sp->off, off, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                     c->stream);
+  if (e == hipSuccess) {
+    launch_neighbor_fill(c->stream, c->dp, c->tanks, c->n, c->A, c->key, c->tag, c->id,
+                         c->cell_off, sp->off, sp->idx);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    e = hipMemcpyAsync(idx, sp->idx, off[n] * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, TIT_E_HIP, hipGetErrorString(e));
   return read_flags(c);
 }
