@@ -114,7 +114,7 @@ tit_status tit_hip_create(int device, const tit_hip_params* params,
                           tit_hip_ctx** out);
 
 /* Switches the context to the semi-analytic wall boundary (2D, REF terms,
- * one tank, no post_integrate): before tit_hip_upload.  Every prepare then
+ * one tank; post_integrate allowed): before tit_hip_upload.  Every prepare then
  * runs compute_gamma (fluid_equations.hpp:168-193), setup_boundary takes
  * normalize(grad gamma) as the wall normal, the fixed particles carry the
  * masses m gamma set by FluidEquations::initialize (:75-89), and the
@@ -138,7 +138,8 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* ctx,
 
 /* Advances every tank by n_steps SSPRK steps and returns the last step size
  * of tank 0.  Replaces: SSPRKIntegrator::step (time_integrator.hpp:159-184),
- * without post_integrate; called as in titwcsph/wcsph.cpp:181. */
+ * ending with FluidEquations::post_integrate when params.post_integrate is 1
+ * (as the reference's step does, :181); called as in titwcsph/wcsph.cpp:181. */
 tit_status tit_hip_step(tit_hip_ctx* ctx, int n_steps, double* dt_last);
 
 /* Copies the state back in reference order.  Any pointer may be NULL.
@@ -334,12 +335,15 @@ typedef struct tit_hip_learned_report {
   double last_dv_err;
 } tit_hip_learned_report;
 /* n_steps learned steps.  The last step of every group of judge_every is
- * judged: it is taken from state S and compared with 4 plain steps from S
- * (max relative density error over fluid particles); the plain result is
- * kept.  Above tol, the group is discarded: the state goes back to the last
- * judged state and the group's interval is covered by plain steps. */
+ * judged: it is taken from state S and compared with 4 plain steps from S;
+ * the plain result is kept.  The group is rejected when the max relative
+ * density error over fluid particles exceeds tol or the error of the step's
+ * velocity change (max_dv_err's measure) exceeds dv_tol: the state goes back
+ * to the last judged state and the group's interval is covered by plain
+ * steps. */
 tit_status tit_hip_learned_run(tit_hip_ctx* ctx, int n_steps, int judge_every,
-                               double tol, tit_hip_learned_report* report);
+                               double tol, double dv_tol,
+                               tit_hip_learned_report* report);
 
 /* The network's BF16 MFMA GEMM on the current device (test hook, no
  * context): A is M x K (K x M with transA), B is K x N (N x K with transB),

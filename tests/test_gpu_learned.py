@@ -173,12 +173,31 @@ def test_training_reduces_loss_and_beats_untrained():
         H.set_scales(sc)
         if name == "trained":
             H.set_params(w)
-        rep = H.run(1, judge_every=1, tol=1e9)
+        rep = H.run(1, judge_every=1, tol=1e9, dv_tol=1e9)
         errs[name] = (rep["max_dv_err"], rep["max_rho_err"])
         hctx.close()
     print(f"held-out (velocity-change error, density error) of one 4 dt step: {errs}")
     assert errs["trained"][0] < 0.8 * errs["untrained"][0]
     assert errs["trained"][1] < 0.01
+
+
+def test_untrained_network_is_rejected_at_the_default_tolerances():
+    """An untrained network passes the 1 % density bound of one 4 dt step
+    (the density barely changes over 4 dt near rest), so the judge also bounds
+    the error of the step's velocity change: at the default tolerances every
+    judged group of an untrained network is rejected and covered by plain
+    steps."""
+    ctx, eq, _ = _batched(99, 2, 0.04)
+    ctx.step(60)
+    L = ta.LearnedStep(ctx, seed=8)
+    L.record(4)  # output scales from this sample, as training would set them
+    rep = L.run(20, judge_every=5)
+    print(f"untrained: {rep}")
+    assert rep["judged"] == 4
+    assert rep["max_rho_err"] < ta.LearnedStep.RHO_TOL  # density alone would accept
+    assert rep["max_dv_err"] > ta.LearnedStep.DV_TOL
+    assert rep["rejected"] == rep["judged"]
+    ctx.close()
 
 
 def test_run_with_zero_tolerance_is_the_plain_trajectory():
@@ -200,7 +219,7 @@ def test_run_accepting_advances_four_dt_per_step():
     ctx, eq, pa = _dam(steps=10)
     L = ta.LearnedStep(ctx, seed=2)
     t0, dt0 = ctx.tank_clock()
-    rep = L.run(3, judge_every=3, tol=1e9)
+    rep = L.run(3, judge_every=3, tol=1e9, dv_tol=1e9)
     assert rep["judged"] == 1 and rep["rejected"] == 0 and rep["learned_steps"] == 3
     assert rep["plain_steps"] == 4
     t1, _ = ctx.tank_clock()

@@ -124,7 +124,7 @@ SIGNATURES = {
     "tit_hip_learned_forward": (C.c_int, [C.c_void_p, _FP]),
     "tit_hip_learned_set_fused": (C.c_int, [C.c_void_p, C.c_int]),
     "tit_hip_learned_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double,
-                                      C.POINTER(LearnedReport)]),
+                                      C.c_double, C.POINTER(LearnedReport)]),
     "tit_hip_nn_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, _FP, C.c_int, _FP,
                                   C.c_int, _FP, C.c_int, C.c_int, _FP]),
     "tit_dam_break_count": (C.c_int, [C.c_int, C.c_double, _DP, _DP, C.c_int,

@@ -338,31 +338,22 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   }
   constexpr int L = DIM - 1;
   const int zlo = max(c[L] - kStencil, 0), zhi = min(c[L] + kStencil, T.nc[L] - 1);
-  uint32_t rb[NCOL], rl[NCOL + 1];
+  // 2D: the five column ranges and their prefix
+  uint32_t rb[DIM == 2 ? NCOL : 1], rl[DIM == 2 ? NCOL + 1 : 1];
   rl[0] = 0;
+  if constexpr (DIM == 2) {
 #pragma unroll
-  for (int q = 0; q < NCOL; ++q) {
-    int ix, iy = 0;
-    bool ok;
-    if constexpr (DIM == 2) {
-      ix = c[0] - kStencil + q;
-      ok = ix >= 0 && ix < T.nc[0];
-    } else {
-      ix = c[0] - kStencil + q / (2 * kStencil + 1);
-      iy = c[1] - kStencil + q % (2 * kStencil + 1);
-      ok = ix >= 0 && ix < T.nc[0] && iy >= 0 && iy < T.nc[1];
+    for (int q = 0; q < NCOL; ++q) {
+      const int ix = c[0] - kStencil + q;
+      uint32_t b = 0, e = 0;
+      if (ix >= 0 && ix < T.nc[0]) {
+        const uint32_t row = T.cell_off + static_cast<uint32_t>(ix) * T.nc[1];
+        b = A.cell_off[row + zlo];
+        e = A.cell_off[row + zhi + 1];
+      }
+      rb[q] = b;
+      rl[q + 1] = rl[q] + (e - b);
     }
-    uint32_t b = 0, e = 0;
-    if (ok) {
-      const uint32_t row = DIM == 2
-                               ? T.cell_off + static_cast<uint32_t>(ix) * T.nc[1]
-                               : T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] +
-                                               static_cast<uint32_t>(iy)) * T.nc[2];
-      b = A.cell_off[row + zlo];
-      e = A.cell_off[row + zhi + 1];
-    }
-    rb[q] = b;
-    rl[q + 1] = rl[q] + (e - b);
   }
   // No stencil cell holds fluid: no fluid neighbour, S = H = 0 as a full scan
   // would leave them.
@@ -392,21 +383,17 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) any |= __shfl_xor(any, off, G);
   }
-  const uint32_t n_cand = any ? rl[NCOL] : 0u;
   double S = 0.0, H = 0.0;
   const double xi_1 = P.xi - 1.0, inv_xi_1 = 1.0 / xi_1, cs2 = P.cs0 * P.cs0;
-  for (uint32_t t = lane; t < n_cand; t += G) {
-    int q = 0;
-    while (t >= rl[q + 1]) ++q;
-    const uint32_t j = rb[q] + (t - rl[q]);
-    if (A.tag[j] & 1u) continue;  // fluid neighbours only (self is fixed)
+  auto visit = [&](uint32_t j) {
+    if (A.tag[j] & 1u) return;  // fluid neighbours only (self is fixed)
     double dd[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) dd[d] = A.s.x[d][j] - xe[d];
     double s2 = dd[0] * dd[0];
 #pragma unroll
     for (int d = 1; d < DIM; ++d) s2 = s2 + dd[d] * dd[d];
-    if (!(s2 <= P.R2)) continue;
+    if (!(s2 <= P.R2)) return;
     const double rho_b = A.s.rho[j];
     const double V_b = P.m / rho_b;
     const double W = w0 * kernel_value_poly_f64<KERNEL>(h_inv * sqrt(s2));
@@ -419,6 +406,28 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     const double H_b = cs2 * (rp - 1.0) * inv_xi_1;
     S += V_b * W;
     H += V_b * (H_b + P.g * rn * ne[1]) * W;
+  };
+  if constexpr (DIM == 2) {
+    // the lanes take every G-th candidate of the five column ranges in turn
+    const uint32_t n_cand = any ? rl[NCOL] : 0u;
+    for (uint32_t t = lane; t < n_cand; t += G) {
+      int q = 0;
+      while (t >= rl[q + 1]) ++q;
+      visit(rb[q] + (t - rl[q]));
+    }
+  } else if (any) {
+    // 25 short column ranges: one range at a time, its bounds loaded by the
+    // whole group (no per-candidate search over the ranges)
+    for (int q = 0; q < NCOL; ++q) {
+      const int ix = c[0] - kStencil + q / (2 * kStencil + 1);
+      const int iy = c[1] - kStencil + q % (2 * kStencil + 1);
+      if (ix < 0 || ix >= T.nc[0] || iy < 0 || iy >= T.nc[1]) continue;
+      const uint32_t row =
+          T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] + static_cast<uint32_t>(iy)) *
+                           T.nc[2];
+      const uint32_t e = A.cell_off[row + zhi + 1];
+      for (uint32_t j = A.cell_off[row + zlo] + lane; j < e; j += G) visit(j);
+    }
   }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) {

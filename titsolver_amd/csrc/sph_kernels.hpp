@@ -27,9 +27,13 @@ constexpr int kBlk3 = 4;
 constexpr int kWin3 = kBlk3 + 2 * kStencil;
 constexpr int kChunkRows3 = 8;
 constexpr int kRows3 = kChunkRows3 + 2 * kStencil;
-constexpr int kWords3 = 8;
+constexpr int kWords3 = 6;
 constexpr int kStage3Threads = 256;
-constexpr int kStage3LdsCap = 1536;
+// (a chunk of 256 targets in the 3D dam break spans 5-6 rows of 16 columns,
+// so its window holds about 64 x 9.5 x 2.9 = 1,800 candidates; 2,304 leaves
+// room for 30 % denser regions at 76 KB of LDS per workgroup, which the
+// kernel's VGPR use limits to two workgroups per CU anyway)
+constexpr int kStage3LdsCap = 2304;
 
 struct StageArgs {
   DevParams P;

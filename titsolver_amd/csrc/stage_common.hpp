@@ -149,103 +149,89 @@ __device__ __forceinline__ uint32_t pop_lowest(uint64_t& mask) {
 // the grid, clamped in).  So for a target whose whole 5^d window is interior
 // (no border cell), a column whose grown box is farther than R from the
 // target in the column plane holds no neighbour, and only the rows within
-// sqrt(R^2 - gap^2) + skin/2 of the target can.  The windows are widened by
-// a relative 1e-6 margin; trimming removes only candidates the exact FP64
-// predicate rejects, so the accepted set and its order are unchanged.
-struct Trim {
-  bool on;
-  double x[3];
-  double lo[3], c[3], inv_c[3];
-  double R2, s;
+// sqrt(R^2 - gap^2) + skin/2 of the target can.  Trimming removes only
+// candidates the exact FP64 predicate rejects, so the accepted set and its
+// order are unchanged; it therefore only has to be conservative, and runs in
+// FP32 with widened bounds: the gaps (FP64, then rounded) shrunk by 1e-6
+// relative, R^2 grown by 1e-5 relative, the row bounds widened by 1e-4 cell.
+//
+// The row windows of the NCOL stencil columns are packed 6 bits per column,
+// ten columns per 64-bit word (3-bit offsets of the first and last row from
+// zc - 2; 7 = empty column), held in named words so that no array is indexed
+// at run time.
+template <int DIM, int NCOL>
+struct Windows {
+  uint64_t w0 = 0, w1 = 0, w2 = 0;
+  static_assert(NCOL <= 30, "three words of ten columns");
+  __device__ __forceinline__ void set(int k, uint64_t v) {
+    const uint64_t x = v << (6 * (k % 10));
+    if (k < 10) {
+      w0 |= x;
+    } else if (k < 20) {
+      w1 |= x;
+    } else {
+      w2 |= x;
+    }
+  }
+  __device__ __forceinline__ uint32_t get(int k) const {
+    const uint64_t w = k < 10 ? w0 : (k < 20 ? w1 : w2);
+    return static_cast<uint32_t>(w >> (6 * (k % 10))) & 63u;
+  }
 };
 
-template <int DIM>
-__device__ __forceinline__ Trim make_trim(const DevParams& P, const DevTank& T,
-                                          const double* xa, int cx, int cy, int zc) {
+template <int DIM, int NCOL>
+__device__ __forceinline__ Windows<DIM, NCOL> stencil_windows(const DevParams& P,
+                                                              const DevTank& T,
+                                                              const double* xa, int cx, int cy,
+                                                              int zc) {
   constexpr int L = DIM - 1;
-  Trim t;
+  Windows<DIM, NCOL> W;
   bool inner = cx - kStencil > 0 && cx + kStencil < T.nc[0] - 1 && zc - kStencil > 0 &&
                zc + kStencil < T.nc[L] - 1;
   if constexpr (DIM == 3) inner = inner && cy - kStencil > 0 && cy + kStencil < T.nc[1] - 1;
-  t.on = inner;
+  constexpr uint64_t kFull = 0u | (4u << 3);  // rows zc - 2 .. zc + 2
+  if (!inner) {
 #pragma unroll
-  for (int k = 0; k < DIM; ++k) {
-    t.x[k] = xa[k];
-    t.lo[k] = T.lo[k];
-    t.inv_c[k] = T.inv_c[k];
-    t.c[k] = 1.0 / T.inv_c[k];
+    for (int k = 0; k < NCOL; ++k) W.set(k, kFull);
+    return W;
   }
-  t.R2 = P.R2 * (1.0 + 1e-6);
-  t.s = P.skin_half * (1.0 + 1e-6);
-  return t;
-}
-
-// Row window [w0, w1] of stencil column k; false: no neighbour in column k.
-template <int DIM>
-__device__ __forceinline__ bool trim_window(const Trim& t, int cx, int cy, int zc, int k,
-                                            int& w0, int& w1) {
-  constexpr int L = DIM - 1;
-  w0 = zc - kStencil;
-  w1 = zc + kStencil;
-  if (!t.on) return true;
-  int ixk, iyk = 0;
-  if constexpr (DIM == 2) {
-    ixk = cx - kStencil + k;
-  } else {
-    ixk = cx - kStencil + k / kCols2;
-    iyk = cy - kStencil + k % kCols2;
-  }
+  // gaps from the target to the grown boxes of the five stencil columns of
+  // each column axis
+  constexpr int NC = 2 * kStencil + 1;
+  float gx[NC];
+  [[maybe_unused]] float gy[NC];
   auto gap = [&](int ax, int ic) {
-    const double a0 = t.lo[ax] + ic * t.c[ax] - t.s;
-    const double a1 = t.lo[ax] + (ic + 1) * t.c[ax] + t.s;
-    return fmax(0.0, fmax(a0 - t.x[ax], t.x[ax] - a1));
+    const double c = 1.0 / T.inv_c[ax];
+    const double a0 = T.lo[ax] + ic * c - P.skin_half;
+    const double a1 = T.lo[ax] + (ic + 1) * c + P.skin_half;
+    return static_cast<float>(fmax(0.0, fmax(a0 - xa[ax], xa[ax] - a1))) * (1.0f - 1e-6f);
   };
-  const double gx = gap(0, ixk);
-  double g2 = gx * gx;
-  if constexpr (DIM == 3) {
-    const double gy = gap(1, iyk);
-    g2 = g2 + gy * gy;
-  }
-  if (g2 > t.R2) return false;
-  const double half = sqrt(t.R2 - g2) + t.s;
-  const double z = t.x[L] - t.lo[L];
-  w0 = max(w0, static_cast<int>(floor((z - half) * t.inv_c[L] - 1e-6)));
-  w1 = min(w1, static_cast<int>(floor((z + half) * t.inv_c[L] + 1e-6)));
-  return w0 <= w1;
-}
-
-// Pair sums of one target over its stencil, column range by column range.
-// The row windows of the NCOL stencil columns, packed 6 bits per column
-// (3-bit offsets of the first and last row from zc - 2; 7 = empty column), so
-// no FP64 trimming state stays live through the pair loop.
-template <int DIM, int NCOL>
-struct Windows {
-  uint32_t w[(NCOL * 6 + 31) / 32];
-  __device__ __forceinline__ void set(int k, uint32_t v) {
-    const int bit = 6 * k;
-    w[bit >> 5] |= v << (bit & 31);
-    if ((bit & 31) > 26) w[(bit >> 5) + 1] |= v >> (32 - (bit & 31));
-  }
-  __device__ __forceinline__ uint32_t get(int k) const {
-    const int bit = 6 * k;
-    uint32_t v = w[bit >> 5] >> (bit & 31);
-    if ((bit & 31) > 26) v |= w[(bit >> 5) + 1] << (32 - (bit & 31));
-    return v & 63u;
-  }
-};
-
-template <int DIM, int NCOL>
-__device__ __forceinline__ Windows<DIM, NCOL> stencil_windows(const Trim& tr, int cx, int cy,
-                                                              int zc) {
-  Windows<DIM, NCOL> W;
 #pragma unroll
-  for (int q = 0; q < (NCOL * 6 + 31) / 32; ++q) W.w[q] = 0u;
+  for (int q = 0; q < NC; ++q) {
+    gx[q] = gap(0, cx - kStencil + q);
+    if constexpr (DIM == 3) gy[q] = gap(1, cy - kStencil + q);
+  }
+  const float R2 = static_cast<float>(P.R2) * (1.0f + 1e-5f);
+  const float inv_cz = static_cast<float>(T.inv_c[L]);
+  const float zr = static_cast<float>((xa[L] - T.lo[L]) * T.inv_c[L]);  // cell units
+  const float sz = static_cast<float>(P.skin_half * T.inv_c[L]) * (1.0f + 1e-5f) + 1e-4f;
+#pragma unroll
   for (int k = 0; k < NCOL; ++k) {
-    int w0, w1;
-    uint32_t v = 7u | (7u << 3);
-    if (trim_window<DIM>(tr, cx, cy, zc, k, w0, w1)) {
-      v = static_cast<uint32_t>(w0 - (zc - kStencil)) |
-          (static_cast<uint32_t>(w1 - (zc - kStencil)) << 3);
+    float g2;
+    if constexpr (DIM == 2) {
+      g2 = gx[k] * gx[k];
+    } else {
+      g2 = gx[k / NC] * gx[k / NC] + gy[k % NC] * gy[k % NC];
+    }
+    uint64_t v = 7u | (7u << 3);
+    if (!(g2 > R2)) {
+      const float half = __builtin_sqrtf(R2 - g2) * inv_cz * (1.0f + 1e-5f) + sz;
+      const int w0 = max(zc - kStencil, static_cast<int>(floorf(zr - half)));
+      const int w1 = min(zc + kStencil, static_cast<int>(floorf(zr + half)));
+      if (w0 <= w1) {
+        v = static_cast<uint64_t>(w0 - (zc - kStencil)) |
+            (static_cast<uint64_t>(w1 - (zc - kStencil)) << 3);
+      }
     }
     W.set(k, v);
   }
@@ -866,7 +852,7 @@ __device__ __forceinline__ void target_step(const StageArgs& A, const DevTank& T
     Sums<DIM> acc;
     acc.init();
     const Windows<DIM, NCOL> W =
-        stencil_windows<DIM, NCOL>(make_trim<DIM>(P, T, xa, cx, cy, zc), cx, cy, zc);
+        stencil_windows<DIM, NCOL>(P, T, xa, cx, cy, zc);
     uint32_t count = 0;
     if constexpr (std::is_same_v<Pos, LdsPos>) {
       gather_words<KERNEL, FLAVOUR, NCOL>(P, ps, A.in, a, i, zc, W, acc, count);

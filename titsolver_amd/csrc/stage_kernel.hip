@@ -186,43 +186,54 @@ __global__ __launch_bounds__(NT) void k_stage_lds3(StageArgs A) {
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      uint32_t lb = 0, nt = 0;
+    static_assert(NW == 64, "one lane of the first wave per window column");
+    if (tid < 64) {
+      // wave 0: LDS offsets of the window columns (lane = column) and the
+      // targets of the block's columns (lanes 0..NB-1), by shuffle scans
+      uint32_t len = 0, nb = 0, first = 0;
       if (rows_fit) {
-        for (int kw = 0; kw < NW; ++kw) {
-          s_lb[kw] = lb;
-          s_delta[kw] = s_coff[kw][0] - lb;
-          lb += s_coff[kw][nrows] - s_coff[kw][0];
+        len = s_coff[tid][nrows] - s_coff[tid][0];
+        if (tid < NB) {
+          const int kw = (kStencil + tid / kBlk3) * kWin3 + kStencil + tid % kBlk3;
+          first = s_coff[kw][c.zs - zlo];
+          nb = s_coff[kw][c.ze - zlo] - first;
         }
-        s_lb[NW] = lb;
-        for (int b = 0; b < NB; ++b) {
-          const int kw = (kStencil + b / kBlk3) * kWin3 + kStencil + b % kBlk3;
-          s_tb[b] = nt;
-          s_first[b] = s_coff[kw][c.zs - zlo];
-          nt += s_coff[kw][c.ze - zlo] - s_coff[kw][c.zs - zlo];
-        }
-        s_tb[NB] = nt;
       }
-      s_fits = rows_fit && lb <= A.lds_cap && nt <= static_cast<uint32_t>(NT);
-      if (!s_fits) A.boverflow[atomicAdd(A.counts + 3, 1u)] = q;
+      uint32_t sl = len, sb = nb;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t yl = __shfl_up(sl, off), yb = __shfl_up(sb, off);
+        if (tid >= off) {
+          sl += yl;
+          sb += yb;
+        }
+      }
+      s_lb[tid] = sl - len;
+      s_delta[tid] = s_coff[tid][0] - (sl - len);
+      if (tid < NB) {
+        s_tb[tid] = sb - nb;
+        s_first[tid] = first;
+      }
+      const uint32_t lb = __shfl(sl, 63), nt = __shfl(sb, 63);
+      if (tid == 0) {
+        s_lb[NW] = lb;
+        s_tb[NB] = nt;
+        s_fits = rows_fit && lb <= A.lds_cap && nt <= static_cast<uint32_t>(NT);
+        if (!s_fits) A.boverflow[atomicAdd(A.counts + 3, 1u)] = q;
+      }
     }
     __syncthreads();
     if (!s_fits) continue;
-    const uint32_t total = s_lb[NW];
-    for (uint32_t l = tid; l < total; l += NT) {
-      int lo = 0, hi = NW - 1;  // the window column holding l: s_lb[kw] <= l < s_lb[kw + 1]
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_lb[mid] <= l) {
-          lo = mid;
-        } else {
-          hi = mid - 1;
-        }
+    // each window column's candidates by 32 lanes (a column holds about 30)
+    for (int r = tid; r < NW * 32; r += NT) {
+      const int kw = r >> 5;
+      const uint32_t e = s_lb[kw + 1], d = s_delta[kw];
+      for (uint32_t l = s_lb[kw] + (r & 31); l < e; l += 32) {
+        const uint32_t j = l + d;
+        s_x[l] = A.in.x[0][j];
+        s_y[l] = A.in.x[1][j];
+        s_z[l] = A.in.x[2][j];
       }
-      const uint32_t j = l + s_delta[lo];
-      s_x[l] = A.in.x[0][j];
-      s_y[l] = A.in.x[1][j];
-      s_z[l] = A.in.x[2][j];
     }
     __syncthreads();
     const LdsPos3 ps{s_x, s_y, s_z, s_coff, s_lb, s_delta, zlo, nz, wx0, wy0,

@@ -1504,10 +1504,10 @@ tit_status tit_hip_learned_train(tit_hip_ctx* c, int iters, double lr, double* l
 }
 
 tit_status tit_hip_learned_run(tit_hip_ctx* c, int n_steps, int judge_every, double tol,
-                               tit_hip_learned_report* rep) {
+                               double dv_tol, tit_hip_learned_report* rep) {
   tit_status st = need_learned(c);
   if (st != TIT_OK) return st;
-  if (n_steps < 0 || judge_every < 1 || !(tol >= 0.0)) {
+  if (n_steps < 0 || judge_every < 1 || !(tol >= 0.0) || !(dv_tol >= 0.0)) {
     return fail(c, TIT_E_ARG, "bad learned-run arguments");
   }
   TIT_HIP_TRY(c, hipSetDevice(c->device));
@@ -1589,7 +1589,7 @@ tit_status tit_hip_learned_run(tit_hip_ctx* c, int n_steps, int judge_every, dou
     R.last_dv_err = dverr;
     R.last_rho_err = err;
     R.plain_steps += kPlainPerLearned;
-    if (!(static_cast<double>(err) <= tol)) {
+    if (!(static_cast<double>(err) <= tol) || !(dverr <= dv_tol)) {
       // fallback: back to the last accepted state, plain steps over the group
       R.rejected += 1;
       TIT_HIP_TRY(c, restore(c, L->accepted));

@@ -505,10 +505,16 @@ class LearnedStep:
     def set_fused(self, on: bool):
         self.ctx._check(self.lib.tit_hip_learned_set_fused(self.ctx.h, 1 if on else 0))
 
-    def run(self, n_steps: int, judge_every: int = 10, tol: float = 0.01) -> dict:
+    # Default judge tolerances: 1 % in density, and half of the velocity change
+    # of the judged step (an untrained network's error is about 1).
+    RHO_TOL = 0.01
+    DV_TOL = 0.5
+
+    def run(self, n_steps: int, judge_every: int = 10, tol: float = RHO_TOL,
+            dv_tol: float = DV_TOL) -> dict:
         rep = _abi.LearnedReport()
         self.ctx._check(self.lib.tit_hip_learned_run(self.ctx.h, n_steps, judge_every, tol,
-                                                     C.byref(rep)))
+                                                     dv_tol, C.byref(rep)))
         return {k: getattr(rep, k) for k, _ in rep._fields_}
 
 
