@@ -17,8 +17,8 @@
  *
  * Differences from the reference, each forced by the device: the particle
  * state lives on the device between steps (step() uploads it on its first
- * call; sync() copies it back, upload() replaces it); the mass is uniform over
- * the fluid; fluid particles come before fixed ones, as the reference's
+ * call; sync() copies it back, upload() replaces it); masses that vary
+ * between particles need 2D; fluid particles come before fixed ones, as the reference's
  * ParticleArray keeps them.  Errors throw Exception, as TIT_ENSURE does
  * (core/exception.hpp:77-85).
  */
@@ -145,8 +145,8 @@ struct FluidEquations {
 };
 
 /// ParticleArray (particle_array.hpp:138-293) in the reference layout:
-/// fluid particles first, then fixed.  Uniform h; per-particle m (uniform
-/// over the fluid); r, v, rho, p; wall normals of fixed particles when no
+/// fluid particles first, then fixed.  Uniform h; per-particle m (varying
+/// masses in 2D only); r, v, rho, p; wall normals of fixed particles when no
 /// wall domain is given.
 template<std::size_t Dim>
 class ParticleArray {
@@ -223,6 +223,10 @@ public:
     view.normal = eq_.domain ? nullptr : nrm.data();
     view.n_fluid = particles.num_fluid();
     view.n_fixed = particles.num_fixed();
+    // per-particle masses only when they vary (uniform: params.m)
+    const bool varying = std::any_of(particles.m.begin(), particles.m.end(),
+                                     [&](double x) { return x != particles.m.front(); });
+    view.m = varying ? particles.m.data() : nullptr;
     // the cell grid covers the particles' box grown by four kernel widths;
     // particles that leave it are clamped into its border cells
     tit_hip_tank box{};
@@ -269,11 +273,6 @@ private:
   void create_(const ParticleArray<Dim>& particles) {
     if (particles.num_fluid() == 0) throw Exception(TIT_E_ARG, "no fluid particles");
     const double m0 = particles.m.front();
-    for (std::size_t a = 0; a < particles.num_fluid(); ++a) {
-      if (particles.m[a] != m0) {
-        throw Exception(TIT_E_ARG, "the device path takes a uniform fluid mass");
-      }
-    }
     if (ctx_) {
       tit_hip_destroy(ctx_);
       ctx_ = nullptr;

@@ -89,6 +89,13 @@ typedef struct tit_hip_host_view {
                            (ignored, may be NULL, with tit_hip_set_walls) */
   size_t n_fluid;
   size_t n_fixed;
+  const double* m;      /* n_fluid + n_fixed particle masses, ParticleArray's m
+                           (fluid_equations.hpp:41-48), or NULL: params.m for
+                           every particle.  Positive and finite; 2D only.  With
+                           tit_hip_set_walls the fixed particles' masses are
+                           multiplied by gamma, as FluidEquations::initialize
+                           does (fluid_equations.hpp:87-88).  The learned step
+                           needs uniform masses (NULL). */
 } tit_hip_host_view;
 
 /* Wall boundary of the semi-analytic formulation, 2D: the tessellated wall

@@ -145,6 +145,17 @@ def params(eq, walls=None) -> OrcParams:
     return p
 
 
+def winding(cverts, cfaces, pts) -> np.ndarray:
+    """orc_winding at each point: the exact winding sum of the polygon
+    (cverts, cfaces)."""
+    from types import SimpleNamespace
+    w = Walls(SimpleNamespace(verts=np.zeros((0, 2)), faces=np.zeros((0, 2)),
+                              cverts=cverts, cfaces=cfaces))
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
+    fn = load().orc_winding
+    return np.array([fn(C.byref(w.c), _dp(pts[i])) for i in range(len(pts))])
+
+
 def pressure(prm, rho):
     """Tait pressure of each density (FP64)."""
     rho = np.ascontiguousarray(rho, dtype=np.float64)
@@ -192,7 +203,7 @@ def naive_search(pts, radius, strict=False):
 class State:
     """Particle state in the reference layout, owned by numpy arrays."""
 
-    def __init__(self, r, rho, n_fluid, normal, v=None, dv_dt=None):
+    def __init__(self, r, rho, n_fluid, normal, v=None, dv_dt=None, mass=None):
         r = np.ascontiguousarray(r, dtype=np.float64)
         self.n, self.dim = r.shape
         self.n_fluid = int(n_fluid)
@@ -208,7 +219,10 @@ class State:
             self.normal = np.zeros((1, self.dim))
         self.gamma = np.ones(self.n)
         self.grad_gamma = np.zeros_like(r)
-        self.mass = np.zeros(self.n)
+        # per-particle masses; zero entries mean the parameters' uniform m
+        self.mass = (np.zeros(self.n) if mass is None
+                     else np.ascontiguousarray(mass, dtype=np.float64).copy())
+        assert self.mass.shape == (self.n,)
         self.c = OrcState(self.n_fluid, self.n - self.n_fluid, _dp(self.r), _dp(self.v),
                           _dp(self.rho), _dp(self.drho_dt), _dp(self.dv_dt),
                           _dp(self.p), _dp(self.cs), _dp(self.normal), _dp(self.gamma),
@@ -216,7 +230,8 @@ class State:
 
     @classmethod
     def from_particles(cls, pa):
-        return cls(pa.r, pa.rho, pa.n_fluid, pa.normal, pa.v, pa.dv_dt)
+        return cls(pa.r, pa.rho, pa.n_fluid, pa.normal, pa.v, pa.dv_dt,
+                   getattr(pa, "m", None))
 
 
 def _csr(off, idx):

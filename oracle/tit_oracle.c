@@ -721,21 +721,25 @@ int orc_compute_gamma(const orc_params* prm, orc_state* s) {
 }
 
 /* FluidEquations::initialize (fluid_equations.hpp:75-89): gamma, then the
- * fixed particles' masses become m gamma. */
+ * fixed particles' masses are multiplied by gamma (:88).  Masses the caller
+ * left at zero are the uniform m first. */
 int orc_initialize(const orc_params* prm, orc_state* s) {
   if (!s->mass) return -1;
   const size_t n = s->n_fluid + s->n_fixed;
-  for (size_t i = 0; i < n; ++i) s->mass[i] = prm->m;
+  for (size_t i = 0; i < n; ++i) {
+    if (!(s->mass[i] > 0.0)) s->mass[i] = prm->m;
+  }
   if (!prm->walls) return 0;
   if (prm->walls->n_verts != s->n_fixed) return -1;
   if (orc_compute_gamma(prm, s)) return -1;
-  for (size_t e = s->n_fluid; e < n; ++e) s->mass[e] = prm->m * s->gamma[e];
+  for (size_t e = s->n_fluid; e < n; ++e) s->mass[e] *= s->gamma[e];
   return 0;
 }
 
-/* Mass of particle i: per particle once initialised with walls, else m. */
+/* Mass of particle i, ParticleArray's m (fluid_equations.hpp:41-48): the
+ * per-particle value where one is set (positive), else the uniform m. */
 static double mass_of(const orc_params* prm, const orc_state* s, size_t i) {
-  return (prm->walls && s->mass) ? s->mass[i] : prm->m;
+  return (s->mass && s->mass[i] > 0.0) ? s->mass[i] : prm->m;
 }
 
 /* ~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~~ */
@@ -847,7 +851,6 @@ int orc_compute_continuity(const orc_params* prm, orc_state* s,
                            const uint64_t* off, const uint32_t* idx) {
   const int dim = prm->dim;
   const size_t n = s->n_fluid + s->n_fixed;
-  const double m = prm->m;
   for (size_t a = 0; a < n; ++a) s->cs[a] = orc_eos_sound_speed(prm, s->rho[a]);
   for (size_t a = 0; a < n; ++a) s->drho_dt[a] = 0.0;
   if (prm->walls) {
@@ -876,8 +879,9 @@ int orc_compute_continuity(const orc_params* prm, orc_state* s,
           ta[k] = v_ab[k] + Psi[k] / s->rho[b];
           tb[k] = -v_ab[k] + Psi[k] / s->rho[a];
         }
-        s->drho_dt[a] += mass_of(prm, s, b) / gamma_of(prm, s, a) * dot(dim, ta, grad);
-        s->drho_dt[b] -= mass_of(prm, s, a) / gamma_of(prm, s, b) * dot(dim, tb, grad);
+        const int swap = g_mutation == 14; /* planted: the target's own mass */
+        s->drho_dt[a] += mass_of(prm, s, swap ? a : b) / gamma_of(prm, s, a) * dot(dim, ta, grad);
+        s->drho_dt[b] -= mass_of(prm, s, swap ? b : a) / gamma_of(prm, s, b) * dot(dim, tb, grad);
       } else {
         /* delta-SPH (Molteni & Colagrossi 2009):
          *   psi_ab = 2 (rho_a - rho_b) r_ab / |r_ab|^2,
@@ -889,8 +893,10 @@ int orc_compute_continuity(const orc_params* prm, orc_state* s,
         for (int k = 0; k < dim; ++k) psi[k] = coef * r_ab[k];
         const double dhc = prm->delta * prm->h * prm->cs0;
         const double vg = dot(dim, v_ab, grad), pg = dot(dim, psi, grad);
-        s->drho_dt[a] += m * vg + dhc * (m / s->rho[b]) * pg;
-        s->drho_dt[b] += m * vg - dhc * (m / s->rho[a]) * pg;
+        const int swap = g_mutation == 14; /* planted: the target's own mass */
+        const double m_a = mass_of(prm, s, swap ? b : a), m_b = mass_of(prm, s, swap ? a : b);
+        s->drho_dt[a] += m_b * vg + dhc * (m_b / s->rho[b]) * pg;
+        s->drho_dt[b] += m_a * vg - dhc * (m_a / s->rho[a]) * pg;
       }
     }
   }
@@ -945,8 +951,9 @@ int orc_compute_momentum(const orc_params* prm, orc_state* s,
                   (rho_m * (norm2(dim, r_ab) + eta2));
         }
       }
-      double fa = mass_of(prm, s, b) / gamma_of(prm, s, a) * (Pi_ab - P_ab);
-      const double fb = mass_of(prm, s, a) / gamma_of(prm, s, b) * (Pi_ab - P_ab);
+      const int swap = g_mutation == 15; /* planted: the target's own mass */
+      double fa = mass_of(prm, s, swap ? a : b) / gamma_of(prm, s, a) * (Pi_ab - P_ab);
+      const double fb = mass_of(prm, s, swap ? b : a) / gamma_of(prm, s, b) * (Pi_ab - P_ab);
       if (g_mutation == 1) fa = m * (Pi_ab + P_ab); /* planted: sign slip */
       for (int k = 0; k < dim; ++k) {
         s->dv_dt[a * dim + k] += fa * grad[k];

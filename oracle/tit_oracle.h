@@ -58,7 +58,7 @@ typedef struct orc_params {
   int flavour;  /* ORC_FLAVOUR_* */
   int post_integrate; /* 1: orc_ssprk_step ends with orc_post_integrate */
   double h;       /* kernel width (uniform field h) */
-  double m;       /* particle mass (uniform) */
+  double m;       /* particle mass where the state sets none */
   double rho0;    /* reference density */
   double cs0;     /* reference sound speed */
   double xi;      /* Tait index */
@@ -85,7 +85,8 @@ typedef struct orc_state {
   const double* normal; /* n_fixed * dim, wall normals of fixed particles */
   double* gamma;        /* n (walls only) */
   double* grad_gamma;   /* n * dim (walls only) */
-  double* mass;         /* n: per-particle masses (walls only, orc_initialize) */
+  double* mass;         /* n: per-particle masses, ParticleArray's m; an entry
+                           <= 0 (or a null array) means the uniform m */
 } orc_state;
 
 /* ---- smoothing kernel ---------------------------------------------------- */
@@ -145,8 +146,9 @@ int orc_face_intersects(const double* a, const double* b, const double* c, doubl
 double orc_winding(const orc_walls* w, const double* x);
 /* compute_gamma (fluid_equations.hpp:168-193) for every particle. */
 int orc_compute_gamma(const orc_params* prm, orc_state* s);
-/* FluidEquations::initialize (fluid_equations.hpp:75-89): s->mass = m, and
- * with walls gamma and the fixed particles' masses m gamma. */
+/* FluidEquations::initialize (fluid_equations.hpp:75-89): masses left at
+ * zero become m; with walls gamma, and the fixed particles' masses times
+ * gamma. */
 int orc_initialize(const orc_params* prm, orc_state* s);
 /* ---- post-integration (particle shifting, free-surface correction) ------- */
 /* Optional outputs of orc_post_integrate (any pointer may be null). */
@@ -186,8 +188,10 @@ int orc_ssprk_generic(int order, double dt, size_t n, double* u,
  * fluid particle, 8: free-surface blend exponent with the wrong sign,
  * 9: wall face normals reversed in the kernel flux, 10: wall face terms of
  * the momentum equation dropped, 11: wall face terms of apply_shifts
- * dropped); 0
- * restores the faithful oracle. */
+ * dropped, 12: anti-diffusive Ferrari term, 13: physical viscosity without
+ * its factor 2, 14: continuity weighted by the target's own mass instead of
+ * the neighbour's, 15: momentum weighted by the target's own mass instead of
+ * the neighbour's); 0 restores the faithful oracle. */
 void orc_set_mutation(int m);
 
 #ifdef __cplusplus

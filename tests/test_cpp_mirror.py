@@ -19,8 +19,9 @@ EXE = ROOT / "tests" / "cpp" / "build" / "dam_break_mirror"
 
 
 def _exe():
-    if not EXE.exists():
-        subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "cpp")], check=True)
+    # make rebuilds the program when the headers or the library changed (a
+    # program built against an older tit_hip_host_view would misread it)
+    subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "cpp")], check=True)
     return str(EXE)
 
 

@@ -45,6 +45,7 @@ class HostView(C.Structure):
         ("rho", C.POINTER(C.c_double)), ("dv_dt", C.POINTER(C.c_double)),
         ("normal", C.POINTER(C.c_double)),
         ("n_fluid", C.c_size_t), ("n_fixed", C.c_size_t),
+        ("m", C.POINTER(C.c_double)),
     ]
 
 

@@ -140,6 +140,31 @@ double kernel_omega(int kernel, int dim) {
   return dim == 2 ? 9.0 / (4.0 * pi) : 495.0 / (256.0 * pi);
 }
 
+// Picks the pair-term flavour (walls, per-particle masses, or the
+// parameters' own) and folds its uniform factors (DevParams::mg, kd, kv).
+void set_flavour(tit_hip_ctx* c, bool masses) {
+  const tit_hip_params& p = c->prm;
+  DevParams& d = c->dp;
+  if (c->walls || (masses && p.flavour == TIT_FLAVOUR_REF)) {
+    d.flavour = tit_amd::kFlavourWall;
+  } else if (masses) {
+    d.flavour = tit_amd::kFlavourCfgM;
+  } else {
+    d.flavour = p.flavour;
+  }
+  const double omega = kernel_omega(p.kernel, p.dim);
+  const double c_w = p.kernel == TIT_KERNEL_WENDLAND_C2 ? -5.0 : -14.0 / 3.0;
+  const double gsc = c_w * omega / std::pow(p.h, p.dim + 2);
+  d.mg = static_cast<float>(p.m * gsc);
+  if (p.flavour == TIT_FLAVOUR_CFG) {
+    d.kd = static_cast<float>(2.0 * p.delta * p.h * p.cs0 * gsc);
+    d.kv = static_cast<float>(masses ? 0.5 * p.alpha * p.h : 0.5 * p.alpha * p.h * p.m * gsc);
+  } else {
+    d.kd = static_cast<float>(gsc);
+    d.kv = static_cast<float>(2.0 * p.mu * p.m * gsc);
+  }
+}
+
 tit_status fill_params(tit_hip_ctx* c, const tit_hip_params& p) {
   if (p.dim != 2 && p.dim != 3) return fail(c, TIT_E_ARG, "dim must be 2 or 3");
   if (p.kernel != TIT_KERNEL_WENDLAND_C2 && p.kernel != TIT_KERNEL_WENDLAND_C4) {
@@ -201,16 +226,8 @@ tit_status fill_params(tit_hip_ctx* c, const tit_hip_params& p) {
   d.inv_rho0_f = static_cast<float>(1.0 / p.rho0);
   d.cs0_f = static_cast<float>(p.cs0);
   d.rho0_f = static_cast<float>(p.rho0);
-  const double gsc = c_w * omega / std::pow(p.h, p.dim + 2);
   d.half_inv_h = static_cast<float>(0.5 / p.h);
-  d.mg = static_cast<float>(p.m * gsc);
-  if (p.flavour == TIT_FLAVOUR_CFG) {
-    d.kd = static_cast<float>(2.0 * p.delta * p.h * p.cs0 * gsc);
-    d.kv = static_cast<float>(0.5 * p.alpha * p.h * p.m * gsc);
-  } else {
-    d.kd = static_cast<float>(gsc);
-    d.kv = static_cast<float>(2.0 * p.mu * p.m * gsc);
-  }
+  set_flavour(c, false);
   return TIT_OK;
 }
 
@@ -434,6 +451,7 @@ StageArgs stage_args(tit_hip_ctx* c) {
   a.tanks = c->tanks;
   a.key = c->key;
   a.tag = c->tag;
+  a.id = c->id;
   a.cell_off = c->cell_off;
   a.dt = c->dt;
   a.n = c->n;
@@ -610,7 +628,7 @@ std::map<TuneKey, TuneVal> g_tune;
 TuneKey tune_key(const tit_hip_ctx* c) {
   int log2n = 0;
   while ((static_cast<size_t>(1) << (log2n + 1)) <= c->n) ++log2n;
-  return TuneKey{c->prm.dim, c->prm.kernel, c->prm.flavour, c->walls ? 1 : 0,
+  return TuneKey{c->prm.dim, c->prm.kernel, c->dp.flavour, c->walls ? 1 : 0,
                  c->n_tanks > 1 ? 1 : 0, log2n};
 }
 
@@ -712,7 +730,7 @@ tit_status tit_hip_set_walls(tit_hip_ctx* c, const tit_hip_walls* w) {
   c->wall_cverts.assign(w->cverts, w->cverts + 2 * static_cast<size_t>(w->n_cverts));
   c->wall_cfaces.assign(w->cfaces, w->cfaces + 2 * static_cast<size_t>(w->n_cfaces));
   c->walls = true;
-  c->dp.flavour = tit_amd::kFlavourWall;
+  set_flavour(c, false);
   return TIT_OK;
 }
 
@@ -830,6 +848,29 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
       return fail(c, TIT_E_ARG, "density must be positive and finite");
     }
   }
+  // Per-particle masses (ParticleArray's m); tanks without them take params.m.
+  // Masses that all equal params.m take the uniform-mass terms.  2D only: the
+  // pair record's third slot carries V = m / rho there.
+  std::vector<double> hm;
+  bool any_m = false;
+  {
+    size_t q = 0;
+    for (size_t t = 0; t < n_tanks; ++t) {
+      const size_t nt = views[t].n_fluid + views[t].n_fixed;
+      if (views[t].m) {
+        if (hm.empty()) hm.assign(n_total, c->prm.m);
+        std::memcpy(&hm[q], views[t].m, nt * sizeof(double));
+      }
+      q += nt;
+    }
+    for (double x : hm) {
+      if (!(x > 0.0) || !std::isfinite(x)) {
+        return fail(c, TIT_E_ARG, "masses must be positive and finite");
+      }
+      any_m = any_m || x != c->prm.m;
+    }
+    if (any_m && dim != 2) return fail(c, TIT_E_ARG, "per-particle masses need dim 2");
+  }
 
   // Semi-analytic wall: faces, face grid, and the fixed particles' gamma (their
   // mass fractions) and grad gamma (their wall normals), FluidEquations::
@@ -919,6 +960,8 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->dt, n_tanks));
   TIT_HIP_TRY(c, dalloc(c, &c->time, n_tanks));
   TIT_HIP_TRY(c, dalloc(c, &c->disp, n_tanks));
+  c->mass = nullptr;
+  if (any_m) TIT_HIP_TRY(c, dalloc(c, &c->mass, n));
   // per-tank reduction slots: kRedStripes copies per tank
   const size_t n_red = static_cast<size_t>(n_tanks) * kRedStripes;
   TIT_HIP_TRY(c, dalloc(c, &c->dt_min, n_red));
@@ -995,6 +1038,12 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
                                 hipMemcpyHostToDevice, s));
   TIT_HIP_TRY(c, hipMemcpyAsync(c->tanks, ht.data(), ht.size() * sizeof(DevTank),
                                 hipMemcpyHostToDevice, s));
+  if (any_m) {
+    TIT_HIP_TRY(c, hipMemcpyAsync(c->mass, hm.data(), hm.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, s));
+  }
+  c->dp.mass = c->mass;
+  set_flavour(c, any_m);
   launch_import(s, c->dp, n, c->stage_r, c->stage_v, c->stage_rho, c->A, c->id);
   std::vector<double> zeros(n_tanks, 0.0);
   std::vector<unsigned long long> inf(static_cast<size_t>(n_tanks) * kRedStripes, ~0ull);

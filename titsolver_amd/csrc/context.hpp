@@ -82,6 +82,7 @@ struct tit_hip_ctx {
   double wall_lo[2] = {0.0, 0.0}, wall_inv_cw[2] = {0.0, 0.0};
   int wall_nc[2] = {0, 0};
   double* wall_mu = nullptr;   // per vertex: gamma at set-up (mass fraction)
+  double* mass = nullptr;      // per particle by id: uploaded masses (null: uniform m)
   double* wall_rho = nullptr;  // per vertex: density of this stage
   double* wall_out = nullptr;  // per slot: face terms / gamma, gamma
   uint32_t* wall_near = nullptr;  // per slot: list of particles near a wall

@@ -22,8 +22,12 @@ constexpr int kFlavourRef = 0;
 constexpr int kFlavourCfg = 1;
 // REF terms with the semi-analytic wall (2D): per-particle masses (fixed
 // particles carry m gamma), pair sums divided by gamma_a, and the face terms
-// added per particle.
+// added per particle.  Without walls (gamma = 1, no face terms) this is the
+// REF flavour with per-particle masses.
 constexpr int kFlavourWall = 2;
+// CFG terms with per-particle masses (2D): m_b = V_b rho_b from the pair
+// record multiplies the whole term, as in kFlavourWall.
+constexpr int kFlavourCfgM = 3;
 
 // Uniform parameters, passed to every kernel by value.
 struct DevParams {
@@ -35,6 +39,7 @@ struct DevParams {
   double eos_B;     // rho0 cs0^2 / xi
   double inv_rho0;
   double skin_half; // allowed displacement since the last cell rebuild
+  const double* mass; // per-particle masses by particle id, or null (uniform m)
   float inv_h;      // 1 / h
   float gsc;        // grad W_ab = gsc * P(q) * (r_a - r_b)
   float wsc;        // W_ab = wsc * w(q)
@@ -50,7 +55,8 @@ struct DevParams {
   float half_inv_h; // 1 / (2h)
   float mg;         // m gsc
   float kd;         // CFG: 2 delta h cs0 gsc; REF: gsc
-  float kv;         // CFG: alpha h m gsc / 2; REF: 2 mu m gsc
+  float kv;         // CFG: alpha h m gsc / 2; CFG with masses: alpha h / 2;
+                    // REF: 2 mu m gsc
 };
 
 // Per-tank cell grid.  Cell size is at least (support radius + skin) / 2, so

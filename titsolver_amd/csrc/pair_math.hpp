@@ -13,7 +13,8 @@
 //     REF  Pi_ab = 2 mu (v_ab . r_ab) / (rho_a rho_b |r_ab|^2)   (:299-300)
 // The uniform factors m gsc, kd, kv are folded in once (FP64, then rounded),
 // so a pair costs the arithmetic that varies per pair only.  With walls
-// (kFlavourWall) the masses differ per particle (fixed particles carry
+// (kFlavourWall) and with per-particle masses (kFlavourWall without walls,
+// kFlavourCfgM) the masses differ per particle (fixed particles carry
 // m gamma, fluid_equations.hpp:87-88): m_b = V_b rho_b comes from the pair
 // record (V_b = m_b / rho_b in its third slot) and multiplies the whole term.
 #pragma once
@@ -46,7 +47,7 @@ struct Target {
   float kd;  // kd (rho_a - rho0)
   float rb;  // CFG: (rho_a + rho0) / 2; REF: rho_a
   float cv;  // CFG: kv cs_a; REF: cs_a
-  float pa;  // mg p_a / rho_a^2
+  float pa;  // mg p_a / rho_a^2 (p_a / rho_a^2 with per-particle masses)
 };
 
 // Compensated per-particle sums.  Pair terms enter in twos: the FP32 sum of
@@ -106,22 +107,31 @@ __device__ __forceinline__ float2 make_aux(const DevParams& P, double rho) {
                      sound_speed_f32(P, static_cast<float>(rho)));
 }
 
+// Mass of the particle with id `id` (its index in the uploaded order):
+// ParticleArray's m (fluid_equations.hpp:41-48) when per-particle masses were
+// uploaded, else the uniform m.
+__device__ __forceinline__ double particle_mass(const DevParams& P, uint32_t id) {
+  return P.mass ? P.mass[id] : P.m;
+}
+
 // (v, rho - rho0) in FP32 for the neighbours' side of the pair terms; in 2D
 // the third slot holds the particle volume m / rho instead of v_z.
 template <int DIM>
-__device__ __forceinline__ float4 make_pv(const DevParams& P, const double* v, double rho) {
+__device__ __forceinline__ float4 make_pv(const DevParams& P, const double* v, double rho,
+                                          double m) {
   return make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
                      DIM == 3 ? static_cast<float>(v[DIM - 1])
-                              : static_cast<float>(P.m / rho),
+                              : static_cast<float>(m / rho),
                      static_cast<float>(rho - P.rho0));
 }
 
-// Writes both per-particle pair records of a particle.
+// Writes both per-particle pair records of a particle of mass m.
 template <int DIM>
 __device__ __forceinline__ void write_pair_data(const DevParams& P, const DevState& st,
-                                                uint32_t i, const double* v, double rho) {
+                                                uint32_t i, const double* v, double rho,
+                                                double m) {
   st.aux[i] = make_aux(P, rho);
-  st.pv[i] = make_pv<DIM>(P, v, rho);
+  st.pv[i] = make_pv<DIM>(P, v, rho, m);
 }
 
 template <int DIM, int FLAVOUR>
@@ -135,14 +145,14 @@ __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
   }
   const float drho = static_cast<float>(rho - P.rho0);
   a.kd = P.kd * drho;
-  if constexpr (FLAVOUR == kFlavourCfg) {
+  if constexpr (FLAVOUR == kFlavourCfg || FLAVOUR == kFlavourCfgM) {
     a.rb = 0.5f * (P.rho0_f + drho) + 0.5f * P.rho0_f;
     a.cv = P.kv * aux.y;
   } else {
     a.rb = P.rho0_f + drho;
     a.cv = aux.y;
   }
-  a.pa = FLAVOUR == kFlavourWall ? aux.x : P.mg * aux.x;
+  a.pa = (FLAVOUR == kFlavourWall || FLAVOUR == kFlavourCfgM) ? aux.x : P.mg * aux.x;
 }
 
 // The kernel-gradient polynomial P(q) of grad W_ab = gsc P(q) r_ab as a
@@ -193,6 +203,17 @@ __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>&
     c = __builtin_fmaf(P.mg, vr, fmaxf(a.cv, csb) * drk * Vb * r);
     const float rb = P.rho0_f + pvb.w;
     Pi = (P.kv * vr) * __builtin_amdgcn_rcpf(a.rb * rb * r2);
+  } else if constexpr (FLAVOUR == kFlavourCfgM) {  // m_b gsc = V_b rho_b gsc
+    const float mb = Vb * (P.rho0_f + pvb.w) * P.gsc;
+    c = __builtin_fmaf(mb, vr, drk * Vb);
+    const float rbar = __builtin_fmaf(pvb.w, 0.5f, a.rb);
+    const float num = __builtin_fmaf(csb, P.kv, a.cv) * fminf(vr, 0.0f);
+    Pi = num * __builtin_amdgcn_rcpf(rbar * (r2 + P.eta2));
+    cont = c * G;
+    const float coef = ((Pi - a.pa) - Pb) * (mb * G);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) mom[k] = coef * x[k];
+    return;
   } else {  // kFlavourWall: m_b gsc = V_b rho_b gsc (kd = gsc here)
     const float rb = P.rho0_f + pvb.w;
     const float mb = Vb * rb * P.kd;

@@ -374,11 +374,15 @@ __device__ __forceinline__ bool lu_inverse(const double* M, double* inv) {
   return true;
 }
 
-// Mass of particle j: m, or m gamma for a fixed particle with walls
-// (FluidEquations::initialize, fluid_equations.hpp:87-88).
+// Mass of particle j: its uploaded mass (or the uniform m), times gamma for a
+// fixed particle with walls (FluidEquations::initialize,
+// fluid_equations.hpp:87-88).
 __device__ __forceinline__ double mass_of(const PiArgs& A, uint32_t j) {
-  if (A.wall_mu && (A.tag[j] & 1u)) return A.P.m * A.wall_mu[A.id[j] - A.wall_id0];
-  return A.P.m;
+  if (!A.P.mass && !A.wall_mu) return A.P.m;
+  const uint32_t id = A.id[j];
+  const double m = particle_mass(A.P, id);
+  if (A.wall_mu && (A.tag[j] & 1u)) return m * A.wall_mu[id - A.wall_id0];
+  return m;
 }
 
 // gamma of fluid particle i (1 without walls; fixed particles' sums are not used).
@@ -824,7 +828,7 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_commit(PiArgs A) {
     }
     const double rho = A.c.rho[i];
     A.a.rho[i] = rho;
-    write_pair_data<DIM>(A.P, A.a, i, v, rho);
+    write_pair_data<DIM>(A.P, A.a, i, v, rho, mass_of(A, i));
     const uint32_t tg = A.tag[i];
     if (!(tg & 1u)) {
       double dv[DIM], v2 = 0.0;

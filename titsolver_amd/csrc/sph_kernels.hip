@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_import(
     st.v[k][i] = vi[k];
   }
   st.rho[i] = rho[i];
-  write_pair_data<DIM>(P, st, i, vi, rho[i]);
+  write_pair_data<DIM>(P, st, i, vi, rho[i], particle_mass(P, i));
   id[i] = i;
 }
 
@@ -188,8 +188,9 @@ __global__ __launch_bounds__(kBlock) void k_permute(
   double v[DIM];
 #pragma unroll
   for (int k = 0; k < DIM; ++k) v[k] = dst.v[k][i];
-  write_pair_data<DIM>(P, dst, i, v, rho);
-  id_dst[i] = id_src[s];
+  const uint32_t pid = id_src[s];
+  write_pair_data<DIM>(P, dst, i, v, rho, particle_mass(P, pid));
+  id_dst[i] = pid;
   tag_dst[i] = tg;
 }
 
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     for (int d = 1; d < DIM; ++d) s2 = s2 + dd[d] * dd[d];
     if (!(s2 <= P.R2)) return;
     const double rho_b = A.s.rho[j];
-    const double V_b = P.m / rho_b;
+    const double V_b = (P.mass ? P.mass[A.id[j]] : P.m) / rho_b;
     const double W = w0 * kernel_value_poly_f64<KERNEL>(h_inv * sqrt(s2));
     double rn = dd[0] * ne[0];
 #pragma unroll
@@ -441,11 +442,12 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     const double rho_e = eos_density_from_potential(P, Hs);
     A.s.rho[i] = rho_e;
     const double zero[DIM] = {};
-    write_pair_data<DIM>(P, A.s, i, zero, rho_e);
+    const double m_e = particle_mass(P, A.id[i]);
+    write_pair_data<DIM>(P, A.s, i, zero, rho_e, m_e);
     if (A.wall_rho) {
       const uint32_t k = A.id[i] - A.wall_id0;
       A.wall_rho[k] = rho_e;
-      A.s.pv[i].z = static_cast<float>(A.wall_mu[k] * P.m / rho_e);  // V_e = m_e / rho_e
+      A.s.pv[i].z = static_cast<float>(A.wall_mu[k] * m_e / rho_e);  // V_e = m_e / rho_e
     }
 #pragma unroll
     for (int d = 0; d < DIM; ++d) A.s.v[d][i] = 0.0;

@@ -43,6 +43,7 @@ struct StageArgs {
   DevState out;      // state written by the stage
   const uint32_t* key;
   const uint32_t* tag;       // tank << 1 | fixed
+  const uint32_t* id;        // particle ids (per-particle masses are by id)
   const uint32_t* cell_off;  // CSR over all cells of all tanks
   const double* dt;          // per tank
   double w;                  // weight of the new state, (1-w) old + w new

@@ -776,12 +776,15 @@ __device__ __forceinline__ void target_step(const StageArgs& A, const DevTank& T
 #pragma unroll
     for (int k = 0; k < DIM; ++k) dv[k] = static_cast<double>(acc.v[k].value());
     if constexpr (FLAVOUR == kFlavourWall) {
-      // face terms (already divided by gamma_a) + pair sums / gamma_a
-      const double* wt = A.wall + 4 * static_cast<size_t>(i);
-      const double ig = 1.0 / wt[3];
-      drho = wt[0] + drho * ig;
+      // face terms (already divided by gamma_a) + pair sums / gamma_a; none
+      // without walls (per-particle masses alone)
+      if (A.wall) {
+        const double* wt = A.wall + 4 * static_cast<size_t>(i);
+        const double ig = 1.0 / wt[3];
+        drho = wt[0] + drho * ig;
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) dv[k] = wt[1 + k] + dv[k] * ig;
+        for (int k = 0; k < DIM; ++k) dv[k] = wt[1 + k] + dv[k] * ig;
+      }
     }
     dv[1] = dv[1] - P.g;
     if constexpr (KIND == kRhsOnly) {
@@ -816,7 +819,7 @@ __device__ __forceinline__ void target_step(const StageArgs& A, const DevTank& T
         A.out.v[k][i] = vn[k];
       }
       A.out.rho[i] = rhon;
-      write_pair_data<DIM>(P, A.out, i, vn, rhon);
+      write_pair_data<DIM>(P, A.out, i, vn, rhon, P.mass ? P.mass[A.id[i]] : P.m);
       if constexpr (FINAL) {
         dtmin = dbl_bits(particle_time_step<DIM>(P, rhon, vn, dv));
         if (A.dv_last[0]) {  // kept for the time step after post-integration

@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kBlk) void k_learned_update(DevParams P, LearnedSca
       vm = static_cast<float>(sqrt(fmax(s0, s1))) * (1.0f + 1e-6f);
       const double rho = st.rho[i] + T * (static_cast<double>(ui[3]) * sc.out_scale[3]);
       st.rho[i] = rho;
-      write_pair_data<DIM>(P, st, i, v1, rho);
+      write_pair_data<DIM>(P, st, i, v1, rho, P.m);  // uniform masses only
       cand = dbl_bits(particle_time_step<DIM>(P, rho, v1, a));
     }
   }
@@ -956,6 +956,7 @@ tit_status need_learned(tit_hip_ctx* c) {
   if (!c) return TIT_E_ARG;
   if (!c->learned) return fail(c, TIT_E_STATE, "learned step not initialised");
   if (!c->uploaded) return fail(c, TIT_E_STATE, "no particles uploaded");
+  if (c->mass) return fail(c, TIT_E_ARG, "the learned step needs uniform masses");
   Learned* L = c->learned;
   if (L->sized_n != c->n || L->sized_tanks != c->n_tanks) {
     TIT_HIP_TRY(c, hipSetDevice(c->device));
@@ -1171,6 +1172,7 @@ tit_status tit_hip_learned_init(tit_hip_ctx* c, uint64_t seed) {
   if (!c) return TIT_E_ARG;
   if (!c->uploaded) return fail(c, TIT_E_STATE, "learned init before upload");
   if (c->walls) return fail(c, TIT_E_ARG, "the learned step runs without walls");
+  if (c->mass) return fail(c, TIT_E_ARG, "the learned step needs uniform masses");
   TIT_HIP_TRY(c, hipSetDevice(c->device));
   destroy_learned(c);
   auto* L = new (std::nothrow) Learned();

@@ -82,6 +82,7 @@ class ParticleArray:
     box_hi: tuple = ()
     v: np.ndarray | None = None
     dv_dt: np.ndarray | None = None
+    m: np.ndarray | None = None   # (n,) per-particle masses; None: FluidEquations.m
 
     @property
     def n(self) -> int:
@@ -101,7 +102,8 @@ class ParticleArray:
         return ParticleArray(self.dim, self.r.copy(), self.rho.copy(), self.n_fluid,
                              self.normal.copy(), self.box_lo, self.box_hi,
                              None if self.v is None else self.v.copy(),
-                             None if self.dv_dt is None else self.dv_dt.copy())
+                             None if self.dv_dt is None else self.dv_dt.copy(),
+                             None if self.m is None else self.m.copy())
 
 
 def dam_break(dim: int, dr: float, tank, column, layers: int = 3,
@@ -277,9 +279,11 @@ class HipContext:
             v = None if pa.v is None else np.ascontiguousarray(pa.v, dtype=np.float64)
             dv = None if pa.dv_dt is None else np.ascontiguousarray(pa.dv_dt, dtype=np.float64)
             nrm = np.ascontiguousarray(pa.normal, dtype=np.float64)
-            keep += [r, rho, v, dv, nrm]
+            m = None if pa.m is None else np.ascontiguousarray(pa.m, dtype=np.float64)
+            assert m is None or m.shape == (pa.n,)
+            keep += [r, rho, v, dv, nrm, m]
             views[t].r, views[t].rho, views[t].v = _dp(r), _dp(rho), _dp(v)
-            views[t].dv_dt, views[t].normal = _dp(dv), _dp(nrm)
+            views[t].dv_dt, views[t].normal, views[t].m = _dp(dv), _dp(nrm), _dp(m)
             views[t].n_fluid, views[t].n_fixed = pa.n_fluid, pa.n_fixed
             for k in range(pa.dim):
                 tanks[t].lo[k] = pa.box_lo[k]
