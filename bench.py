@@ -98,8 +98,9 @@ def parse():
                     help="fluid particles of the held-out evaluation batch")
     ap.add_argument("--learned-steps", type=int, default=50,
                     help="timed learned steps (4 dt each; judge every 10)")
-    ap.add_argument("--train-iters", type=int, default=2400)
-    ap.add_argument("--train-tanks", type=int, default=8,
+    ap.add_argument("--train-iters", type=int, default=9600,
+                    help="Adam iterations (about 110 s on 4 tanks x 20 samples)")
+    ap.add_argument("--train-tanks", type=int, default=4,
                     help="seeded tanks (dr = 5 mm) whose trajectories train the network")
     ap.add_argument("--train-samples", type=int, default=20,
                     help="training samples, one every 300 plain steps from step 400")
@@ -211,6 +212,11 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, nf),
+                     "traffic_source": ("stored figure: rocprofv3 FETCH_SIZE and WRITE_SIZE "
+                                        "runs on this workload, scaled by a 256 MiB copy "
+                                        "(profiles/stage_kernel_2d_dam_break_2M_hbm_traffic.md); "
+                                        "counters are not collected inside this run"
+                                        if measured_traffic(args.workload, nf) else None),
                      "bytes_per_launch": bytes_launch,
                      "avg_launch_us": avg_stage_s * 1e6},
         "kernels": kernels,
@@ -224,9 +230,13 @@ def main():
         out["learned"] = learned_leg(args)
     if args.copy_probe_mb > 0:
         out["copy_probe"] = copy_probe(args.copy_probe_mb)
-    if not args.no_cpu and args.cpu_steps > 0 and len(arrays) == 1:
+    if not args.no_cpu and args.cpu_steps > 0:
+        # batched tanks: the CPU steps tanks one after another, so its rate
+        # is sampled on the first tank (a batched tank equals its own run)
         out["cpu_baseline"], out["parity"] = cpu_baseline(
             pa, eq, args.cpu_steps, args.post_integrate, walls, args.parity_steps)
+        if len(arrays) > 1:
+            out["cpu_baseline"]["sample"] += f" (tank 0 of the {len(arrays)}-tank batch)"
     print(json.dumps(out), flush=True)
 
 
