@@ -222,6 +222,10 @@ def main():
         "kernels": kernels,
     }
     out["search_roofline"] = search_roofline(ctx, arrays)
+    if dim == 2 and len(arrays) == 1 and walls is None:
+        out["roofline"].update(stencil_stats(ctx, pa, eq))
+        if out["roofline"].get("traffic"):
+            out["roofline"]["traffic_ratio"] = out["roofline"]["traffic"] / bytes_launch
     if args.moving_steps > 0 and args.workload in ("dam2d", "dam3d"):
         out["moving"] = moving_leg(args, eq, arrays)
     ctx.close()
@@ -311,6 +315,54 @@ def search_roofline(ctx, arrays, reps=5):
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "bytes_per_particle": per,
             "particles": n, "radix_passes": passes, "rebuild_us": t * 1e6,
             "per_kernel_us": {k: 1e3 * st[k][1] / reps for k in names if k in st}}
+
+
+def stencil_stats(ctx, pa, eq):
+    """Work per target of a 2D stage launch on the uploaded state: candidates
+    in the 5 x 5 cell stencil, candidates left after the per-target trimming
+    of the stencil (the stage kernels' rule restated on the host: a column
+    whose box, grown by skin/2, lies farther than 2h from the target is
+    skipped, and only rows within sqrt((2h)^2 - gap^2) + skin/2 of it are
+    scanned; border windows keep all five rows) and accepted neighbours
+    (the neighbour lists' sizes without self)."""
+    nf = pa.n_fluid
+    radius = 2.0 * eq.h
+    skin_half = 0.5 * 0.1 * radius
+    csz = (radius + 2.0 * skin_half) / 2.0 * (1.0 + 1e-9)
+    lo, hi = np.asarray(pa.box_lo[:2]), np.asarray(pa.box_hi[:2])
+    nc = np.maximum(1, np.ceil((hi - lo) / csz).astype(np.int64))
+    f = (pa.r - lo) / csz
+    c = np.clip(np.floor(f).astype(np.int64), 0, nc - 1)
+    counts = np.bincount(c[:, 0] * nc[1] + c[:, 1], minlength=nc[0] * nc[1]).reshape(nc)
+    cum = np.zeros((nc[0], nc[1] + 1), dtype=np.int64)
+    cum[:, 1:] = np.cumsum(counts, axis=1)
+    x, z = pa.r[:nf, 0], pa.r[:nf, 1]
+    cx, cz = c[:nf, 0], c[:nf, 1]
+    inner = (cx - 2 > 0) & (cx + 2 < nc[0] - 1) & (cz - 2 > 0) & (cz + 2 < nc[1] - 1)
+    zr = (z - lo[1]) / csz
+    full = np.zeros(nf)
+    trimmed = np.zeros(nf)
+    for q in range(-2, 3):
+        ix = cx + q
+        ok = (ix >= 0) & (ix < nc[0])
+        ixc = np.clip(ix, 0, nc[0] - 1)
+        r0, r1 = np.clip(cz - 2, 0, nc[1] - 1), np.clip(cz + 2, 0, nc[1] - 1)
+        full += np.where(ok, cum[ixc, r1 + 1] - cum[ixc, r0], 0)
+        a0 = lo[0] + ix * csz - skin_half
+        a1 = lo[0] + (ix + 1) * csz + skin_half
+        gap = np.maximum(0.0, np.maximum(a0 - x, x - a1))
+        live = ok & (gap * gap <= radius * radius)
+        half = np.sqrt(np.maximum(radius * radius - gap * gap, 0.0)) / csz + skin_half / csz
+        t0 = np.where(inner, np.maximum(np.floor(zr - half).astype(np.int64), cz - 2), cz - 2)
+        t1 = np.where(inner, np.minimum(np.floor(zr + half).astype(np.int64), cz + 2), cz + 2)
+        t0, t1 = np.clip(t0, 0, nc[1] - 1), np.clip(t1, 0, nc[1] - 1)
+        live = live | ~inner & ok
+        trimmed += np.where(live & (t1 >= t0), cum[ixc, t1 + 1] - cum[ixc, t0], 0)
+    off = ctx.neighbor_offsets().astype(np.int64)
+    pairs = np.diff(off)[:nf] - 1
+    return {"candidates_per_target_5x5": float(full.mean()),
+            "candidates_per_target_trimmed": float(trimmed.mean()),
+            "pairs_per_target": float(pairs.mean())}
 
 
 def terms_of(eq, dr):

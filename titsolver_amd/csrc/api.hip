@@ -278,6 +278,10 @@ void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode) {
                         c->cell_gaps, c->cell_n_gaps, rc.ctl);
   }
   {
+    ProfScope ps(c->prof, s, "fixed_active");
+    launch_fixed_active(s, boundary_args(c, c->A), rc.ctl);
+  }
+  {
     ProfScope ps(c->prof, s, "build_chunks");
     StageArgs a = stage_args(c);
     launch_build_chunks(s, a, c->block_tot, rc.ctl);
@@ -285,7 +289,7 @@ void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode) {
   launch_reset_disp(s, c->n_tanks, c->disp, rc);
 }
 
-void enqueue_boundary(tit_hip_ctx* c, const DevState& st, bool for_post_integrate) {
+BoundaryArgs boundary_args(tit_hip_ctx* c, const DevState& st) {
   BoundaryArgs a{};
   a.P = c->dp;
   a.tanks = c->tanks;
@@ -295,6 +299,8 @@ void enqueue_boundary(tit_hip_ctx* c, const DevState& st, bool for_post_integrat
   a.id = c->id;
   a.cell_off = c->cell_off;
   a.fixed_list = c->fixed_list;
+  a.active = c->fixed_active;
+  a.n_active = c->counter + 2;
   a.normal = c->normal;
   a.cell_fluid = c->cell_fluid;
   a.n_fixed = c->n_fixed;
@@ -303,6 +309,11 @@ void enqueue_boundary(tit_hip_ctx* c, const DevState& st, bool for_post_integrat
     a.wall_rho = c->wall_rho;
     a.wall_id0 = c->wall_id0;
   }
+  return a;
+}
+
+void enqueue_boundary(tit_hip_ctx* c, const DevState& st, bool for_post_integrate) {
+  const BoundaryArgs a = boundary_args(c, st);
   {
     ProfScope ps(c->prof, c->stream, "boundary");
     launch_boundary(c->stream, a);
@@ -546,8 +557,8 @@ void enqueue_post_integrate(tit_hip_ctx* c) {
     a.pi_face = c->wall_pi_face;
   }
   ProfScope ps(c->prof, c->stream, "post_integrate");
-  launch_post_integrate(c->stream, a, c->n_tanks, c->disp, c->dp.skin_half, c->counter + 1,
-                        c->n_cu);
+  launch_post_integrate(c->stream, a, c->n_tanks, c->disp, c->dp.skin_half,
+                        rebuild_ctl(c).ctl, c->n_cu);
 }
 
 tit_status check_launch(tit_hip_ctx* c, const char* what) {
@@ -948,6 +959,7 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->cell_gaps, static_cast<size_t>(n_total) + 1));
   TIT_HIP_TRY(c, dalloc(c, &c->cell_n_gaps, 1));
   TIT_HIP_TRY(c, dalloc(c, &c->fixed_list, std::max<uint32_t>(c->n_fixed, 1)));
+  TIT_HIP_TRY(c, dalloc(c, &c->fixed_active, std::max<uint32_t>(c->n_fixed, 1)));
   TIT_HIP_TRY(c, dalloc(c, &c->counter, 4));
   TIT_HIP_TRY(c, dalloc(c, &c->rebuild_ctl, 4));
   TIT_HIP_TRY(c, dalloc(c, &c->tank_need, n_tanks));

@@ -44,7 +44,8 @@ struct tit_hip_ctx {
   uint32_t *id = nullptr, *id_alt = nullptr, *tag = nullptr, *tag_alt = nullptr;
   uint32_t* cell_off = nullptr;
   uint32_t* fixed_list = nullptr;
-  uint32_t* counter = nullptr;  // [0] fixed count, [1] flags
+  uint32_t* fixed_active = nullptr;  // fixed particles with fluid in their stencil
+  uint32_t* counter = nullptr;  // [0] fixed count, [1] flags, [2] fixed_active count
   uint32_t* rebuild_ctl = nullptr;       // RebuildCtl::ctl
   uint32_t* tank_need = nullptr;         // RebuildCtl::need
   unsigned long long* rebuild_count = nullptr;
@@ -157,6 +158,8 @@ void enqueue_rebuild(tit_hip_ctx* c, RebuildMode mode);
 // setup_boundary (and with walls compute_gamma and the face terms; before
 // post-integration also apply_shifts' face terms).
 void enqueue_boundary(tit_hip_ctx* c, const DevState& st, bool for_post_integrate = false);
+// The boundary extrapolation's arguments on state st.
+BoundaryArgs boundary_args(tit_hip_ctx* c, const DevState& st);
 void enqueue_step(tit_hip_ctx* c);
 void enqueue_post_integrate(tit_hip_ctx* c);
 tit_status check_launch(tit_hip_ctx* c, const char* what);

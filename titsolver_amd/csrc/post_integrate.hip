@@ -795,9 +795,12 @@ __global__ __launch_bounds__(kPiBlock) void k_pi_fsc(PiArgs A) {
 }
 
 // Shift displacement into the index's bound; the next step size is taken
-// again after the shifts.  One wave per tank.
+// again after the shifts.  One wave per tank.  The passes above read the
+// neighbour sets of the unshifted positions, which the index still covers; a
+// shift that takes a tank past skin/2 marks a rebuild of every tank pending
+// (ctl[1]), which the next step or hook runs before it searches again.
 __global__ __launch_bounds__(64) void k_pi_bound(unsigned long long* shift_max, double* disp,
-                                                 double skin_half, uint32_t* flags,
+                                                 double skin_half, uint32_t* ctl,
                                                  unsigned long long* dt_min) {
   const uint32_t t = blockIdx.x;
   const unsigned long long sm = red_fold<true>(shift_max, t, true);
@@ -805,7 +808,7 @@ __global__ __launch_bounds__(64) void k_pi_bound(unsigned long long* shift_max, 
   if (threadIdx.x != 0) return;
   const double d = disp[t] + __longlong_as_double(static_cast<long long>(sm));
   disp[t] = d;
-  if (!(d <= skin_half)) atomicOr(flags, 1u);
+  if (!(d <= skin_half)) ctl[1] = 1u;
 }
 
 // Persistent: a lane walks a contiguous run of particles and keeps running
@@ -873,7 +876,7 @@ void launch_export_f64(hipStream_t s, uint32_t n, const uint32_t* id, const doub
 }
 
 void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* disp,
-                           double skin_half, uint32_t* flags, int n_cu) {
+                           double skin_half, uint32_t* ctl, int n_cu) {
   if (a.n == 0) return;
   const uint32_t nb = (a.n + kPiBlock - 1) / kPiBlock;
   const int dim = a.P.dim, kernel = a.P.kernel;
@@ -913,7 +916,7 @@ void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* 
       k_pi_fsc<3, kKernelC4><<<nb, kPiBlock, 0, s>>>(a);
     }
   }
-  k_pi_bound<<<n_tanks, 64, 0, s>>>(a.shift_max, disp, skin_half, flags, a.dt_min);
+  k_pi_bound<<<n_tanks, 64, 0, s>>>(a.shift_max, disp, skin_half, ctl, a.dt_min);
   if (dim == 2) {
     k_pi_commit<2><<<np_commit, kPiBlock, 0, s>>>(a);
   } else {

@@ -6,6 +6,8 @@
 // every stencil row along that axis is one contiguous particle range.  The
 // inclusive FP64 predicate decides membership, so the neighbour set is the
 // reference's exactly, independent of how the cells were built.
+#include <algorithm>
+
 #include "pair_math.hpp"
 #include "sph_kernels.hpp"
 
@@ -125,7 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(
     const uint32_t* __restrict__ need) {
   if (!ctl[0]) return;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i == 0) flags[-1] = 0u;  // fixed-particle counter, filled by k_permute
+  if (i == 0) {
+    flags[-1] = 0u;  // fixed-particle counter, filled by k_permute
+    flags[1] = 0u;   // listed fixed particles with fluid near, filled by k_fixed_active
+  }
   if (i >= n) return;
   const uint32_t t = tag[i] >> 1;
   if (!need[t]) return;  // the tank keeps its keys, hence its order
@@ -287,16 +292,135 @@ __global__ __launch_bounds__(kBlock) void k_cell_gaps(uint32_t* __restrict__ off
 // two sums are reduced with shuffles inside the group.
 
 constexpr int kBoundaryLanes = 16;
+// fixed particles per workgroup of k_fixed_active (its LDS list)
+constexpr uint32_t kFixedPerBlock = 1024;
 
+// Cell coordinates (per axis) of particle i's cell in its tank.
+template <int DIM>
+__device__ __forceinline__ void cell_coords(const BoundaryArgs& A, const DevTank& T, uint32_t i,
+                                            int* c) {
+  const uint32_t local = A.key[i] - T.cell_off;
+  if constexpr (DIM == 2) {
+    c[0] = static_cast<int>(local / T.nc[1]);
+    c[1] = static_cast<int>(local % T.nc[1]);
+  } else {
+    const uint32_t q = local / T.nc[2];
+    c[2] = static_cast<int>(local % T.nc[2]);
+    c[1] = static_cast<int>(q % T.nc[1]);
+    c[0] = static_cast<int>(q / T.nc[1]);
+  }
+}
+
+// Whether a cell of the 5^d stencil around cell c held fluid at the last
+// rebuild; the G lanes of a group split the cells.
+template <int DIM, int G>
+__device__ __forceinline__ bool stencil_has_fluid(const BoundaryArgs& A, const DevTank& T,
+                                                  const int* c, int lane) {
+  constexpr int NCOL = DIM == 2 ? 2 * kStencil + 1 : (2 * kStencil + 1) * (2 * kStencil + 1);
+  constexpr int L = DIM - 1;
+  const int zlo = max(c[L] - kStencil, 0), zhi = min(c[L] + kStencil, T.nc[L] - 1);
+  const int nrow = zhi - zlo + 1;
+  int any = 0;
+  for (int t = lane; t < NCOL * nrow; t += G) {
+    const int q = t / nrow, z = zlo + t % nrow;
+    int ix, iy = 0;
+    bool ok;
+    if constexpr (DIM == 2) {
+      ix = c[0] - kStencil + q;
+      ok = ix >= 0 && ix < T.nc[0];
+    } else {
+      ix = c[0] - kStencil + q / (2 * kStencil + 1);
+      iy = c[1] - kStencil + q % (2 * kStencil + 1);
+      ok = ix >= 0 && ix < T.nc[0] && iy >= 0 && iy < T.nc[1];
+    }
+    if (ok) {
+      const uint32_t row = DIM == 2
+                               ? T.cell_off + static_cast<uint32_t>(ix) * T.nc[1]
+                               : T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] +
+                                               static_cast<uint32_t>(iy)) * T.nc[2];
+      any |= A.cell_fluid[row + static_cast<uint32_t>(z)];
+    }
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) any |= __shfl_xor(any, off, G);
+  return any != 0;
+}
+
+// The fixed particle's extrapolated state from the Shepard sums S and H.
+template <int DIM>
+__device__ __forceinline__ void write_fixed(const BoundaryArgs& A, uint32_t i, double S,
+                                            double H) {
+  const DevParams& P = A.P;
+  const double tiny = 6.0554544523933395e-06;  // cbrt(DBL_EPSILON)
+  const double Hs = fabs(S) <= tiny ? 0.0 : H / S;
+  const double rho_e = eos_density_from_potential(P, Hs);
+  A.s.rho[i] = rho_e;
+  const double zero[DIM] = {};
+  const double m_e = particle_mass(P, A.id[i]);
+  write_pair_data<DIM>(P, A.s, i, zero, rho_e, m_e);
+  if (A.wall_rho) {
+    const uint32_t k = A.id[i] - A.wall_id0;
+    A.wall_rho[k] = rho_e;
+    A.s.pv[i].z = static_cast<float>(A.wall_mu[k] * m_e / rho_e);  // V_e = m_e / rho_e
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) A.s.v[d][i] = 0.0;
+}
+
+// After a cell rebuild: lists the fixed particles with fluid in their
+// stencil (k_boundary extrapolates those, every stage), and writes the
+// others' state once: with no fluid neighbour S = H = 0, the value a full
+// scan gives, and it holds until the next rebuild because the stencil covers
+// every neighbour until then.  The list's order is irrelevant (each fixed
+// particle is extrapolated on its own).  A workgroup compacts its share in
+// LDS and takes its slots with one atomic.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_fixed_active(BoundaryArgs A, const uint32_t* ctl,
+                                                         uint32_t per_block) {
+  if (!ctl[0]) return;
+  constexpr int G = kBoundaryLanes;
+  constexpr int kGroups = kBlock / G;
+  __shared__ uint32_t s_list[kFixedPerBlock];
+  __shared__ uint32_t s_n, s_base;
+  const int lane = threadIdx.x % G;
+  const uint32_t grp = threadIdx.x / G;
+  const uint32_t beg = blockIdx.x * per_block;
+  const uint32_t end = min(beg + per_block, A.n_fixed);
+  if (threadIdx.x == 0) s_n = 0u;
+  __syncthreads();
+  for (uint32_t base = beg; base < end; base += kGroups) {
+    const uint32_t k = base + grp;
+    if (k < end) {  // uniform in the group
+      const uint32_t i = A.fixed_list[k];
+      const DevTank T = A.tanks[A.tag[i] >> 1];
+      int c[3];
+      cell_coords<DIM>(A, T, i, c);
+      const bool any = stencil_has_fluid<DIM, G>(A, T, c, lane);
+      if (lane == 0) {
+        if (any) {
+          s_list[atomicAdd(&s_n, 1u)] = i;
+        } else {
+          write_fixed<DIM>(A, i, 0.0, 0.0);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) s_base = s_n ? atomicAdd(A.n_active, s_n) : 0u;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < s_n; t += kBlock) A.active[s_base + t] = s_list[t];
+}
+
+// Extrapolation of the listed fixed particles, one group of G lanes each.
 template <int DIM, int KERNEL>
 __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   constexpr int NCOL = DIM == 2 ? 2 * kStencil + 1 : (2 * kStencil + 1) * (2 * kStencil + 1);
   constexpr int G = kBoundaryLanes;
   const uint32_t k = blockIdx.x * (kBlock / G) + (threadIdx.x / G);
   const int lane = threadIdx.x % G;
-  if (k >= A.n_fixed) return;  // uniform in the group
+  if (k >= *A.n_active) return;  // uniform in the group
   const DevParams& P = A.P;
-  const uint32_t i = A.fixed_list[k];
+  const uint32_t i = A.active[k];
   const DevTank T = A.tanks[A.tag[i] >> 1];
   double xe[DIM], ne[DIM];
   const double* nrm = A.normal + static_cast<size_t>(A.id[i]) * DIM;
@@ -326,17 +450,8 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   for (int d = 0; d < DIM; ++d) w0 *= h_inv;
 
   // Stencil column ranges (wave-uniform).
-  const uint32_t local = A.key[i] - T.cell_off;
   int c[3];
-  if constexpr (DIM == 2) {
-    c[0] = static_cast<int>(local / T.nc[1]);
-    c[1] = static_cast<int>(local % T.nc[1]);
-  } else {
-    const uint32_t q = local / T.nc[2];
-    c[2] = static_cast<int>(local % T.nc[2]);
-    c[1] = static_cast<int>(q % T.nc[1]);
-    c[0] = static_cast<int>(q / T.nc[1]);
-  }
+  cell_coords<DIM>(A, T, i, c);
   constexpr int L = DIM - 1;
   const int zlo = max(c[L] - kStencil, 0), zhi = min(c[L] + kStencil, T.nc[L] - 1);
   // 2D: the five column ranges and their prefix
@@ -355,34 +470,6 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
       rb[q] = b;
       rl[q + 1] = rl[q] + (e - b);
     }
-  }
-  // No stencil cell holds fluid: no fluid neighbour, S = H = 0 as a full scan
-  // would leave them.
-  int any = 0;
-  {
-    const int nrow = zhi - zlo + 1;
-    for (int t = lane; t < NCOL * nrow; t += G) {
-      const int q = t / nrow, z = zlo + t % nrow;
-      int ix, iy = 0;
-      bool ok;
-      if constexpr (DIM == 2) {
-        ix = c[0] - kStencil + q;
-        ok = ix >= 0 && ix < T.nc[0];
-      } else {
-        ix = c[0] - kStencil + q / (2 * kStencil + 1);
-        iy = c[1] - kStencil + q % (2 * kStencil + 1);
-        ok = ix >= 0 && ix < T.nc[0] && iy >= 0 && iy < T.nc[1];
-      }
-      if (ok) {
-        const uint32_t row = DIM == 2
-                                 ? T.cell_off + static_cast<uint32_t>(ix) * T.nc[1]
-                                 : T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] +
-                                                 static_cast<uint32_t>(iy)) * T.nc[2];
-        any |= A.cell_fluid[row + static_cast<uint32_t>(z)];
-      }
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) any |= __shfl_xor(any, off, G);
   }
   double S = 0.0, H = 0.0;
   const double xi_1 = P.xi - 1.0, inv_xi_1 = 1.0 / xi_1, cs2 = P.cs0 * P.cs0;
@@ -410,13 +497,13 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   };
   if constexpr (DIM == 2) {
     // the lanes take every G-th candidate of the five column ranges in turn
-    const uint32_t n_cand = any ? rl[NCOL] : 0u;
+    const uint32_t n_cand = rl[NCOL];
     for (uint32_t t = lane; t < n_cand; t += G) {
       int q = 0;
       while (t >= rl[q + 1]) ++q;
       visit(rb[q] + (t - rl[q]));
     }
-  } else if (any) {
+  } else {
     // 25 short column ranges: one range at a time, its bounds loaded by the
     // whole group (no per-candidate search over the ranges).  Trimming the
     // ranges as the stage kernels do cost more per group than it saved
@@ -437,21 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     S += __shfl_xor(S, off);
     H += __shfl_xor(H, off);
   }
-  if (lane == 0) {
-    const double Hs = fabs(S) <= tiny ? 0.0 : H / S;
-    const double rho_e = eos_density_from_potential(P, Hs);
-    A.s.rho[i] = rho_e;
-    const double zero[DIM] = {};
-    const double m_e = particle_mass(P, A.id[i]);
-    write_pair_data<DIM>(P, A.s, i, zero, rho_e, m_e);
-    if (A.wall_rho) {
-      const uint32_t k = A.id[i] - A.wall_id0;
-      A.wall_rho[k] = rho_e;
-      A.s.pv[i].z = static_cast<float>(A.wall_mu[k] * m_e / rho_e);  // V_e = m_e / rho_e
-    }
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) A.s.v[d][i] = 0.0;
-  }
+  if (lane == 0) write_fixed<DIM>(A, i, S, H);
 }
 
 // Time step of the current state with a given dv/dt (reference order) or
@@ -689,6 +762,20 @@ void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
   k_cell_offsets<<<grid_for(n + 1), kBlock, 0, s>>>(n_cells, n, key, tag, cell_off, fluid,
                                                      gaps, n_gaps, ctl);
   k_cell_gaps<<<1024, kBlock, 0, s>>>(cell_off, fluid, gaps, n_gaps, ctl);
+}
+
+void launch_fixed_active(hipStream_t s, const BoundaryArgs& a, const uint32_t* ctl) {
+  if (a.n_fixed == 0) return;
+  // at most kFixedPerBlock per workgroup; at least 16 (one round) when there
+  // are few, so that the list is built by many workgroups
+  const uint32_t grid = std::max((a.n_fixed + kFixedPerBlock - 1) / kFixedPerBlock,
+                                 std::min<uint32_t>(1024u, (a.n_fixed + 15) / 16));
+  const uint32_t per_block = (a.n_fixed + grid - 1) / grid;
+  if (a.P.dim == 2) {
+    k_fixed_active<2><<<grid, kBlock, 0, s>>>(a, ctl, per_block);
+  } else {
+    k_fixed_active<3><<<grid, kBlock, 0, s>>>(a, ctl, per_block);
+  }
 }
 
 void launch_boundary(hipStream_t s, const BoundaryArgs& a) {

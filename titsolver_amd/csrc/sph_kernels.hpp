@@ -136,6 +136,11 @@ struct BoundaryArgs {
   const uint32_t* id;
   const uint32_t* cell_off;
   const uint32_t* fixed_list;
+  // the fixed particles with fluid in their stencil at the last rebuild
+  // (k_fixed_active lists them; the others keep the state it wrote) and
+  // their count
+  uint32_t* active;
+  uint32_t* n_active;
   const double* normal;  // reference order, dim per particle (fixed only)
   const uint8_t* cell_fluid;  // per cell: holds a fluid particle (last rebuild)
   uint32_t n_fixed;
@@ -185,6 +190,10 @@ void launch_restore_order(hipStream_t s, int dim, uint32_t n, const uint32_t* pe
 void launch_cell_offsets(hipStream_t s, uint32_t n_cells, uint32_t n,
                          const uint32_t* key, const uint32_t* tag, uint32_t* cell_off,
                          uint8_t* fluid, uint4* gaps, uint32_t* n_gaps, const uint32_t* ctl);
+// After a rebuild (ctl[0]): the fixed particles with fluid in their stencil
+// into a.active / a.n_active, the others' state written into a.s.
+void launch_fixed_active(hipStream_t s, const BoundaryArgs& a, const uint32_t* ctl);
+// Extrapolation of the listed fixed particles into a.s.
 void launch_boundary(hipStream_t s, const BoundaryArgs& a);
 
 // Semi-analytic wall (wall.hip): per fluid particle of the stage input state,
@@ -215,10 +224,11 @@ struct WallArgs {
   double* pi_face;                 // post-integration: 12 per near-list index
 };
 void launch_wall(hipStream_t s, const WallArgs& a, int n_cu);
-// post_integrate.hip: the passes in order, the shift into disp[] (flags bit 0
-// when it passes skin_half), the next step size into dt_min.
+// post_integrate.hip: the passes in order, the shift into disp[] (a rebuild
+// of every tank marked pending in ctl[1] when it passes skin_half), the next
+// step size into dt_min.
 void launch_post_integrate(hipStream_t s, const PiArgs& a, int n_tanks, double* disp,
-                           double skin_half, uint32_t* flags, int n_cu);
+                           double skin_half, uint32_t* ctl, int n_cu);
 // dst[id[i]] = src[i] (FP64).
 void launch_export_f64(hipStream_t s, uint32_t n, const uint32_t* id, const double* src,
                        double* dst);

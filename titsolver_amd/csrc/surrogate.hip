@@ -566,8 +566,9 @@ struct Sample {
 };
 
 // Everything a plain step reads besides the parameters: the state in cell
-// order with its cell index (keys, offsets, chunk list, fixed list), the
-// clocks and the rebuild bookkeeping.  Restoring it makes the following
+// order with its cell index (keys, offsets, the cells holding fluid, chunk
+// lists, the fixed particles and those with fluid near), the clocks and the
+// rebuild bookkeeping.  Restoring it makes the following
 // plain steps repeat bit for bit what they would have done.
 struct Snapshot {
   DevState s{};
@@ -578,6 +579,10 @@ struct Snapshot {
   uint4* chunks = nullptr;
   uint32_t* chunk_counts = nullptr;
   uint32_t* fixed_list = nullptr;
+  uint32_t* fixed_active = nullptr;
+  uint8_t* cell_fluid = nullptr;
+  uint4* bchunks = nullptr;  // 3D block chunks and their per-block counts
+  uint32_t* bcount = nullptr;
   uint32_t* counter = nullptr;
   uint32_t* ctl = nullptr;
   uint32_t* need = nullptr;
@@ -586,7 +591,8 @@ struct Snapshot {
   double* time = nullptr;
   double* dt = nullptr;
   unsigned long long* dt_min = nullptr;  // pending step size of each tank
-  size_t chunk_cap = 0;
+  size_t chunk_cap = 0, bchunk_cap = 0;
+  uint32_t n_blocks = 0;
   bool alloc = false;
 };
 
@@ -1075,7 +1081,10 @@ tit_status learned_step(tit_hip_ctx* c) {
 }
 
 hipError_t alloc_snapshot(tit_hip_ctx* c, Snapshot& S) {
-  if (S.alloc && S.chunk_cap == c->chunk_cap) return hipSuccess;
+  if (S.alloc && S.chunk_cap == c->chunk_cap && S.bchunk_cap == c->bchunk_cap &&
+      S.n_blocks == c->n_blocks) {
+    return hipSuccess;
+  }
   Learned* L = c->learned;
   auto& m = L->mem_n;
   if (!S.alloc) {
@@ -1093,6 +1102,8 @@ hipError_t alloc_snapshot(tit_hip_ctx* c, Snapshot& S) {
     TIT_TRY_E(lalloc(m, &S.cell_off, static_cast<size_t>(c->n_cells) + 1));
     TIT_TRY_E(lalloc(m, &S.chunk_counts, 4));
     TIT_TRY_E(lalloc(m, &S.fixed_list, std::max<uint32_t>(c->n_fixed, 1)));
+    TIT_TRY_E(lalloc(m, &S.fixed_active, std::max<uint32_t>(c->n_fixed, 1)));
+    TIT_TRY_E(lalloc(m, &S.cell_fluid, std::max<uint32_t>(c->n_cells, 1)));
     TIT_TRY_E(lalloc(m, &S.counter, 4));
     TIT_TRY_E(lalloc(m, &S.ctl, 4));
     TIT_TRY_E(lalloc(m, &S.need, c->n_tanks));
@@ -1102,9 +1113,15 @@ hipError_t alloc_snapshot(tit_hip_ctx* c, Snapshot& S) {
     TIT_TRY_E(lalloc(m, &S.dt, c->n_tanks));
     TIT_TRY_E(lalloc(m, &S.dt_min, static_cast<size_t>(c->n_tanks) * kRedStripes));
   }
-  // the chunk buffer follows the context's schedule
+  // the chunk buffers follow the context's schedule
   TIT_TRY_E(lalloc(m, &S.chunks, c->chunk_cap));
   S.chunk_cap = c->chunk_cap;
+  if (c->bchunk_cap) {
+    TIT_TRY_E(lalloc(m, &S.bchunks, c->bchunk_cap));
+    TIT_TRY_E(lalloc(m, &S.bcount, static_cast<size_t>(c->n_blocks) + 1));
+  }
+  S.bchunk_cap = c->bchunk_cap;
+  S.n_blocks = c->n_blocks;
   S.alloc = true;
   return hipSuccess;
 }
@@ -1135,6 +1152,12 @@ hipError_t copy_state(tit_hip_ctx* c, Snapshot& S, bool to_snapshot) {
   TIT_TRY_E(cp(c->chunks, S.chunks, c->chunk_cap));
   TIT_TRY_E(cp(c->chunk_counts, S.chunk_counts, 4));
   TIT_TRY_E(cp(c->fixed_list, S.fixed_list, std::max<uint32_t>(c->n_fixed, 1)));
+  TIT_TRY_E(cp(c->fixed_active, S.fixed_active, std::max<uint32_t>(c->n_fixed, 1)));
+  TIT_TRY_E(cp(c->cell_fluid, S.cell_fluid, std::max<uint32_t>(c->n_cells, 1)));
+  if (c->bchunk_cap) {
+    TIT_TRY_E(cp(c->bchunks, S.bchunks, c->bchunk_cap));
+    TIT_TRY_E(cp(c->bcount, S.bcount, static_cast<size_t>(c->n_blocks) + 1));
+  }
   TIT_TRY_E(cp(c->counter, S.counter, 4));
   TIT_TRY_E(cp(c->rebuild_ctl, S.ctl, 4));
   TIT_TRY_E(cp(c->tank_need, S.need, nt));
