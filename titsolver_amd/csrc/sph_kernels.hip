@@ -454,6 +454,32 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   cell_coords<DIM>(A, T, i, c);
   constexpr int L = DIM - 1;
   const int zlo = max(c[L] - kStencil, 0), zhi = min(c[L] + kStencil, T.nc[L] - 1);
+  // Stencil trimming (interior windows only): a fluid particle lies within
+  // skin/2 of its cell's box, so one in a column whose box (grown by skin/2)
+  // is g away in the column plane has |dz| <= sqrt((2h)^2 - g^2); the rows
+  // beyond cannot hold a neighbour (FP64, bounds widened by 1e-9).
+  bool inner = c[0] - kStencil > 0 && c[0] + kStencil < T.nc[0] - 1 && c[L] - kStencil > 0 &&
+               c[L] + kStencil < T.nc[L] - 1;
+  if constexpr (DIM == 3) inner = inner && c[1] - kStencil > 0 && c[1] + kStencil < T.nc[1] - 1;
+  auto gap = [&](int ax, int ic) {
+    const double cw = 1.0 / T.inv_c[ax];
+    const double a0 = T.lo[ax] + ic * cw - P.skin_half;
+    const double a1 = T.lo[ax] + (ic + 1) * cw + P.skin_half;
+    return fmax(0.0, fmax(a0 - xe[ax], xe[ax] - a1)) * (1.0 - 1e-9);
+  };
+  // rows [w0, w1] of a column at squared in-plane gap g2; false: none
+  auto rows = [&](double g2, int& w0, int& w1) {
+    w0 = zlo;
+    w1 = zhi;
+    if (!inner) return true;
+    if (g2 > P.R2 * (1.0 + 1e-9)) return false;
+    const double hz = (sqrt(fmax(P.R2 * (1.0 + 1e-9) - g2, 0.0)) + P.skin_half) *
+                          T.inv_c[L] * (1.0 + 1e-9) + 1e-9;
+    const double zr = (xe[L] - T.lo[L]) * T.inv_c[L];
+    w0 = max(w0, static_cast<int>(floor(zr - hz)));
+    w1 = min(w1, static_cast<int>(floor(zr + hz)));
+    return w0 <= w1;
+  };
   // 2D: the five column ranges and their prefix
   uint32_t rb[DIM == 2 ? NCOL : 1], rl[DIM == 2 ? NCOL + 1 : 1];
   rl[0] = 0;
@@ -462,10 +488,14 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     for (int q = 0; q < NCOL; ++q) {
       const int ix = c[0] - kStencil + q;
       uint32_t b = 0, e = 0;
+      int w0, w1;
       if (ix >= 0 && ix < T.nc[0]) {
-        const uint32_t row = T.cell_off + static_cast<uint32_t>(ix) * T.nc[1];
-        b = A.cell_off[row + zlo];
-        e = A.cell_off[row + zhi + 1];
+        const double gx = gap(0, ix);
+        if (rows(gx * gx, w0, w1)) {
+          const uint32_t row = T.cell_off + static_cast<uint32_t>(ix) * T.nc[1];
+          b = A.cell_off[row + w0];
+          e = A.cell_off[row + w1 + 1];
+        }
       }
       rb[q] = b;
       rl[q + 1] = rl[q] + (e - b);
@@ -504,19 +534,57 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
       visit(rb[q] + (t - rl[q]));
     }
   } else {
-    // 25 short column ranges: one range at a time, its bounds loaded by the
-    // whole group (no per-candidate search over the ranges).  Trimming the
-    // ranges as the stage kernels do cost more per group than it saved
-    // (812 -> 1,063 us per launch on the 2M-particle 3D dam break).
-    for (int q = 0; q < NCOL; ++q) {
+    // The 25 trimmed column ranges, laid end to end: each lane loads two
+    // columns' bounds, the group scans their lengths, and the lanes then
+    // take every G-th candidate of the whole list (no round trip per
+    // column).
+    __shared__ uint32_t s_rb[kBlock / G][NCOL];
+    __shared__ uint32_t s_rl[kBlock / G][NCOL + 1];
+    const int grp = threadIdx.x / G;
+    uint32_t beg[2] = {0u, 0u}, len[2] = {0u, 0u};
+#pragma unroll
+    for (int hq = 0; hq < 2; ++hq) {
+      const int q = lane + hq * G;
+      if (q >= NCOL) continue;
       const int ix = c[0] - kStencil + q / (2 * kStencil + 1);
       const int iy = c[1] - kStencil + q % (2 * kStencil + 1);
       if (ix < 0 || ix >= T.nc[0] || iy < 0 || iy >= T.nc[1]) continue;
+      const double gx = gap(0, ix), gy = gap(1, iy);
+      int w0, w1;
+      if (!rows(gx * gx + gy * gy, w0, w1)) continue;
       const uint32_t row =
           T.cell_off + (static_cast<uint32_t>(ix) * T.nc[1] + static_cast<uint32_t>(iy)) *
                            T.nc[2];
-      const uint32_t e = A.cell_off[row + zhi + 1];
-      for (uint32_t j = A.cell_off[row + zlo] + lane; j < e; j += G) visit(j);
+      beg[hq] = A.cell_off[row + w0];
+      len[hq] = A.cell_off[row + w1 + 1] - beg[hq];
+    }
+    // columns 0..15 on the lanes' first slot, 16..24 on the second
+    uint32_t x0 = len[0], x1 = len[1];
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      const uint32_t y0 = __shfl_up(x0, off, G), y1 = __shfl_up(x1, off, G);
+      if (lane >= off) {
+        x0 += y0;
+        x1 += y1;
+      }
+    }
+    const uint32_t tot0 = __shfl(x0, G - 1, G), tot1 = __shfl(x1, G - 1, G);
+    s_rb[grp][lane] = beg[0];
+    s_rl[grp][lane] = x0 - len[0];
+    if (lane + G < NCOL) {
+      s_rb[grp][lane + G] = beg[1];
+      s_rl[grp][lane + G] = tot0 + x1 - len[1];
+    }
+    if (lane == 0) s_rl[grp][NCOL] = tot0 + tot1;
+    // the group's lanes read what the others wrote: wavefront-scope ordering
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t n_cand = tot0 + tot1;
+    int q = 0;
+    for (uint32_t t = lane; t < n_cand; t += G) {
+      while (t >= s_rl[grp][q + 1]) ++q;
+      visit(s_rb[grp][q] + (t - s_rl[grp][q]));
     }
   }
 #pragma unroll
