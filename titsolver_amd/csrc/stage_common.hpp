@@ -44,11 +44,6 @@ struct LdsPos {
     e = coff[k][z1 + 1 - zlo];
     return true;
   }
-  __device__ __forceinline__ void pos(int k, uint32_t j, double* x) const {
-    const double2 p = xy[lb[k] + (j - gb[k])];
-    x[0] = p.x;
-    x[1] = p.y;
-  }
 };
 
 // Positions from global memory (any dimension).
@@ -87,10 +82,6 @@ struct GlobalPos {
     e = coff[row + z1 + 1];
     return true;
   }
-  __device__ __forceinline__ void pos(int, uint32_t j, double* x) const {
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) x[k] = S.x[k][j];
-  }
 };
 
 template <int DIM>
@@ -99,14 +90,6 @@ struct PairIn {
   float4 pv;
   float2 aux;
 };
-
-template <int DIM, class Pos>
-__device__ __forceinline__ void load_pair(const Pos& ps, const DevState& S, int k,
-                                          uint32_t j, PairIn<DIM>& p) {
-  ps.pos(k, j, p.x);
-  p.pv = S.pv[j];
-  p.aux = S.aux[j];
-}
 
 template <int DIM, int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void term_of(const DevParams& P, const Target<DIM>& a,
@@ -144,78 +127,15 @@ __device__ __forceinline__ uint32_t pop_lowest(uint64_t& mask) {
   return t;
 }
 
-template <int DIM, int KERNEL, int FLAVOUR, int NCOL, class Pos>
-__device__ __forceinline__ void gather(const DevParams& P, const Pos& ps,
-                                       const DevState& S, const Target<DIM>& a,
-                                       uint32_t i, int zc, const Windows<DIM, NCOL>& W,
-                                       Sums<DIM>& acc, uint32_t& count) {
-  for (int k = 0; k < NCOL; ++k) {
-    uint32_t b, e;
-    const uint32_t v = W.get(k);
-    if ((v & 7u) == 7u) continue;
-    const int w0 = zc - kStencil + static_cast<int>(v & 7u);
-    const int w1 = zc - kStencil + static_cast<int>(v >> 3);
-    if (!ps.range(k, w0, w1, b, e)) continue;
-    for (uint32_t c0 = b; c0 < e; c0 += 64) {
-      const uint32_t len = min(e - c0, 64u);
-      uint64_t mask = 0;
-      uint32_t t = 0;
-      for (; t + 4 <= len; t += 4) {
-        double x0[DIM], x1[DIM], x2[DIM], x3[DIM];
-        ps.pos(k, c0 + t, x0);
-        ps.pos(k, c0 + t + 1, x1);
-        ps.pos(k, c0 + t + 2, x2);
-        ps.pos(k, c0 + t + 3, x3);
-        const uint64_t m = (within<DIM>(P, x0, a.x) ? 1ull : 0ull) |
-                           (within<DIM>(P, x1, a.x) ? 2ull : 0ull) |
-                           (within<DIM>(P, x2, a.x) ? 4ull : 0ull) |
-                           (within<DIM>(P, x3, a.x) ? 8ull : 0ull);
-        mask |= m << t;
-      }
-      for (; t < len; ++t) {
-        double x0[DIM];
-        ps.pos(k, c0 + t, x0);
-        if (within<DIM>(P, x0, a.x)) mask |= 1ull << t;
-      }
-      if (i >= c0 && i < c0 + len) mask &= ~(1ull << (i - c0));  // not self
-      count += static_cast<uint32_t>(__builtin_popcountll(mask));
-      if (!mask) continue;
-      PairIn<DIM> A, B;
-      if (acc.pend) {  // the odd term of an earlier batch waits for its partner
-        load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), A);
-        sum_pair<DIM, KERNEL, FLAVOUR>(P, a, A, acc);
-        if (!mask) continue;
-      }
-      // Two register sets alternate: the next accepted candidate's loads are
-      // in flight while the current one is summed; two terms enter the
-      // compensated sums together.
-      load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), A);
-      while (true) {
-        const bool hb = mask != 0;
-        if (hb) load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), B);
-        float ca, fa[DIM];
-        term_of<DIM, KERNEL, FLAVOUR>(P, a, A, ca, fa);
-        if (!hb) {
-          acc.add(ca, fa);
-          break;
-        }
-        const bool ha = mask != 0;
-        if (ha) load_pair<DIM>(ps, S, k, c0 + pop_lowest(mask), A);
-        float cb, fb[DIM];
-        term_of<DIM, KERNEL, FLAVOUR>(P, a, B, cb, fb);
-        acc.add2(ca, fa, cb, fb);
-        if (!ha) break;
-      }
-    }
-  }
-}
-
+// Candidate order, the same on every path so that all schedules produce the
+// same bits: stencil columns in order, within a column ascending index; the
+// accepted candidates' terms enter the compensated sums two at a time.
+//
 // 2D with LDS positions.  The acceptance masks of up to kWords candidate
 // batches (normally one per stencil column) are kept per lane in LDS and
 // their accepted candidates are summed in one loop, so a wave's pair loop
 // runs for the largest accepted count among its lanes, not for the sum over
-// columns of each column's largest count.  Candidates are visited in the
-// same order as in gather(), so the sums are identical.
+// columns of each column's largest count.
 template <int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void sum_words(const DevParams& P, const LdsPos& ps,
                                           const DevState& S, const Target<2>& a,
@@ -250,8 +170,9 @@ __device__ __forceinline__ void sum_words(const DevParams& P, const LdsPos& ps,
     sum_pair<2, KERNEL, FLAVOUR>(P, a, A, acc);
     if (--rem == 0) return;
   }
-  // two register sets alternate, as in gather(); their terms enter the
-  // compensated sums together
+  // two register sets alternate (the next candidate's loads in flight while
+  // the current one is summed); their terms enter the compensated sums
+  // together
   take(A);
   --rem;
   while (true) {
@@ -348,7 +269,7 @@ __device__ __forceinline__ void gather_words(const DevParams& P, const LdsPos& p
 
 // Global-memory path, the same scheme with 32-candidate masks (the 3D stencil
 // has 25 short column ranges): all columns' accepted candidates are summed
-// in one loop per lane, in the same order as gather().
+// in one loop per lane, in the common candidate order.
 template <int DIM, int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void sum_words_g(const DevParams& P, const GlobalPos<DIM>& ps,
                                             const DevState& S, const Target<DIM>& a,
