@@ -385,7 +385,8 @@ tit_status build_schedule(tit_hip_ctx* c) {
     c->cols = nullptr;
     c->block_tot = nullptr;
     TIT_HIP_TRY(c, dalloc(c, &c->cols, cols.size()));
-    TIT_HIP_TRY(c, dalloc(c, &c->block_tot, cols.size() / 1024 + 2));
+    // per workgroup of the chunk builder (16 columns each)
+    TIT_HIP_TRY(c, dalloc(c, &c->block_tot, cols.size() / 16 + 2));
     TIT_HIP_TRY(c, hipMemcpy(c->cols, cols.data(), cols.size() * sizeof(uint4),
                              hipMemcpyHostToDevice));
     c->n_cols = static_cast<uint32_t>(cols.size());

@@ -370,48 +370,73 @@ __global__ __launch_bounds__(64) void k_block_chunks(StageArgs A, int emit, cons
 
 constexpr int kScanBlock = 1024;
 
-// The chunks of column col_i, in order: the column's particles in
+// The chunks of a column, in order: the column's particles in
 // ceil(n / chunk_targets) equal parts, each cut further into pieces whose
 // targets span at most kChunkRows cells (a piece starting at particle p ends
-// before the first particle with key >= key[p] + kChunkRows).  Calls
-// emit(k, p0, p1) for the k-th chunk and returns the count.  A column gains
+// before the first particle with key >= key[p] + kChunkRows).  A column gains
 // at most (nz - 1) / kChunkRows cuts: each cut starts a piece kChunkRows
 // cells or more above the last, and the parts' spans do not overlap.
-template <class F>
-__device__ __forceinline__ uint32_t column_chunks(const StageArgs& A, uint32_t col_i, F&& emit) {
+struct ColumnParts {
+  uint32_t beg, n, nch;
+};
+
+__device__ __forceinline__ ColumnParts column_parts(const StageArgs& A, uint32_t col_i) {
   const uint4 col = A.cols[col_i];
   const DevTank& T = A.tanks[col.x];
   const int nz = T.nc[A.P.dim - 1];
-  const uint32_t beg = A.cell_off[col.w];
-  const uint32_t n = A.cell_off[col.w + nz] - beg;
-  const uint32_t nch = (n + A.chunk_targets - 1) / A.chunk_targets;
-  uint32_t count = 0;
-  for (uint32_t k = 0; k < nch; ++k) {
-    const uint32_t p0 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * k) / nch);
-    const uint32_t p1 = beg + static_cast<uint32_t>((static_cast<uint64_t>(n) * (k + 1)) / nch);
-    uint32_t p = p0;
-    while (p < p1) {
-      const uint32_t lim = A.key[p] + static_cast<uint32_t>(kChunkRows);
-      uint32_t q = p1;
-      if (A.key[p1 - 1] >= lim) {
-        // the first particle in (p, p1) with key >= lim (keys ascend)
-        uint32_t lo = p + 1, hi = p1 - 1;
-        while (lo < hi) {
-          const uint32_t mid = lo + (hi - lo) / 2;
-          if (A.key[mid] >= lim) {
-            hi = mid;
-          } else {
-            lo = mid + 1;
-          }
+  ColumnParts cp;
+  cp.beg = A.cell_off[col.w];
+  cp.n = A.cell_off[col.w + nz] - cp.beg;
+  cp.nch = (cp.n + A.chunk_targets - 1) / A.chunk_targets;
+  return cp;
+}
+
+// The pieces of part k of a column: emit(i, p0, p1) for its i-th piece;
+// returns their count.
+template <class F>
+__device__ __forceinline__ uint32_t part_pieces(const StageArgs& A, const ColumnParts& cp,
+                                                uint32_t k, F&& emit) {
+  const uint32_t p0 = cp.beg + static_cast<uint32_t>((static_cast<uint64_t>(cp.n) * k) / cp.nch);
+  const uint32_t p1 =
+      cp.beg + static_cast<uint32_t>((static_cast<uint64_t>(cp.n) * (k + 1)) / cp.nch);
+  uint32_t p = p0, count = 0;
+  while (p < p1) {
+    const uint32_t lim = A.key[p] + static_cast<uint32_t>(kChunkRows);
+    uint32_t q = p1;
+    if (A.key[p1 - 1] >= lim) {
+      // the first particle in (p, p1) with key >= lim (keys ascend)
+      uint32_t lo = p + 1, hi = p1 - 1;
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (A.key[mid] >= lim) {
+          hi = mid;
+        } else {
+          lo = mid + 1;
         }
-        q = lo;
       }
-      emit(count, p, q);
-      ++count;
-      p = q;
+      q = lo;
     }
+    emit(count, p, q);
+    ++count;
+    p = q;
   }
   return count;
+}
+
+// Pieces of the column's parts that lane takes (parts lane, lane + 64, ...).
+__device__ __forceinline__ uint32_t lane_pieces(const StageArgs& A, const ColumnParts& cp,
+                                                int lane) {
+  uint32_t cnt = 0;
+  for (uint32_t k = lane; k < cp.nch; k += 64) {
+    cnt += part_pieces(A, cp, k, [](uint32_t, uint32_t, uint32_t) {});
+  }
+  return cnt;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
 }
 
 // Exclusive scan over a 1024-thread block: wave shuffles, then the 16 wave
@@ -443,19 +468,28 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum,
   return r;
 }
 
-// Chunk list in column order, in three passes: chunks per column and their
+// Chunk list in column order, in three passes, one wave per column (its
+// lanes take the column's parts in turn): chunks per column and their
 // per-workgroup totals; an exclusive scan of the totals (one workgroup, also
 // writes the chunk count); each column's chunks at its scanned offset.
+constexpr uint32_t kColsPerBlock = kScanBlock / 64;
+
 __global__ __launch_bounds__(kScanBlock) void k_chunk_count(StageArgs A, uint32_t* block_tot,
                                                            const uint32_t* ctl) {
   if (!ctl[0]) return;
-  __shared__ uint32_t wsum[kScanBlock / 64 + 1];
-  const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
-  uint32_t nch = 0;
-  if (c < A.n_cols) nch = column_chunks(A, c, [](uint32_t, uint32_t, uint32_t) {});
-  uint32_t tot;
-  (void)block_excl_scan(nch, wsum, &tot);
-  if (threadIdx.x == 0) block_tot[blockIdx.x] = tot;
+  __shared__ uint32_t wtot[kColsPerBlock];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t c = blockIdx.x * kColsPerBlock + wave;
+  uint32_t cnt = 0;
+  if (c < A.n_cols) cnt = lane_pieces(A, column_parts(A, c), lane);
+  cnt = wave_sum_u32(cnt);
+  if (lane == 0) wtot[wave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < kColsPerBlock; ++w) t += wtot[w];
+    block_tot[blockIdx.x] = t;
+  }
 }
 
 // Exclusive scan of n_blocks counts in place (one workgroup); out[0] = the
@@ -483,17 +517,40 @@ __global__ __launch_bounds__(kScanBlock) void k_chunk_emit(StageArgs A,
                                                           const uint32_t* block_tot,
                                                           const uint32_t* ctl) {
   if (!ctl[0]) return;
-  __shared__ uint32_t wsum[kScanBlock / 64 + 1];
-  const uint32_t c = blockIdx.x * kScanBlock + threadIdx.x;
-  // the count again (the scan needs every column's before the emission)
-  uint32_t nch = 0;
-  if (c < A.n_cols) nch = column_chunks(A, c, [](uint32_t, uint32_t, uint32_t) {});
-  uint32_t tot;
-  const uint32_t off = block_tot[blockIdx.x] + block_excl_scan(nch, wsum, &tot);
+  __shared__ uint32_t wtot[kColsPerBlock];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t c = blockIdx.x * kColsPerBlock + wave;
+  // the counts again (the offsets need every earlier column's)
+  ColumnParts cp{0u, 0u, 0u};
+  uint32_t cnt = 0;
   if (c < A.n_cols) {
-    (void)column_chunks(A, c, [&](uint32_t k, uint32_t p0, uint32_t p1) {
-      A.chunks[off + k] = make_uint4(c, p0, p1, 0u);
-    });
+    cp = column_parts(A, c);
+    cnt = lane_pieces(A, cp, lane);
+  }
+  cnt = wave_sum_u32(cnt);
+  if (lane == 0) wtot[wave] = cnt;
+  __syncthreads();
+  if (c >= A.n_cols) return;
+  uint32_t off = block_tot[blockIdx.x];
+  for (int w = 0; w < wave; ++w) off += wtot[w];
+  // parts in rounds of 64, each part's pieces at its exclusive prefix
+  for (uint32_t k0 = 0; k0 < cp.nch; k0 += 64) {
+    const uint32_t k = k0 + static_cast<uint32_t>(lane);
+    const uint32_t np =
+        k < cp.nch ? part_pieces(A, cp, k, [](uint32_t, uint32_t, uint32_t) {}) : 0u;
+    uint32_t x = np;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d);
+      if (lane >= d) x += y;
+    }
+    const uint32_t base = off + x - np;
+    if (k < cp.nch) {
+      (void)part_pieces(A, cp, k, [&](uint32_t i, uint32_t p0, uint32_t p1) {
+        A.chunks[base + i] = make_uint4(c, p0, p1, 0u);
+      });
+    }
+    off += __shfl(x, 63);
   }
 }
 
@@ -599,7 +656,7 @@ void stage_grids(const DevParams& P, int n_cu, StageGrids* g) {
 
 void launch_build_chunks(hipStream_t s, const StageArgs& a, uint32_t* block_tot,
                          const uint32_t* ctl) {
-  const uint32_t nb = (a.n_cols + kScanBlock - 1) / kScanBlock;
+  const uint32_t nb = (a.n_cols + kColsPerBlock - 1) / kColsPerBlock;
   if (nb == 0) return;
   k_chunk_count<<<nb, kScanBlock, 0, s>>>(a, block_tot, ctl);
   k_chunk_base<<<1, kScanBlock, 0, s>>>(block_tot, nb, a.counts, ctl);
