@@ -17,8 +17,8 @@
  *
  * Differences from the reference, each forced by the device: the particle
  * state lives on the device between steps (step() uploads it on its first
- * call; sync() copies it back, upload() replaces it); masses that vary
- * between particles need 2D; fluid particles come before fixed ones, as the reference's
+ * call; sync() copies it back, upload() replaces it); fluid particles come
+ * before fixed ones, as the reference's
  * ParticleArray keeps them.  Errors throw Exception, as TIT_ENSURE does
  * (core/exception.hpp:77-85).
  */
@@ -145,9 +145,8 @@ struct FluidEquations {
 };
 
 /// ParticleArray (particle_array.hpp:138-293) in the reference layout:
-/// fluid particles first, then fixed.  Uniform h; per-particle m (varying
-/// masses in 2D only); r, v, rho, p; wall normals of fixed particles when no
-/// wall domain is given.
+/// fluid particles first, then fixed.  Uniform h; per-particle m; r, v, rho,
+/// p; wall normals of fixed particles when no wall domain is given.
 template<std::size_t Dim>
 class ParticleArray {
 public:

@@ -91,7 +91,7 @@ typedef struct tit_hip_host_view {
   size_t n_fixed;
   const double* m;      /* n_fluid + n_fixed particle masses, ParticleArray's m
                            (fluid_equations.hpp:41-48), or NULL: params.m for
-                           every particle.  Positive and finite; 2D only.  With
+                           every particle.  Positive and finite.  With
                            tit_hip_set_walls the fixed particles' masses are
                            multiplied by gamma, as FluidEquations::initialize
                            does (fluid_equations.hpp:87-88).  The learned step

@@ -125,6 +125,8 @@ hipError_t alloc_state(tit_hip_ctx* c, DevState& s, uint32_t n, int dim) {
   if (e == hipSuccess) e = dalloc(c, &s.rho, n);
   if (e == hipSuccess) e = dalloc(c, &s.aux, n);
   if (e == hipSuccess) e = dalloc(c, &s.pv, n);
+  s.vol = nullptr;
+  if (e == hipSuccess && dim == 3) e = dalloc(c, &s.vol, n);
   return e;
 }
 
@@ -881,7 +883,6 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
       }
       any_m = any_m || x != c->prm.m;
     }
-    if (any_m && dim != 2) return fail(c, TIT_E_ARG, "per-particle masses need dim 2");
   }
 
   // Semi-analytic wall: faces, face grid, and the fixed particles' gamma (their

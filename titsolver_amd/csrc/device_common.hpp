@@ -22,12 +22,17 @@ constexpr int kFlavourRef = 0;
 constexpr int kFlavourCfg = 1;
 // REF terms with the semi-analytic wall (2D): per-particle masses (fixed
 // particles carry m gamma), pair sums divided by gamma_a, and the face terms
-// added per particle.  Without walls (gamma = 1, no face terms) this is the
-// REF flavour with per-particle masses.
+// added per particle.  Without walls (gamma = 1, no face terms; 2D or 3D)
+// this is the REF flavour with per-particle masses.
 constexpr int kFlavourWall = 2;
-// CFG terms with per-particle masses (2D): m_b = V_b rho_b from the pair
-// record multiplies the whole term, as in kFlavourWall.
+// CFG terms with per-particle masses: m_b = V_b rho_b (V_b from the pair
+// record in 2D, the volume array in 3D) multiplies the whole term, as in
+// kFlavourWall.
 constexpr int kFlavourCfgM = 3;
+// Flavours whose pair terms take each neighbour's own mass.
+constexpr bool mass_flavour(int flavour) {
+  return flavour == kFlavourWall || flavour == kFlavourCfgM;
+}
 
 // Uniform parameters, passed to every kernel by value.
 struct DevParams {

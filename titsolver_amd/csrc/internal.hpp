@@ -59,6 +59,8 @@ struct DevState {
   float2* aux;  // per particle (p / rho^2, cs) of rho, FP32
   float4* pv;   // per particle (v (FP32, z = 0 in 2D), rho - rho0): what the
                 // pair terms read of a neighbour besides its position
+  float* vol;   // 3D: per particle m / rho (FP32), written with per-particle
+                // masses only (in 2D pv.z holds it); null in 2D
 };
 
 // ---- radix sort (radix_sort.hip) -------------------------------------------

@@ -16,7 +16,8 @@
 // (kFlavourWall) and with per-particle masses (kFlavourWall without walls,
 // kFlavourCfgM) the masses differ per particle (fixed particles carry
 // m gamma, fluid_equations.hpp:87-88): m_b = V_b rho_b comes from the pair
-// record (V_b = m_b / rho_b in its third slot) and multiplies the whole term.
+// record (V_b = m_b / rho_b in its third slot in 2D, in the volume array in
+// 3D) and multiplies the whole term.
 #pragma once
 
 #include "internal.hpp"
@@ -125,13 +126,17 @@ __device__ __forceinline__ float4 make_pv(const DevParams& P, const double* v, d
                      static_cast<float>(rho - P.rho0));
 }
 
-// Writes both per-particle pair records of a particle of mass m.
+// Writes the per-particle pair records of a particle of mass m (in 3D with
+// per-particle masses also its volume m / rho).
 template <int DIM>
 __device__ __forceinline__ void write_pair_data(const DevParams& P, const DevState& st,
                                                 uint32_t i, const double* v, double rho,
                                                 double m) {
   st.aux[i] = make_aux(P, rho);
   st.pv[i] = make_pv<DIM>(P, v, rho, m);
+  if constexpr (DIM == 3) {
+    if (P.mass) st.vol[i] = static_cast<float>(m / rho);
+  }
 }
 
 template <int DIM, int FLAVOUR>
@@ -152,7 +157,7 @@ __device__ __forceinline__ void make_target(const DevParams& P, const double* x,
     a.rb = P.rho0_f + drho;
     a.cv = aux.y;
   }
-  a.pa = (FLAVOUR == kFlavourWall || FLAVOUR == kFlavourCfgM) ? aux.x : P.mg * aux.x;
+  a.pa = mass_flavour(FLAVOUR) ? aux.x : P.mg * aux.x;
 }
 
 // The kernel-gradient polynomial P(q) of grad W_ab = gsc P(q) r_ab as a
@@ -172,11 +177,12 @@ __device__ __forceinline__ float grad_poly_r(const DevParams& P, float r) {
 
 // One pair's continuity and momentum contributions.  d = r_b - r_a in FP64
 // (as in the neighbour predicate); pvb = (v_b, rho_b - rho0) with m / rho_b
-// in the third slot in 2D; Pb = p_b / rho_b^2, csb = cs(rho_b).
+// in the third slot in 2D; Pb = p_b / rho_b^2, csb = cs(rho_b); volb = m_b /
+// rho_b in 3D with per-particle masses (unused otherwise).
 template <int DIM, int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>& a,
                                           const double* d, const float4& pvb, float Pb,
-                                          float csb, float& cont, float* mom) {
+                                          float csb, float volb, float& cont, float* mom) {
   const float vb[3] = {pvb.x, pvb.y, DIM == 3 ? pvb.z : 0.0f};
   float x[DIM];  // r_a - r_b
 #pragma unroll
@@ -191,7 +197,9 @@ __device__ __forceinline__ void pair_term(const DevParams& P, const Target<DIM>&
   float vr = 0.0f;
 #pragma unroll
   for (int k = 0; k < DIM; ++k) vr = __builtin_fmaf(a.v[k] - vb[k], x[k], vr);
-  const float Vb = DIM == 2 ? pvb.z : P.m_f * __builtin_amdgcn_rcpf(P.rho0_f + pvb.w);
+  const float Vb = DIM == 2 ? pvb.z
+                  : (mass_flavour(FLAVOUR) ? volb
+                                           : P.m_f * __builtin_amdgcn_rcpf(P.rho0_f + pvb.w));
   const float drk = __builtin_fmaf(pvb.w, -P.kd, a.kd);  // kd (rho_a - rho_b)
   float c, Pi;
   if constexpr (FLAVOUR == kFlavourCfg) {

@@ -216,6 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_restore_order(
   dst.rho[i] = src.rho[i];
   dst.aux[i] = src.aux[i];
   dst.pv[i] = src.pv[i];
+  if (dst.vol) dst.vol[i] = src.vol[i];
   id_dst[i] = id_src[i];
   tag_dst[i] = tag_src[i];
 }

@@ -89,6 +89,7 @@ struct PairIn {
   double x[DIM];
   float4 pv;
   float2 aux;
+  float vol;  // 3D with per-particle masses: m / rho
 };
 
 template <int DIM, int KERNEL, int FLAVOUR>
@@ -97,7 +98,7 @@ __device__ __forceinline__ void term_of(const DevParams& P, const Target<DIM>& a
   double d[DIM];
 #pragma unroll
   for (int q = 0; q < DIM; ++q) d[q] = p.x[q] - a.x[q];
-  pair_term<DIM, KERNEL, FLAVOUR>(P, a, d, p.pv, p.aux.x, p.aux.y, c, f);
+  pair_term<DIM, KERNEL, FLAVOUR>(P, a, d, p.pv, p.aux.x, p.aux.y, p.vol, c, f);
 }
 
 template <int DIM, int KERNEL, int FLAVOUR>
@@ -161,6 +162,7 @@ __device__ __forceinline__ void sum_words(const DevParams& P, const LdsPos& ps,
     p.x[1] = x.y;
     p.pv = S.pv[j0 + t];
     p.aux = S.aux[j0 + t];
+    p.vol = 0.0f;
   };
   uint32_t rem = n_acc;
   PairIn<2> A, B;
@@ -292,6 +294,7 @@ __device__ __forceinline__ void sum_words_g(const DevParams& P, const GlobalPos<
     for (int k = 0; k < DIM; ++k) p.x[k] = S.x[k][j];
     p.pv = S.pv[j];
     p.aux = S.aux[j];
+    p.vol = (DIM == 3 && mass_flavour(FLAVOUR)) ? S.vol[j] : 0.0f;
   };
   uint32_t rem = n_acc;
   PairIn<DIM> A, B;
@@ -459,6 +462,7 @@ __device__ __forceinline__ void sum_words3(const DevParams& P, const LdsPos3& ps
     const uint32_t j = l + d0;
     p.pv = S.pv[j];
     p.aux = S.aux[j];
+    p.vol = mass_flavour(FLAVOUR) ? S.vol[j] : 0.0f;
   };
   uint32_t rem = n_acc;
   PairIn<3> A, B;
@@ -663,6 +667,7 @@ __device__ __forceinline__ void target_step(const StageArgs& A, const DevTank& T
       A.out.rho[i] = A.in.rho[i];
       A.out.aux[i] = A.in.aux[i];
       A.out.pv[i] = A.in.pv[i];
+      if constexpr (DIM == 3 && mass_flavour(FLAVOUR)) A.out.vol[i] = A.in.vol[i];
     }
   }
   if (fluid) {

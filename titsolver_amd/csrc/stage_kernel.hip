@@ -605,9 +605,9 @@ void dispatch_flavour(hipStream_t s, const StageArgs& a, int kind, bool final) {
   if (a.P.flavour == kFlavourCfg) {
     dispatch_kind<DIM, KERNEL, kFlavourCfg>(s, a, kind, final);
   } else if (a.P.flavour == kFlavourWall) {
-    if constexpr (DIM == 2) dispatch_kind<DIM, KERNEL, kFlavourWall>(s, a, kind, final);
+    dispatch_kind<DIM, KERNEL, kFlavourWall>(s, a, kind, final);
   } else if (a.P.flavour == kFlavourCfgM) {
-    if constexpr (DIM == 2) dispatch_kind<DIM, KERNEL, kFlavourCfgM>(s, a, kind, final);
+    dispatch_kind<DIM, KERNEL, kFlavourCfgM>(s, a, kind, final);
   } else {
     dispatch_kind<DIM, KERNEL, kFlavourRef>(s, a, kind, final);
   }

@@ -1096,6 +1096,8 @@ hipError_t alloc_snapshot(tit_hip_ctx* c, Snapshot& S) {
     TIT_TRY_E(lalloc(m, &S.s.rho, c->n));
     TIT_TRY_E(lalloc(m, &S.s.aux, c->n));
     TIT_TRY_E(lalloc(m, &S.s.pv, c->n));
+    S.s.vol = nullptr;
+    if (c->A.vol) TIT_TRY_E(lalloc(m, &S.s.vol, c->n));
     TIT_TRY_E(lalloc(m, &S.id, c->n));
     TIT_TRY_E(lalloc(m, &S.tag, c->n));
     TIT_TRY_E(lalloc(m, &S.key, c->n));
@@ -1145,6 +1147,7 @@ hipError_t copy_state(tit_hip_ctx* c, Snapshot& S, bool to_snapshot) {
   TIT_TRY_E(cp(c->A.rho, S.s.rho, n));
   TIT_TRY_E(cp(c->A.aux, S.s.aux, n));
   TIT_TRY_E(cp(c->A.pv, S.s.pv, n));
+  if (c->A.vol) TIT_TRY_E(cp(c->A.vol, S.s.vol, n));
   TIT_TRY_E(cp(c->id, S.id, n));
   TIT_TRY_E(cp(c->tag, S.tag, n));
   TIT_TRY_E(cp(c->key, S.key, n));
