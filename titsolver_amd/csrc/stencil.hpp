@@ -1,5 +1,4 @@
-// Per-target trimming of the 5^d cell stencil, shared by the stage kernels
-// and the boundary extrapolation.
+// Per-target trimming of the 5^d cell stencil of the stage kernels.
 #pragma once
 
 #include "internal.hpp"
@@ -10,13 +9,15 @@ namespace tit_amd {
 // skin/2 of that cell's box (it is sorted into it at the last rebuild and
 // has moved less than skin/2 since; only border cells hold particles outside
 // the grid, clamped in).  So for a target whose whole 5^d window is interior
-// (no border cell), a column whose grown box is farther than R from the
-// target in the column plane holds no neighbour, and only the rows within
-// sqrt(R^2 - gap^2) + skin/2 of the target can.  Trimming removes only
-// candidates the exact FP64 predicate rejects, so the accepted set and its
-// order are unchanged; it therefore only has to be conservative, and runs in
-// FP32 with widened bounds: the gaps (FP64, then rounded) shrunk by 1e-6
-// relative, R^2 grown by 1e-5 relative, the row bounds widened by 1e-4 cell.
+// (no border cell), a window cell can hold a neighbour only if its box grown
+// by skin/2 lies within R of the target: gx^2 (+ gy^2) + gz^2 <= R^2 with the
+// per-axis gaps from the target to the grown box.  Along a column those
+// cells are contiguous (the distance to consecutive boxes falls, then
+// rises).  Trimming removes only candidates the exact FP64 predicate
+// rejects, so the accepted set and its order are unchanged; it therefore
+// only has to be conservative, and runs in FP32 with widened bounds: the
+// gaps (FP64, then rounded) shrunk by 1e-6 relative, R^2 grown by 1e-5
+// relative.
 //
 // The row windows of the NCOL stencil columns are packed 6 bits per column,
 // ten columns per 64-bit word (3-bit offsets of the first and last row from
@@ -59,25 +60,26 @@ __device__ __forceinline__ Windows<DIM, NCOL> stencil_windows(const DevParams& P
     return W;
   }
   // gaps from the target to the grown boxes of the five stencil columns of
-  // each column axis
+  // each column axis and of the five rows
   constexpr int NC = 2 * kStencil + 1;
-  float gx[NC];
+  float gx[NC], gz2[NC];
   [[maybe_unused]] float gy[NC];
+  double cw[DIM];
+#pragma unroll
+  for (int ax = 0; ax < DIM; ++ax) cw[ax] = 1.0 / T.inv_c[ax];
   auto gap = [&](int ax, int ic) {
-    const double c = 1.0 / T.inv_c[ax];
-    const double a0 = T.lo[ax] + ic * c - P.skin_half;
-    const double a1 = T.lo[ax] + (ic + 1) * c + P.skin_half;
+    const double a0 = T.lo[ax] + ic * cw[ax] - P.skin_half;
+    const double a1 = T.lo[ax] + (ic + 1) * cw[ax] + P.skin_half;
     return static_cast<float>(fmax(0.0, fmax(a0 - xa[ax], xa[ax] - a1))) * (1.0f - 1e-6f);
   };
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
     gx[q] = gap(0, cx - kStencil + q);
     if constexpr (DIM == 3) gy[q] = gap(1, cy - kStencil + q);
+    const float g = gap(L, zc - kStencil + q);
+    gz2[q] = g * g;
   }
   const float R2 = static_cast<float>(P.R2) * (1.0f + 1e-5f);
-  const float inv_cz = static_cast<float>(T.inv_c[L]);
-  const float zr = static_cast<float>((xa[L] - T.lo[L]) * T.inv_c[L]);  // cell units
-  const float sz = static_cast<float>(P.skin_half * T.inv_c[L]) * (1.0f + 1e-5f) + 1e-4f;
 #pragma unroll
   for (int k = 0; k < NCOL; ++k) {
     float g2;
@@ -86,15 +88,13 @@ __device__ __forceinline__ Windows<DIM, NCOL> stencil_windows(const DevParams& P
     } else {
       g2 = gx[k / NC] * gx[k / NC] + gy[k % NC] * gy[k % NC];
     }
+    uint32_t rows = 0;  // bit q: row zc - 2 + q can hold a neighbour
+#pragma unroll
+    for (int q = 0; q < NC; ++q) rows |= (g2 + gz2[q] <= R2 ? 1u : 0u) << q;
     uint64_t v = 7u | (7u << 3);
-    if (!(g2 > R2)) {
-      const float half = __builtin_sqrtf(R2 - g2) * inv_cz * (1.0f + 1e-5f) + sz;
-      const int w0 = max(zc - kStencil, static_cast<int>(floorf(zr - half)));
-      const int w1 = min(zc + kStencil, static_cast<int>(floorf(zr + half)));
-      if (w0 <= w1) {
-        v = static_cast<uint64_t>(w0 - (zc - kStencil)) |
-            (static_cast<uint64_t>(w1 - (zc - kStencil)) << 3);
-      }
+    if (rows) {
+      v = static_cast<uint64_t>(__builtin_ctz(rows)) |
+          (static_cast<uint64_t>(31 - __builtin_clz(rows)) << 3);
     }
     W.set(k, v);
   }
