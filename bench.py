@@ -275,7 +275,7 @@ def moving_leg(args, eq, arrays):
     ctx.close()
     nf = sum(a.n_fluid for a in arrays)
     names = ["cell_keys", "radix_hist", "radix_digit_scan", "radix_scatter", "radix_copy",
-             "permute", "cell_offsets", "build_chunks"]
+             "permute", "cell_offsets", "fixed_active", "build_chunks"]
     rebuild_ms = sum(st[k][1] for k in names if k in st) / max(args.profile_steps, 1)
     return {"start": f"fluid velocity sqrt(2gH) y/H along x, max {v0:.2f} m/s",
             "steps": args.moving_steps, "ms_per_step": 1e3 * wall / args.moving_steps,
