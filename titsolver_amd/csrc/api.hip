@@ -967,8 +967,8 @@ tit_status tit_hip_upload_batch(tit_hip_ctx* c, const tit_hip_host_view* views,
   TIT_HIP_TRY(c, dalloc(c, &c->tank_need, n_tanks));
   TIT_HIP_TRY(c, dalloc(c, &c->rebuild_count, 1));
   c->sort.max_tiles = radix_sort_tiles(n);
-  TIT_HIP_TRY(c, dalloc(c, &c->sort.hist, 256ull * std::max<uint32_t>(c->sort.max_tiles, 1)));
-  TIT_HIP_TRY(c, dalloc(c, &c->sort.tot, 256));
+  TIT_HIP_TRY(c, dalloc(c, &c->sort.hist, 2048ull * std::max<uint32_t>(c->sort.max_tiles, 1)));
+  TIT_HIP_TRY(c, dalloc(c, &c->sort.tot, 2048));
   TIT_HIP_TRY(c, dalloc(c, &c->tanks, n_tanks));
   TIT_HIP_TRY(c, dalloc(c, &c->normal, static_cast<size_t>(n) * dim));
   TIT_HIP_TRY(c, dalloc(c, &c->dt, n_tanks));

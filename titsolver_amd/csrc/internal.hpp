@@ -65,8 +65,8 @@ struct DevState {
 
 // ---- radix sort (radix_sort.hip) -------------------------------------------
 struct SortScratch {
-  uint32_t* hist = nullptr;  // 256 * max_tiles
-  uint32_t* tot = nullptr;   // 256
+  uint32_t* hist = nullptr;  // 2048 * max_tiles (digits of up to 11 bits)
+  uint32_t* tot = nullptr;   // 2048
   uint32_t max_tiles = 0;
 };
 uint32_t radix_sort_tiles(uint32_t n);
