@@ -293,6 +293,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_gaps(uint32_t* __restrict__ off
 // two sums are reduced with shuffles inside the group.
 
 constexpr int kBoundaryLanes = 16;
+// candidates whose loads a lane of k_boundary has in flight at once
+constexpr int kBoundaryUnroll = 2;
 // fixed particles per workgroup of k_fixed_active (its LDS list)
 constexpr uint32_t kFixedPerBlock = 1024;
 
@@ -413,7 +415,10 @@ __global__ __launch_bounds__(kBlock) void k_fixed_active(BoundaryArgs A, const u
 }
 
 // Extrapolation of the listed fixed particles, one group of G lanes each.
-template <int DIM, int KERNEL>
+// XI_INT: the Tait exponent is a small integer (P.xi_int), so H(rho) takes
+// integer powers; the general FP64 pow otherwise (it alone needs about 55
+// more VGPRs, which would cost a wave per SIMD).
+template <int DIM, int KERNEL, bool XI_INT>
 __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   constexpr int NCOL = DIM == 2 ? 2 * kStencil + 1 : (2 * kStencil + 1) * (2 * kStencil + 1);
   constexpr int G = kBoundaryLanes;
@@ -504,35 +509,71 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
   }
   double S = 0.0, H = 0.0;
   const double xi_1 = P.xi - 1.0, inv_xi_1 = 1.0 / xi_1, cs2 = P.cs0 * P.cs0;
-  auto visit = [&](uint32_t j) {
-    if (A.tag[j] & 1u) return;  // fluid neighbours only (self is fixed)
-    double dd[DIM];
+  // U candidates per lane at a time, in three rounds of loads that each have
+  // all U candidates in flight: the tags; the positions of the fluid ones;
+  // ρ (and the id, for per-particle masses) of those within the radius.  The
+  // terms are then added in candidate order.
+  constexpr int U = kBoundaryUnroll;
+  auto batch = [&](const uint32_t (&j)[U], const bool (&live)[U]) {
+    bool ok[U];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) dd[d] = A.s.x[d][j] - xe[d];
-    double s2 = dd[0] * dd[0];
+    for (int u = 0; u < U; ++u) ok[u] = live[u] && !(A.tag[j[u]] & 1u);  // fluid only
+    double dd[U][DIM], s2[U];
 #pragma unroll
-    for (int d = 1; d < DIM; ++d) s2 = s2 + dd[d] * dd[d];
-    if (!(s2 <= P.R2)) return;
-    const double rho_b = A.s.rho[j];
-    const double V_b = (P.mass ? P.mass[A.id[j]] : P.m) / rho_b;
-    const double W = w0 * kernel_value_poly_f64<KERNEL>(h_inv * sqrt(s2));
-    double rn = dd[0] * ne[0];
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
 #pragma unroll
-    for (int d = 1; d < DIM; ++d) rn = rn + dd[d] * ne[d];
-    // H(rho_b) = cs0^2 ((rho_b/rho0)^(xi-1) - 1) / (xi - 1)
-    const double ratio = rho_b * P.inv_rho0;
-    const double rp = P.xi_int ? ipow_f64(ratio, P.xi_int - 1) : pow(ratio, xi_1);
-    const double H_b = cs2 * (rp - 1.0) * inv_xi_1;
-    S += V_b * W;
-    H += V_b * (H_b + P.g * rn * ne[1]) * W;
+      for (int d = 0; d < DIM; ++d) dd[u][d] = A.s.x[d][j[u]] - xe[d];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      s2[u] = dd[u][0] * dd[u][0];
+#pragma unroll
+      for (int d = 1; d < DIM; ++d) s2[u] = s2[u] + dd[u][d] * dd[u][d];
+      ok[u] = s2[u] <= P.R2;
+    }
+    double rho[U], mb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      rho[u] = A.s.rho[j[u]];
+      mb[u] = P.mass ? P.mass[A.id[j[u]]] : P.m;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      const double rho_b = rho[u];
+      const double V_b = mb[u] / rho_b;
+      const double W = w0 * kernel_value_poly_f64<KERNEL>(h_inv * sqrt(s2[u]));
+      double rn = dd[u][0] * ne[0];
+#pragma unroll
+      for (int d = 1; d < DIM; ++d) rn = rn + dd[u][d] * ne[d];
+      // H(rho_b) = cs0^2 ((rho_b/rho0)^(xi-1) - 1) / (xi - 1)
+      const double ratio = rho_b * P.inv_rho0;
+      const double rp = XI_INT ? ipow_f64(ratio, P.xi_int - 1) : pow(ratio, xi_1);
+      const double H_b = cs2 * (rp - 1.0) * inv_xi_1;
+      S += V_b * W;
+      H += V_b * (H_b + P.g * rn * ne[1]) * W;
+    }
   };
   if constexpr (DIM == 2) {
     // the lanes take every G-th candidate of the five column ranges in turn
     const uint32_t n_cand = rl[NCOL];
-    for (uint32_t t = lane; t < n_cand; t += G) {
+    auto cand = [&](uint32_t t) {
       int q = 0;
       while (t >= rl[q + 1]) ++q;
-      visit(rb[q] + (t - rl[q]));
+      return rb[q] + (t - rl[q]);
+    };
+    for (uint32_t t = lane; t < n_cand; t += U * G) {
+      uint32_t j[U];
+      bool live[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        live[u] = t + u * G < n_cand;
+        j[u] = live[u] ? cand(t + u * G) : 0u;
+      }
+      batch(j, live);
     }
   } else {
     // The 25 trimmed column ranges, laid end to end: each lane loads two
@@ -583,9 +624,19 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint32_t n_cand = tot0 + tot1;
     int q = 0;
-    for (uint32_t t = lane; t < n_cand; t += G) {
+    auto cand = [&](uint32_t t) {  // candidates come in increasing t
       while (t >= s_rl[grp][q + 1]) ++q;
-      visit(s_rb[grp][q] + (t - s_rl[grp][q]));
+      return s_rb[grp][q] + (t - s_rl[grp][q]);
+    };
+    for (uint32_t t = lane; t < n_cand; t += U * G) {
+      uint32_t j[U];
+      bool live[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        live[u] = t + u * G < n_cand;
+        j[u] = live[u] ? cand(t + u * G) : 0u;
+      }
+      batch(j, live);
     }
   }
 #pragma unroll
@@ -850,17 +901,22 @@ void launch_fixed_active(hipStream_t s, const BoundaryArgs& a, const uint32_t* c
 void launch_boundary(hipStream_t s, const BoundaryArgs& a) {
   if (a.n_fixed == 0) return;
   const uint32_t g = grid_for(a.n_fixed, kBlock / kBoundaryLanes);
+  const bool c4 = a.P.kernel == kKernelC4, xi_int = a.P.xi_int != 0;
   if (a.P.dim == 2) {
-    if (a.P.kernel == kKernelC4) {
-      k_boundary<2, kKernelC4><<<g, kBlock, 0, s>>>(a);
+    if (c4) {
+      if (xi_int) k_boundary<2, kKernelC4, true><<<g, kBlock, 0, s>>>(a);
+      else k_boundary<2, kKernelC4, false><<<g, kBlock, 0, s>>>(a);
     } else {
-      k_boundary<2, kKernelC2><<<g, kBlock, 0, s>>>(a);
+      if (xi_int) k_boundary<2, kKernelC2, true><<<g, kBlock, 0, s>>>(a);
+      else k_boundary<2, kKernelC2, false><<<g, kBlock, 0, s>>>(a);
     }
   } else {
-    if (a.P.kernel == kKernelC4) {
-      k_boundary<3, kKernelC4><<<g, kBlock, 0, s>>>(a);
+    if (c4) {
+      if (xi_int) k_boundary<3, kKernelC4, true><<<g, kBlock, 0, s>>>(a);
+      else k_boundary<3, kKernelC4, false><<<g, kBlock, 0, s>>>(a);
     } else {
-      k_boundary<3, kKernelC2><<<g, kBlock, 0, s>>>(a);
+      if (xi_int) k_boundary<3, kKernelC2, true><<<g, kBlock, 0, s>>>(a);
+      else k_boundary<3, kKernelC2, false><<<g, kBlock, 0, s>>>(a);
     }
   }
 }
