@@ -122,6 +122,18 @@ __device__ __forceinline__ double kernel_value_poly_f64(double q) {
 }
 
 // ---- Tait equation of state (source/tit/sph/equation_of_state.hpp:19-72) ---
+
+// General powers, kept out of line.  Inlined, the FP64 pow alone takes about
+// 55 VGPRs, which every kernel evaluating the equation of state would pay for
+// in waves per SIMD even where the Tait exponent is a small integer and the
+// branch never runs.
+[[maybe_unused]] static __device__ __noinline__ double pow_general(double x, double y) {
+  return pow(x, y);
+}
+[[maybe_unused]] static __device__ __noinline__ float powf_general(float x, float y) {
+  return __builtin_powf(x, y);
+}
+
 __device__ __forceinline__ double ipow_f64(double x, int n) {
   double r = 1.0, b = x;
   while (n > 0) {
@@ -135,7 +147,7 @@ __device__ __forceinline__ double ipow_f64(double x, int n) {
 // (rho/rho0)^xi
 __device__ __forceinline__ double eos_ratio_pow_xi(const DevParams& P,
                                                    double ratio) {
-  return P.xi_int ? ipow_f64(ratio, P.xi_int) : pow(ratio, P.xi);
+  return P.xi_int ? ipow_f64(ratio, P.xi_int) : pow_general(ratio, P.xi);
 }
 
 // p(rho) = B ((rho/rho0)^xi - 1)
@@ -152,7 +164,7 @@ __device__ __forceinline__ double eos_sound_speed(const DevParams& P,
     const double rk = ipow_f64(ratio, k >> 1);
     return P.cs0 * ((k & 1) ? rk * sqrt(ratio) : rk);
   }
-  return P.cs0 * pow(ratio, (P.xi - 1.0) / 2.0);
+  return P.cs0 * pow_general(ratio, (P.xi - 1.0) / 2.0);
 }
 
 // H(rho) = cs0^2 ((rho/rho0)^(xi-1) - 1) / (xi - 1)
@@ -160,7 +172,7 @@ __device__ __forceinline__ double eos_potential(const DevParams& P,
                                                 double rho) {
   const double xi_1 = P.xi - 1.0;
   const double ratio = rho / P.rho0;
-  const double rp = P.xi_int ? ipow_f64(ratio, P.xi_int - 1) : pow(ratio, xi_1);
+  const double rp = P.xi_int ? ipow_f64(ratio, P.xi_int - 1) : pow_general(ratio, xi_1);
   return P.cs0 * P.cs0 * (rp - 1.0) / xi_1;
 }
 
@@ -168,7 +180,7 @@ __device__ __forceinline__ double eos_potential(const DevParams& P,
 __device__ __forceinline__ double eos_density_from_potential(const DevParams& P,
                                                              double H) {
   const double xi_1 = P.xi - 1.0;
-  return P.rho0 * pow(1.0 + xi_1 * H / (P.cs0 * P.cs0), 1.0 / xi_1);
+  return P.rho0 * pow_general(1.0 + xi_1 * H / (P.cs0 * P.cs0), 1.0 / xi_1);
 }
 
 // FluidEquations::compute_time_step for one fluid particle

@@ -36,7 +36,7 @@ __device__ __forceinline__ float sound_speed_f32(const DevParams& P, float rho) 
     }
     return P.cs0_f * ((k & 1) ? r * __builtin_sqrtf(ratio) : r);
   }
-  return P.cs0_f * __builtin_powf(ratio, 0.5f * (static_cast<float>(P.xi) - 1.0f));
+  return P.cs0_f * powf_general(ratio, 0.5f * (static_cast<float>(P.xi) - 1.0f));
 }
 
 // Per-target values of the pair terms, with the uniform factors folded in

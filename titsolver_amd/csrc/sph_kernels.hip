@@ -551,7 +551,7 @@ __global__ __launch_bounds__(kBlock) void k_boundary(BoundaryArgs A) {
       for (int d = 1; d < DIM; ++d) rn = rn + dd[u][d] * ne[d];
       // H(rho_b) = cs0^2 ((rho_b/rho0)^(xi-1) - 1) / (xi - 1)
       const double ratio = rho_b * P.inv_rho0;
-      const double rp = XI_INT ? ipow_f64(ratio, P.xi_int - 1) : pow(ratio, xi_1);
+      const double rp = XI_INT ? ipow_f64(ratio, P.xi_int - 1) : pow_general(ratio, xi_1);
       const double H_b = cs2 * (rp - 1.0) * inv_xi_1;
       S += V_b * W;
       H += V_b * (H_b + P.g * rn * ne[1]) * W;

@@ -30,8 +30,6 @@ struct LdsPos {
   const uint32_t* lb;                    // LDS index of each column's row zlo
   int zlo, nz, ix, ncx;
   uint32_t gbase;   // global index of the chunk's first candidate
-  uint64_t* wmask;  // this wave's [kWords][64] acceptance masks
-  uint32_t* wjl;    // their bit 0: global index - gbase (bits 10..31), LDS index
 
   // Candidates of column k in rows [w0, w1] (inside the target's 5-row window).
   __device__ __forceinline__ bool range(int k, int w0, int w1, uint32_t& b,
@@ -133,28 +131,49 @@ __device__ __forceinline__ uint32_t pop_lowest(uint64_t& mask) {
 // accepted candidates' terms enter the compensated sums two at a time.
 //
 // 2D with LDS positions.  The acceptance masks of up to kWords candidate
-// batches (normally one per stencil column) are kept per lane in LDS and
-// their accepted candidates are summed in one loop, so a wave's pair loop
+// batches (normally one per stencil column) are queued per lane in registers
+// and their accepted candidates are summed in one loop, so a wave's pair loop
 // runs for the largest accepted count among its lanes, not for the sum over
 // columns of each column's largest count.
+struct WordQueue {
+  uint64_t m[kWords];   // acceptance masks, oldest first (never empty)
+  uint32_t jl[kWords];  // their bit 0: global index - gbase (bits 10..31), LDS index
+
+  // Appends at slot n (a run-time index: one select per slot).
+  __device__ __forceinline__ void put(int n, uint64_t mask, uint32_t v) {
+#pragma unroll
+    for (int q = 0; q < kWords; ++q) {
+      if (q == n) {
+        m[q] = mask;
+        jl[q] = v;
+      }
+    }
+  }
+  // Removes the oldest word.
+  __device__ __forceinline__ void pop(uint64_t& mask, uint32_t& v) {
+    mask = m[0];
+    v = jl[0];
+#pragma unroll
+    for (int q = 0; q + 1 < kWords; ++q) {
+      m[q] = m[q + 1];
+      jl[q] = jl[q + 1];
+    }
+  }
+};
+
 template <int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void sum_words(const DevParams& P, const LdsPos& ps,
                                           const DevState& S, const Target<2>& a,
-                                          uint32_t n_acc, Sums<2>& acc) {
-  const int lane = threadIdx.x & 63;
-  int w = 0;
-  uint64_t m = 0, m_next = ps.wmask[lane];  // the next word is read one word ahead
-  uint32_t l0 = 0, j0 = 0, jl_next = ps.wjl[lane];
-  // The next of the n_acc accepted candidates (stored words are never
-  // empty; slots past the last stored word are read but never used).
+                                          uint32_t n_acc, Sums<2>& acc, WordQueue& wq) {
+  uint64_t m = 0;
+  uint32_t l0 = 0, j0 = 0;
+  // The next of the n_acc accepted candidates.
   auto take = [&](PairIn<2>& p) {
     if (m == 0) {
-      m = m_next;
-      j0 = ps.gbase + (jl_next >> 10);
-      l0 = jl_next & 1023u;
-      w = min(w + 1, kWords - 1);
-      m_next = ps.wmask[w * 64 + lane];
-      jl_next = ps.wjl[w * 64 + lane];
+      uint32_t jl;
+      wq.pop(m, jl);
+      j0 = ps.gbase + (jl >> 10);
+      l0 = jl & 1023u;
     }
     const uint32_t t = pop_lowest(m);
     const double2 x = ps.xy[l0 + t];
@@ -206,7 +225,7 @@ __device__ __forceinline__ void gather_words(const DevParams& P, const LdsPos& p
                                              const DevState& S, const Target<2>& a,
                                              uint32_t i, int zc, const Windows<2, NCOL>& W,
                                              Sums<2>& acc, uint32_t& count) {
-  const int lane = threadIdx.x & 63;
+  WordQueue wq;
   int k = 0, kc = 0;
   uint32_t c0 = 0, e = 0;
   bool more = true;
@@ -259,13 +278,12 @@ __device__ __forceinline__ void gather_words(const DevParams& P, const LdsPos& p
       count += na;
       n_acc += na;
       if (mask) {
-        ps.wmask[nw * 64 + lane] = mask;
-        ps.wjl[nw * 64 + lane] = ((c0 - ps.gbase) << 10) | lc;
+        wq.put(nw, mask, ((c0 - ps.gbase) << 10) | lc);
         ++nw;
       }
       c0 += len;
     }
-    sum_words<KERNEL, FLAVOUR>(P, ps, S, a, n_acc, acc);
+    sum_words<KERNEL, FLAVOUR>(P, ps, S, a, n_acc, acc, wq);
   }
 }
 

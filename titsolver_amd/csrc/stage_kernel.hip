@@ -11,8 +11,8 @@
 // 2D: the FP64 positions of the chunk's candidates (the 5 neighbouring
 // columns, the chunk's rows +- 2) are staged in LDS.  Each lane walks its own
 // 5 x 5 cell stencil one column range at a time: the exact FP64 predicate
-// (four LDS loads per batch) builds a 64-bit acceptance mask per batch, kept
-// in LDS; then the FP32 pair terms run over the accepted candidates of all
+// (four LDS loads per batch) builds a 64-bit acceptance mask per batch, queued
+// in registers; then the FP32 pair terms run over the accepted candidates of all
 // batches in one loop, reading their pair records (velocity, m/rho,
 // rho - rho0 and p/rho^2, cs) from L2 one candidate ahead of the arithmetic.
 // Chunks whose candidates exceed the LDS capacity, and 3D, take positions
@@ -25,15 +25,14 @@ namespace tit_amd {
 namespace {
 
 // 2D chunks with LDS-staged positions; chunks that do not fit go to the
-// overflow list.
+// overflow list.  18 KB of LDS and at most 128 VGPRs: four waves per SIMD.
 template <int KERNEL, int FLAVOUR, int KIND, bool FINAL, int NT, int CAP>
-__global__ __launch_bounds__(NT) void k_stage_lds(StageArgs A) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_stage_lds(
+    StageArgs A) {
   __shared__ double2 s_xy[CAP];
   __shared__ uint32_t s_coff[kCols2][kRowsMax + 1];
   __shared__ uint32_t s_gb[kCols2], s_lb[kCols2 + 1];
   __shared__ int s_fits;
-  __shared__ uint64_t s_wmask[NT / 64][kWords * 64];
-  __shared__ uint32_t s_wjl[NT / 64][kWords * 64];
   __shared__ uint32_t s_gbase;
   const int tid = threadIdx.x;
   const Band band = xcd_band(A.counts[0]);
@@ -86,8 +85,7 @@ __global__ __launch_bounds__(NT) void k_stage_lds(StageArgs A) {
       s_xy[l] = make_double2(A.in.x[0][j], A.in.x[1][j]);
     }
     __syncthreads();
-    const LdsPos ps{s_xy,   s_coff,   s_gb,      s_lb,           zlo,           c.nz,
-                    c.cx,   c.T.nc[0], s_gbase,  s_wmask[tid >> 6], s_wjl[tid >> 6]};
+    const LdsPos ps{s_xy, s_coff, s_gb, s_lb, zlo, c.nz, c.cx, c.T.nc[0], s_gbase};
     chunk_targets<2, KERNEL, FLAVOUR, KIND, FINAL>(A, c, ps, red);
   }
   red.flush(A);
