@@ -25,15 +25,15 @@ constexpr int kRowsMax = kChunkRows + 2 * kStencil;
 // kStage3LdsCap candidates staged in LDS.
 constexpr int kBlk3 = 4;
 constexpr int kWin3 = kBlk3 + 2 * kStencil;
-constexpr int kChunkRows3 = 8;
+constexpr int kChunkRows3 = 6;
 constexpr int kRows3 = kChunkRows3 + 2 * kStencil;
 constexpr int kWords3 = 6;
 constexpr int kStage3Threads = 256;
 // (a chunk of 256 targets in the 3D dam break spans 5-6 rows of 16 columns,
-// so its window holds about 64 x 9.5 x 2.9 = 1,800 candidates; 2,304 leaves
-// room for 30 % denser regions at 76 KB of LDS per workgroup, which the
-// kernel's VGPR use limits to two workgroups per CU anyway)
-constexpr int kStage3LdsCap = 2304;
+// so its window holds about 64 x 9.5 x 2.9 = 1,800 candidates, and one of 6
+// rows at most 64 x 10 x 2.9 = 1,860; 2,048 candidates keep the workgroup's
+// LDS under 53 KB: three workgroups of four waves per CU)
+constexpr int kStage3LdsCap = 2048;
 
 struct StageArgs {
   DevParams P;

@@ -433,9 +433,6 @@ struct LdsPos3 {
   const uint32_t* lb;
   const uint32_t* delta;
   int zlo, nz, wx0, wy0, ncx, ncy;
-  // this wave's [kWords3][64] words: 32-bit acceptance mask | LDS index of
-  // its bit 0 << 32 | window column << 48
-  uint64_t* wword;
 
   // LDS range of stencil column k of target column (cx, cy), rows [w0, w1]
   // (inside the target's 5-row window, so inside the chunk's window).
@@ -453,24 +450,40 @@ struct LdsPos3 {
   }
 };
 
-// The pair loop of gather_words3's stored words (as sum_words_g).
+// gather_words3's words, queued per lane in registers: 32-bit acceptance mask
+// | LDS index of its bit 0 << 32 | window column << 48 (never an empty mask).
+struct WordQueue3 {
+  uint64_t w[kWords3];  // oldest first
+
+  // Appends at slot n (a run-time index: one select per slot).
+  __device__ __forceinline__ void put(int n, uint64_t v) {
+#pragma unroll
+    for (int q = 0; q < kWords3; ++q) {
+      if (q == n) w[q] = v;
+    }
+  }
+  // Removes the oldest word.
+  __device__ __forceinline__ uint64_t pop() {
+    const uint64_t v = w[0];
+#pragma unroll
+    for (int q = 0; q + 1 < kWords3; ++q) w[q] = w[q + 1];
+    return v;
+  }
+};
+
+// The pair loop of gather_words3's queued words (as sum_words_g).
 template <int KERNEL, int FLAVOUR>
 __device__ __forceinline__ void sum_words3(const DevParams& P, const LdsPos3& ps,
                                            const DevState& S, const Target<3>& a,
-                                           uint32_t n_acc, Sums<3>& acc) {
-  const int lane = threadIdx.x & 63;
-  int w = 0;
-  uint64_t next = ps.wword[lane];
+                                           uint32_t n_acc, Sums<3>& acc, WordQueue3& wq) {
   uint32_t m = 0, l0 = 0, d0 = 0;
-  // the next of the n_acc accepted candidates (stored masks are never empty;
-  // slots past the last stored word are read but never used)
+  // the next of the n_acc accepted candidates
   auto take = [&](PairIn<3>& p) {
     if (m == 0) {
+      const uint64_t next = wq.pop();
       m = static_cast<uint32_t>(next);
       l0 = static_cast<uint32_t>(next >> 32) & 0xFFFFu;
       d0 = ps.delta[static_cast<uint32_t>(next >> 48)];
-      w = min(w + 1, kWords3 - 1);
-      next = ps.wword[w * 64 + lane];
     }
     const uint32_t l = l0 + static_cast<uint32_t>(__builtin_ctz(m));
     m &= m - 1u;
@@ -526,7 +539,7 @@ __device__ __forceinline__ void gather_words3(const DevParams& P, const LdsPos3&
                                               const Windows<3, kCols2 * kCols2>& W,
                                               Sums<3>& acc, uint32_t& count) {
   constexpr int NCOL = kCols2 * kCols2;
-  const int lane = threadIdx.x & 63;
+  WordQueue3 wq;
   int k = 0, kw = 0;
   uint32_t c0 = 0, e = 0;
   bool more = true;
@@ -579,14 +592,13 @@ __device__ __forceinline__ void gather_words3(const DevParams& P, const LdsPos3&
       count += na;
       n_acc += na;
       if (mask) {
-        ps.wword[nw * 64 + lane] = static_cast<uint64_t>(mask) |
-                                   (static_cast<uint64_t>(c0) << 32) |
-                                   (static_cast<uint64_t>(kw) << 48);
+        wq.put(nw, static_cast<uint64_t>(mask) | (static_cast<uint64_t>(c0) << 32) |
+                       (static_cast<uint64_t>(kw) << 48));
         ++nw;
       }
       c0 += len;
     }
-    sum_words3<KERNEL, FLAVOUR>(P, ps, S, a, n_acc, acc);
+    sum_words3<KERNEL, FLAVOUR>(P, ps, S, a, n_acc, acc, wq);
   }
 }
 

@@ -163,7 +163,6 @@ __global__ __launch_bounds__(NT) void k_stage_lds3(StageArgs A) {
   __shared__ uint32_t s_lb[NW + 1], s_delta[NW];
   __shared__ uint32_t s_tb[NB + 1], s_first[NB];
   __shared__ int s_fits;
-  __shared__ uint64_t s_word[NT / 64][kWords3 * 64];
   const int tid = threadIdx.x;
   const Band band = xcd_band(A.counts[2]);
   WaveReduce red;
@@ -235,7 +234,7 @@ __global__ __launch_bounds__(NT) void k_stage_lds3(StageArgs A) {
     }
     __syncthreads();
     const LdsPos3 ps{s_x, s_y, s_z, s_coff, s_lb, s_delta, zlo, nz, wx0, wy0,
-                     c.T.nc[0], c.T.nc[1], s_word[tid >> 6]};
+                     c.T.nc[0], c.T.nc[1]};
     const uint32_t t = static_cast<uint32_t>(tid);
     const bool in_range = t < s_tb[NB];
     uint32_t i = 0;
